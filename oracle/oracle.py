@@ -1,0 +1,410 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so) and, where it was
+built, for the reference kernels (oracle/_ref/libenkfref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, as the checker.  The product library never
+imports it.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libenkfref.so")
+
+MODE_ENKF, MODE_ENOI, MODE_HYBRID = 1, 2, 3
+SCHEME_DENKF, SCHEME_ETKF = 1, 2
+NLOBSMAX_NONE = 2147483647
+MAXLOCRAD = 8
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_sp = C.POINTER(C.c_short)
+_up = C.POINTER(C.c_ubyte)
+
+
+class ObsType(C.Structure):
+    _fields_ = [("nlocrad", C.c_int), ("locrad", C.c_double * MAXLOCRAD),
+                ("locweight", C.c_double * MAXLOCRAD), ("rfactor", C.c_double),
+                ("geographic", C.c_int), ("statsonly", C.c_int), ("nlobsmax", C.c_int)]
+
+
+class KdTree(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("n", C.c_int), ("coords", _dp), ("left", _ip),
+                ("right", _ip), ("parent", _ip), ("dir", _up), ("data", _ip), ("depth", C.c_int)]
+
+
+class Obs(C.Structure):
+    _fields_ = [("nobs", C.c_int), ("ntypes", C.c_int), ("type", _sp), ("status", _sp),
+                ("lon", _fp), ("lat", _fp), ("estd", _fp), ("types", C.POINTER(ObsType)),
+                ("ntobs", _ip), ("obsids", C.POINTER(_ip)), ("trees", C.POINTER(KdTree))]
+
+
+class CalcSpec(C.Structure):
+    _fields_ = [("mode", C.c_int), ("scheme", C.c_int), ("nmem", C.c_int),
+                ("nmem_dynamic", C.c_int), ("alpha", C.c_double)]
+
+
+class CalcStats(C.Structure):
+    _fields_ = [("nlobs_sum", C.c_longlong), ("nlobs_max", C.c_int), ("n_inv_obs", C.c_int),
+                ("n_inv_ens", C.c_int), ("ncell", C.c_int)]
+
+
+class Field(C.Structure):
+    _fields_ = [("level", C.c_int), ("issurfacevar", C.c_int), ("inflation", C.c_float),
+                ("inf_ratio", C.c_double), ("applylog", C.c_int)]
+
+
+class UpdateSpec(C.Structure):
+    _fields_ = [("mode", C.c_int), ("nmem", C.c_int), ("nmem_dynamic", C.c_int),
+                ("mni", C.c_int), ("mnj", C.c_int), ("stride", C.c_int), ("periodic_i", C.c_int),
+                ("surfk", C.c_int), ("nlevels", _ip), ("outputinc", C.c_int),
+                ("ni_s", C.c_int), ("nj_s", C.c_int), ("T", _fp), ("w", _fp)]
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.or_taper_gc.restype = C.c_double
+        L.or_taper_gc.argtypes = [C.c_double]
+        L.or_calclcoeff.restype = C.c_double
+        L.or_calclcoeff.argtypes = [C.c_int, _dp, _dp, C.c_double]
+        L.or_getmaxlocrad.restype = C.c_double
+        L.or_ll2xyz.argtypes = [_dp, _dp]
+        L.or_kd_build.argtypes = [C.POINTER(KdTree), C.c_int, C.c_int, _dp]
+        L.or_kd_free.argtypes = [C.POINTER(KdTree)]
+        L.or_kd_range.argtypes = [C.POINTER(KdTree), _dp, C.c_double, C.c_int, C.c_int, _ip, _dp]
+        L.or_obs_build_trees.argtypes = [C.POINTER(Obs), _sp]
+        L.or_obs_free_trees.argtypes = [C.POINTER(Obs)]
+        L.or_findlocal.argtypes = [C.POINTER(Obs), C.c_double, C.c_double, C.c_int, _up, C.c_int, _ip, _dp]
+        L.or_standardise.argtypes = [C.c_int, C.c_int, _sp, _fp, C.POINTER(ObsType), _fp, _dp]
+        L.or_traceprod.restype = C.c_double
+        L.or_calc_G.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.or_calc_w.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        for fn in (L.or_calc_wT_denkf, L.or_calc_wT_etkf):
+            fn.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.or_calc_transforms.argtypes = [C.POINTER(CalcSpec), C.POINTER(Obs), _fp, _dp, C.c_int, _dp, _dp,
+                                         C.c_int, _up, _fp, _fp, _dp, _dp, _ip, _fp, _fp,
+                                         C.POINTER(CalcStats), C.c_int, C.c_int]
+        L.or_update_fields.argtypes = [C.POINTER(UpdateSpec), C.c_int, C.POINTER(Field), C.c_int, C.c_int,
+                                       C.POINTER(_fp), _fp]
+        L.or_update_bg.argtypes = [C.POINTER(UpdateSpec), C.c_int, C.POINTER(Field), C.c_int, C.c_int,
+                                   C.POINTER(_fp)]
+        L.or_sethybridensemble.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _fp, _dp]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference kernels (calc/calcs.c + common/kdtree.c, unmodified)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref not built (needs the reference tree and MKL): make -C oracle ref")
+        os.environ.setdefault("MKL_NUM_THREADS", "1")
+        R = C.CDLL(REF_PATH)
+        R.ref_local_analysis.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                                         _dp, _dp, _dp]
+        R.ref_kd_build.restype = C.c_void_p
+        R.ref_kd_build.argtypes = [C.c_int, C.c_int, _dp]
+        R.ref_kd_destroy.argtypes = [C.c_void_p]
+        R.ref_kd_size.argtypes = [C.c_void_p]
+        R.ref_kd_range.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp]
+        R.ref_calc_transforms.argtypes = [C.POINTER(CalcSpec), C.POINTER(Obs), _sp, _fp, _dp, C.c_int, _dp,
+                                          _dp, C.c_int, _up, _fp, _fp, _dp, _dp, _ip, _fp, _fp,
+                                          C.POINTER(CalcStats), C.c_int, C.c_int]
+        _ref = R
+    return _ref
+
+
+# ---------------------------------------------------------------- geometry
+def ll2xyz(lon, lat):
+    out = (C.c_double * 3)()
+    ll = (C.c_double * 2)(lon, lat)
+    lib().or_ll2xyz(ll, out)
+    return np.array(out[:])
+
+
+def taper_gc(x):
+    return lib().or_taper_gc(float(x))
+
+
+def calclcoeff(locrad, locweight, dist):
+    r = np.ascontiguousarray(locrad, np.float64)
+    w = np.ascontiguousarray(locweight, np.float64)
+    return lib().or_calclcoeff(len(r), _ptr(r, _dp), _ptr(w, _dp), float(dist))
+
+
+# ---------------------------------------------------------------- kd-tree
+class OracleKd:
+    def __init__(self, coords):
+        coords = np.ascontiguousarray(coords, np.float64)
+        self.ndim = coords.shape[1]
+        self.t = KdTree()
+        r = lib().or_kd_build(C.byref(self.t), self.ndim, coords.shape[0], _ptr(coords, _dp))
+        if r != 0:
+            raise RuntimeError("or_kd_build: %d" % r)
+
+    def range(self, q, rng, ordered=False):
+        n = max(self.t.n, 1)
+        data = np.zeros(n, np.int32)
+        d2 = np.zeros(n, np.float64)
+        q = np.ascontiguousarray(q, np.float64)
+        k = lib().or_kd_range(C.byref(self.t), _ptr(q, _dp), float(rng), int(ordered), n,
+                              _ptr(data, _ip), _ptr(d2, _dp))
+        if k < 0:
+            raise RuntimeError("or_kd_range: %d" % k)
+        return data[:k].copy(), d2[:k].copy()
+
+    def arrays(self):
+        n = self.t.n
+        return dict(left=np.ctypeslib.as_array(self.t.left, (n,)).copy(),
+                    right=np.ctypeslib.as_array(self.t.right, (n,)).copy(),
+                    data=np.ctypeslib.as_array(self.t.data, (n,)).copy(),
+                    depth=self.t.depth)
+
+    def __del__(self):
+        try:
+            lib().or_kd_free(C.byref(self.t))
+        except Exception:
+            pass
+
+
+class RefKd:
+    def __init__(self, coords):
+        coords = np.ascontiguousarray(coords, np.float64)
+        self.ndim = coords.shape[1]
+        self.n = coords.shape[0]
+        self._coords = coords
+        self.h = ref().ref_kd_build(self.ndim, self.n, _ptr(coords, _dp))
+
+    def range(self, q, rng, ordered=False):
+        n = max(self.n, 1)
+        data = np.zeros(n, np.int32)
+        d2 = np.zeros(n, np.float64)
+        q = np.ascontiguousarray(q, np.float64)
+        k = ref().ref_kd_range(self.h, _ptr(q, _dp), float(rng), int(ordered), n, _ptr(data, _ip),
+                               _ptr(d2, _dp))
+        return data[:k].copy(), d2[:k].copy()
+
+    def __del__(self):
+        try:
+            ref().ref_kd_destroy(self.h)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- obstypes / obs
+def make_obstypes(specs):
+    """specs: list of dicts with locrad (list), locweight (list, normalised
+    here as obstypes.c:533-541 does), rfactor, geographic, statsonly, nlobsmax."""
+    arr = (ObsType * max(len(specs), 1))()
+    for k, sp in enumerate(specs):
+        lr = list(sp.get("locrad", [250.0]))
+        lw = list(sp.get("locweight", [1.0] * len(lr)))
+        s = sum(lw)
+        lw = [x / s for x in lw]
+        arr[k].nlocrad = len(lr)
+        for i in range(len(lr)):
+            arr[k].locrad[i] = lr[i]
+            arr[k].locweight[i] = lw[i]
+        arr[k].rfactor = sp.get("rfactor", 1.0)
+        arr[k].geographic = sp.get("geographic", 1)
+        arr[k].statsonly = sp.get("statsonly", 0)
+        arr[k].nlobsmax = sp.get("nlobsmax", NLOBSMAX_NONE)
+    return arr
+
+
+class OracleObs:
+    """Observations plus per-type kd-trees, as obs_createkdtrees builds them."""
+
+    def __init__(self, otypes, type_, lon, lat, estd, status=None, status_at_build=None):
+        self.types = make_obstypes(otypes)
+        self.ntypes = len(otypes)
+        self.type = np.ascontiguousarray(type_, np.int16)
+        self.lon = np.ascontiguousarray(lon, np.float32)
+        self.lat = np.ascontiguousarray(lat, np.float32)
+        self.estd = np.ascontiguousarray(estd, np.float32)
+        n = len(self.type)
+        self.status = np.ascontiguousarray(status if status is not None else np.zeros(n), np.int16)
+        self.status_build = np.ascontiguousarray(
+            status_at_build if status_at_build is not None else self.status, np.int16)
+        self.o = Obs(n, self.ntypes, _ptr(self.type, _sp), _ptr(self.status, _sp), _ptr(self.lon, _fp),
+                     _ptr(self.lat, _fp), _ptr(self.estd, _fp), self.types, None, None, None)
+        r = lib().or_obs_build_trees(C.byref(self.o), _ptr(self.status_build, _sp))
+        if r != 0:
+            raise RuntimeError("or_obs_build_trees: %d" % r)
+
+    def findlocal(self, lon, lat, geographic=1, visible=None):
+        cap = max(len(self.type), 1)
+        ids = np.zeros(cap, np.int32)
+        lc = np.zeros(cap, np.float64)
+        vis = None if visible is None else np.ascontiguousarray(visible, np.uint8)
+        p = lib().or_findlocal(C.byref(self.o), float(lon), float(lat), int(geographic), _ptr(vis, _up), cap,
+                               _ptr(ids, _ip), _ptr(lc, _dp))
+        if p < 0:
+            raise RuntimeError("or_findlocal: %d" % p)
+        return ids[:p].copy(), lc[:p].copy()
+
+    def __del__(self):
+        try:
+            lib().or_obs_free_trees(C.byref(self.o))
+        except Exception:
+            pass
+
+
+def standardise(nmem, obs, S, s):
+    S = np.ascontiguousarray(S, np.float32).copy()
+    s = np.ascontiguousarray(s, np.float64).copy()
+    lib().or_standardise(nmem, len(obs.type), _ptr(obs.type, _sp), _ptr(obs.estd, _fp), obs.types,
+                         _ptr(S, _fp), _ptr(s, _dp))
+    return S, s
+
+
+# ---------------------------------------------------------------- per-column algebra
+def local_analysis(mode, scheme, m, mout, S, s, use_ref=False):
+    """S: (p, m) array (row o, column e); returns T (mout, m) as C [e1][e2],
+    w (m), dfs, srf (double, ensemble-space as the reference computes)."""
+    S = np.asarray(S, np.float64)
+    p = S.shape[0]
+    Scm = np.ascontiguousarray(S.T)   # column-major p x m == C [m][p]
+    s = np.ascontiguousarray(s, np.float64)
+    T = np.zeros((mout, m))
+    w = np.zeros(m)
+    G = np.zeros((p, m))              # column-major m x p == C [p][m]
+    if use_ref:
+        dfs = C.c_double()
+        srf = C.c_double()
+        ref().ref_local_analysis(mode, scheme, m, mout, p, _ptr(Scm, _dp), _ptr(s, _dp), _ptr(T, _dp),
+                                 _ptr(w, _dp), C.byref(dfs), C.byref(srf), _ptr(G, _dp))
+        return T, w, dfs.value, srf.value
+    L = lib()
+    if mode == MODE_ENOI:
+        r = L.or_calc_G(m, p, _ptr(Scm, _dp), _ptr(G, _dp))
+        L.or_calc_w(m, p, _ptr(G, _dp), _ptr(s, _dp), _ptr(w, _dp))
+        T = None
+    elif scheme == SCHEME_DENKF:
+        r = L.or_calc_wT_denkf(m, mout, p, _ptr(s, _dp), _ptr(Scm, _dp), _ptr(G, _dp), _ptr(w, _dp),
+                               _ptr(T, _dp))
+    else:
+        r = L.or_calc_wT_etkf(m, mout, p, _ptr(s, _dp), _ptr(Scm, _dp), _ptr(G, _dp), _ptr(w, _dp),
+                              _ptr(T, _dp))
+    if r != 0:
+        raise RuntimeError("oracle local analysis failed: %d" % r)
+    L.or_traceprod.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+    dfs = L.or_traceprod(0, 0, p, m, _ptr(G, _dp), _ptr(Scm, _dp))
+    srf = math.sqrt(L.or_traceprod(0, 1, m, p, _ptr(Scm, _dp), _ptr(Scm, _dp)) / dfs) - 1.0
+    return T, w, dfs, srf
+
+
+# ---------------------------------------------------------------- calc loop
+def calc_transforms(mode, scheme, nmem, nmem_dynamic, alpha, obs, S, s, lon, lat, geographic=1,
+                    visible=None, node0=0, nnode_do=-1, use_ref=False, want_T=True):
+    """Runs the das_calctransforms loop.  S: [nmem][nobs] float32 (already
+    standardised), s: [nobs] float64, lon/lat: node coordinates [nnode]."""
+    lon = np.ascontiguousarray(lon, np.float64)
+    lat = np.ascontiguousarray(lat, np.float64)
+    nnode = len(lon)
+    S = np.ascontiguousarray(S, np.float32)
+    s = np.ascontiguousarray(s, np.float64)
+    md = nmem_dynamic
+    nt = obs.ntypes
+    dotT = mode in (MODE_ENKF, MODE_HYBRID) and want_T
+    T = np.zeros((nnode, md, nmem), np.float32) if dotT else None
+    T64 = np.zeros((nnode, md, nmem), np.float64) if dotT else None
+    w = np.zeros((nnode, nmem), np.float32)
+    w64 = np.zeros((nnode, nmem), np.float64)
+    pnlobs = np.zeros((nt + 1, nnode), np.int32)
+    pdfs = np.zeros((nt + 1, nnode), np.float32)
+    psrf = np.zeros((nt + 1, nnode), np.float32)
+    st = CalcStats()
+    spec = CalcSpec(mode, scheme, nmem, md, alpha)
+    vis = None if visible is None else np.ascontiguousarray(visible, np.uint8)
+    if use_ref:
+        r = ref().ref_calc_transforms(C.byref(spec), C.byref(obs.o), _ptr(obs.status_build, _sp), _ptr(S, _fp),
+                                      _ptr(s, _dp), nnode, _ptr(lon, _dp), _ptr(lat, _dp), int(geographic),
+                                      _ptr(vis, _up), _ptr(T, _fp), _ptr(w, _fp), _ptr(T64, _dp),
+                                      _ptr(w64, _dp), _ptr(pnlobs, _ip), _ptr(pdfs, _fp), _ptr(psrf, _fp),
+                                      C.byref(st), node0, nnode_do)
+    else:
+        r = lib().or_calc_transforms(C.byref(spec), C.byref(obs.o), _ptr(S, _fp), _ptr(s, _dp), nnode,
+                                     _ptr(lon, _dp), _ptr(lat, _dp), int(geographic), _ptr(vis, _up),
+                                     _ptr(T, _fp), _ptr(w, _fp), _ptr(T64, _dp), _ptr(w64, _dp),
+                                     _ptr(pnlobs, _ip), _ptr(pdfs, _fp), _ptr(psrf, _fp), C.byref(st),
+                                     node0, nnode_do)
+    if r != 0:
+        raise RuntimeError("calc_transforms failed: %d" % r)
+    stats = dict(nlobs_sum=st.nlobs_sum, nlobs_max=st.nlobs_max, n_inv_obs=st.n_inv_obs,
+                 n_inv_ens=st.n_inv_ens, ncell=st.ncell)
+    return dict(T=T, w=w, T64=T64, w64=w64, pnlobs=pnlobs, pdfs=pdfs, psrf=psrf, stats=stats)
+
+
+# ---------------------------------------------------------------- update
+def _fields(fields):
+    arr = (Field * max(len(fields), 1))()
+    for k, f in enumerate(fields):
+        arr[k].level = f.get("level", 0)
+        arr[k].issurfacevar = f.get("issurfacevar", 0)
+        arr[k].inflation = f.get("inflation", 1.0)
+        arr[k].inf_ratio = f.get("inf_ratio", 1.0)
+        arr[k].applylog = f.get("applylog", 0)
+    return arr
+
+
+def update_fields(mode, nmem, nmem_dynamic, stride, periodic_i, surfk, nlevels, T, w, fields, buffers,
+                  j1, j2, outputinc=0, want_infl=False):
+    """buffers: list of float32 arrays [nmem][jsize][mni], updated in place."""
+    nlevels = np.ascontiguousarray(nlevels, np.int32)
+    mnj, mni = nlevels.shape
+    T = np.ascontiguousarray(T, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    nj_s, ni_s = w.shape[0], w.shape[1]
+    sp = UpdateSpec(mode, nmem, nmem_dynamic, mni, mnj, stride, periodic_i, surfk, _ptr(nlevels, _ip),
+                    outputinc, ni_s, nj_s, _ptr(T, _fp), _ptr(w, _fp))
+    fa = _fields(fields)
+    bufs = (_fp * len(buffers))(*[_ptr(b, _fp) for b in buffers])
+    infl = np.zeros((len(fields), j2 - j1 + 1, mni), np.float32) if want_infl else None
+    r = lib().or_update_fields(C.byref(sp), len(fields), fa, j1, j2, bufs, _ptr(infl, _fp))
+    if r != 0:
+        raise RuntimeError("or_update_fields: %d" % r)
+    return infl
+
+
+def update_bg(nmem, stride, periodic_i, surfk, nlevels, w, fields, buffers, j1, j2):
+    nlevels = np.ascontiguousarray(nlevels, np.int32)
+    mnj, mni = nlevels.shape
+    w = np.ascontiguousarray(w, np.float32)
+    nj_s, ni_s = w.shape[0], w.shape[1]
+    sp = UpdateSpec(MODE_ENOI, nmem, nmem, mni, mnj, stride, periodic_i, surfk, _ptr(nlevels, _ip), 0,
+                    ni_s, nj_s, None, _ptr(w, _fp))
+    fa = _fields(fields)
+    bufs = (_fp * len(buffers))(*[_ptr(b, _fp) for b in buffers])
+    r = lib().or_update_bg(C.byref(sp), len(fields), fa, j1, j2, bufs)
+    if r != 0:
+        raise RuntimeError("or_update_bg: %d" % r)
+
+
+def sethybridensemble(nmem, nmem_dynamic, gamma, v):
+    """v: float32 [nmem][nij] updated in place."""
+    nij = v.shape[1]
+    vmean = np.zeros(nij, np.float64)
+    lib().or_sethybridensemble(nmem, nmem_dynamic, float(gamma), nij, _ptr(v, _fp), _ptr(vmean, _dp))
