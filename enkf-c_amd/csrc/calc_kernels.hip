@@ -1,0 +1,678 @@
+// calc_kernels.hip -- per-column ensemble transforms for the FP64 path.
+//
+// For each stride-grid node the reference (calc/transforms.c:794-879 and
+// calc/calcs.c) gathers the local standardised anomalies S (p x m, tapered,
+// FP64), forms M = I + S^T S, factorises it and builds G = M^-1 S^T (m x p),
+// T and w.  Here the p x m matrices are never formed:
+//
+//   gram   (MFMA FP64)  G0 = S^T S and S^T s, streamed from the gathered
+//                       FP32 rows of S with the taper applied in FP64;
+//   factor (LDS/VGPR)   M = I + G0 inverted by in-register Gauss-Jordan, then
+//                         DEnKF  T = (I + M^-1)/2          (= I - G S / 2)
+//                         ETKF   T = M^-1/2                (= I - (M+M^1/2)^-1 S^T S)
+//                         w = M^-1 S^T s,  dfs = tr(M^-1 G0) (= tr(G S)),
+//                         srf = sqrt(tr(G0) / dfs) - 1
+//                       followed by relaxation to prior and the FP32 cast.
+//   M^-1/2 comes from the scaled, coupled Denman-Beavers iteration
+//   (Y0 = M, Z0 = I, Y <- (mu Y + Z^-1/mu)/2, Z <- (mu Z + Y^-1/mu)/2, with
+//   determinant scaling mu = |det Y det Z|^(-1/(2m)); Higham, "Functions of
+//   Matrices", SIAM 2008, eq. 6.15 and sec. 6.3).
+//
+// The identities used are those of the User Guide (doc/enkf-userguide.tex:
+// 440-451, 490-522); the per-type diagnostics re-solve the analysis with only
+// the obs of one type, as the USE_MPIQUEUE build does (transforms.c:853-879).
+#include "enkfg_internal.h"
+
+namespace enkfg {
+
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ utilities
+__global__ void standardise_kernel(int nmem, int nobs, const short* __restrict__ type,
+                                   const float* __restrict__ estd, const double* __restrict__ sqrt_rfactor,
+                                   double mult, float* __restrict__ S, double* __restrict__ sv)
+{
+    // calc/ensobs.c:849-898: S /= estd * sqrt(rfactor) * sqrt(m - 1)
+    const long long n = (long long)nobs * nmem;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        const long long o = k / nmem;
+        const double div = (double)estd[o] * sqrt_rfactor[type[o]] * mult;
+        S[k] = (float)((double)S[k] / div);
+        if (sv != nullptr && k % nmem == 0)
+            sv[o] = sv[o] / div;
+    }
+}
+
+__global__ void transpose_kernel(const float* __restrict__ in, float* __restrict__ out, long long rows, long long cols)
+{
+    __shared__ float tile[32][33];
+    const long long c0 = (long long)blockIdx.x * 32, r0 = (long long)blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        long long r = r0 + k, c = c0 + threadIdx.x;
+        if (r < rows && c < cols)
+            tile[k][threadIdx.x] = in[r * cols + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        long long c = c0 + k, r = r0 + threadIdx.x;
+        if (r < rows && c < cols)
+            out[c * rows + r] = tile[threadIdx.x][k];
+    }
+}
+
+// ------------------------------------------------------------------ Gram (MFMA)
+// Upper-triangular 16x16 tiles (I <= J) of an MT x MT tile grid, enumerated
+// row by row; wave w of the 4 waves owns tiles w, w + 4, w + 8, ...
+__host__ __device__ constexpr int tri_count(int MT) { return MT * (MT + 1) / 2; }
+__host__ __device__ constexpr int tri_I(int MT, int f)
+{
+    int I = 0;
+    while (f >= MT - I) {
+        f -= MT - I;
+        ++I;
+    }
+    return I;
+}
+__host__ __device__ constexpr int tri_J(int MT, int f)
+{
+    int I = 0;
+    while (f >= MT - I) {
+        f -= MT - I;
+        ++I;
+    }
+    return I + f;
+}
+
+template <int MT, int W>
+__device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, double* __restrict__ gout,
+                                 double* __restrict__ stsout)
+{
+    constexpr int NT = tri_count(MT);
+    constexpr int NTW = (NT - W + 3) / 4;   // tiles of this wave
+    constexpr int MP = 16 * MT;
+    const int lane = threadIdx.x & 63;
+    const int kq = lane >> 4;   // which of the 4 obs of a k-step
+    const int col = lane & 15;
+    const int m = a.m;
+
+    dbl4 acc[NTW > 0 ? NTW : 1];
+#pragma unroll
+    for (int t = 0; t < NTW; ++t)
+        acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    double sts[MT];
+#pragma unroll
+    for (int X = 0; X < MT; ++X)
+        sts[X] = 0.0;
+
+    for (int k0 = 0; k0 < cnt; k0 += 4) {
+        const int k = k0 + kq;
+        double c = 0.0, sl = 0.0;
+        const float* row = a.S;
+        if (k < cnt) {
+            const int id = a.ids[off + k];
+            c = a.lc[off + k];
+            row = a.S + (size_t)id * m;
+            if (W == 0)
+                sl = a.sv[id] * c;
+        }
+        double frag[MT];
+#pragma unroll
+        for (int X = 0; X < MT; ++X) {
+            const int e = 16 * X + col;
+            frag[X] = (k < cnt && e < m) ? (double)row[e] * c : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < NTW; ++t) {
+            const int f = W + 4 * t;
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[tri_I(MT, f)], frag[tri_J(MT, f)], acc[t], 0, 0, 0);
+        }
+        if (W == 0) {
+#pragma unroll
+            for (int X = 0; X < MT; ++X)
+                sts[X] += frag[X] * sl;
+        }
+    }
+    // Operand layouts of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l
+    // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register
+    // i of lane l holds D[4 i + l / 16][l % 16].
+#pragma unroll
+    for (int t = 0; t < NTW; ++t) {
+        const int f = W + 4 * t;
+        const int I = tri_I(MT, f), J = tri_J(MT, f);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 16 * I + 4 * i + kq;
+            const int cc = 16 * J + col;
+            gout[(size_t)r * MP + cc] = acc[t][i];
+            if (I != J)
+                gout[(size_t)cc * MP + r] = acc[t][i];
+        }
+    }
+    if (W == 0) {
+#pragma unroll
+        for (int X = 0; X < MT; ++X) {
+            double v = sts[X];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (kq == 0)
+                stsout[16 * X + col] = v;
+        }
+    }
+}
+
+template <int MT>
+__global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
+{
+    constexpr int MP = 16 * MT;
+    const int b = blockIdx.x;
+    const int node = a.node0 + b / a.nseg;
+    const int seg = b % a.nseg;
+    long long off;
+    int cnt;
+    if (a.pertype) {
+        off = a.offsets[(size_t)node * a.ntypes + seg];
+        cnt = a.counts[(size_t)node * a.ntypes + seg];
+    } else {
+        off = a.offsets[(size_t)node * a.ntypes];
+        cnt = (int)(a.offsets[(size_t)(node + 1) * a.ntypes] - off);
+    }
+    if (cnt == 0)
+        return;
+    double* gout = a.gram + (size_t)b * MP * MP;
+    double* stsout = a.sts + (size_t)b * MP;
+    switch (threadIdx.x >> 6) {
+    case 0: gram_wave<MT, 0>(a, off, cnt, gout, stsout); break;
+    case 1: gram_wave<MT, 1>(a, off, cnt, gout, stsout); break;
+    case 2: gram_wave<MT, 2>(a, off, cnt, gout, stsout); break;
+    default: gram_wave<MT, 3>(a, off, cnt, gout, stsout); break;
+    }
+}
+
+// ------------------------------------------------------------------ factor
+// 256 threads as a 16 x 16 grid; thread (ti, tj) owns the B x B block of rows
+// ti*B.., columns tj*B.. of the m x m matrix (B = ceil(m / 16)).
+struct FactorShared {
+    double prow[2][256];
+    double pcol[2][256];
+    double red[256];
+    double vec[16][256];
+    int flag;
+};
+
+// In-place inversion of the SPD matrix held in blocks `x` by Gauss-Jordan
+// elimination without pivoting (stable for SPD matrices, whose pivots are
+// the positive Schur-complement diagonals).  Returns sum(log(pivot)) (the log
+// of the determinant); sets sh.flag if a pivot is not positive.
+template <int B>
+__device__ inline double gj_invert(double (&x)[B][B], int m, FactorShared& sh)
+{
+    const int tid = threadIdx.x;
+    const int ti = tid >> 4, tj = tid & 15;
+    double logdet = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const int buf = k & 1;
+        const int kb = k / B, kr = k % B;
+        // (register arrays are only indexed with compile-time indices)
+        if (ti == kb) {
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                double v = 0.0;
+#pragma unroll
+                for (int rr = 0; rr < B; ++rr)
+                    v = (rr == kr) ? x[rr][c] : v;
+                sh.prow[buf][tj * B + c] = v;
+            }
+        }
+        if (tj == kb) {
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                double v = 0.0;
+#pragma unroll
+                for (int cc = 0; cc < B; ++cc)
+                    v = (cc == kr) ? x[r][cc] : v;
+                sh.pcol[buf][ti * B + r] = v;
+            }
+        }
+        __syncthreads();
+        const double d = sh.prow[buf][k];
+        if (!(d > 0.0) && tid == 0)
+            sh.flag = 1;
+        const double rd = 1.0 / d;
+        logdet += log(d);
+        double pc[B], pr[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+            pc[r] = sh.pcol[buf][ti * B + r];
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            pr[c] = sh.prow[buf][tj * B + c];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int i = ti * B + r;
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int j = tj * B + c;
+                double v = x[r][c];
+                if (i == k && j == k)
+                    v = rd;
+                else if (i == k)
+                    v = pr[c] * rd;
+                else if (j == k)
+                    v = -pc[r] * rd;
+                else
+                    v = v - (pc[r] * rd) * pr[c];
+                x[r][c] = v;
+            }
+        }
+    }
+    __syncthreads();
+    return logdet;
+}
+
+__device__ inline double block_sum(double v, FactorShared& sh)
+{
+    // deterministic block reduction: wave reduction then 4 partials in order
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        sh.red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = sh.red[0] + sh.red[1] + sh.red[2] + sh.red[3];
+    __syncthreads();
+    return s;
+}
+
+// Loads G0 = sum of the node's segment Gram matrices (one segment if seg >= 0)
+// into x (plus `diag` on the diagonal); returns the trace of G0.
+template <int B>
+__device__ inline double load_gram(const CalcArgs& a, int bnode, int seg, double diag, double (&x)[B][B], FactorShared& sh)
+{
+    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+    const int mp = 16 * ((a.m + 15) / 16);
+    const int m = a.m;
+    double tr = 0.0;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            x[r][c] = 0.0;
+    for (int s = 0; s < a.nseg; ++s) {
+        if (seg >= 0 && s != seg)
+            continue;
+        const int node = a.node0 + bnode;
+        const int cnt = a.pertype ? a.counts[(size_t)node * a.ntypes + s] : 1;
+        if (cnt == 0)
+            continue;
+        const double* g = a.gram + ((size_t)bnode * a.nseg + s) * mp * mp;
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int i = ti * B + r;
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int j = tj * B + c;
+                if (i < m && j < m)
+                    x[r][c] += g[(size_t)i * mp + j];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const int i = ti * B + r;
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int j = tj * B + c;
+            if (i == j && i < m) {
+                tr += x[r][c];
+                x[r][c] += diag;
+            }
+        }
+    }
+    return block_sum(tr, sh);
+}
+
+// <X, G0> for the inverse in x against the node's Gram matrices
+template <int B>
+__device__ inline double dot_gram(const CalcArgs& a, int bnode, int seg, const double (&x)[B][B], FactorShared& sh)
+{
+    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+    const int mp = 16 * ((a.m + 15) / 16);
+    const int m = a.m;
+    double acc = 0.0;
+    for (int s = 0; s < a.nseg; ++s) {
+        if (seg >= 0 && s != seg)
+            continue;
+        const int node = a.node0 + bnode;
+        const int cnt = a.pertype ? a.counts[(size_t)node * a.ntypes + s] : 1;
+        if (cnt == 0)
+            continue;
+        const double* g = a.gram + ((size_t)bnode * a.nseg + s) * mp * mp;
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const int i = ti * B + r;
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int j = tj * B + c;
+                if (i < m && j < m)
+                    acc += x[r][c] * g[(size_t)i * mp + j];
+            }
+        }
+    }
+    return block_sum(acc, sh);
+}
+
+// y = X v for the node's S^T s (summed over segments); result in sh.vec[0][0..m)
+template <int B>
+__device__ inline void matvec_sts(const CalcArgs& a, int bnode, const double (&x)[B][B], FactorShared& sh)
+{
+    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+    const int mp = 16 * ((a.m + 15) / 16);
+    const int m = a.m;
+    double v[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c)
+        v[c] = 0.0;
+    for (int s = 0; s < a.nseg; ++s) {
+        const int node = a.node0 + bnode;
+        const int cnt = a.pertype ? a.counts[(size_t)node * a.ntypes + s] : 1;
+        if (cnt == 0)
+            continue;
+        const double* st = a.sts + ((size_t)bnode * a.nseg + s) * mp;
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int j = tj * B + c;
+            if (j < m)
+                v[c] += st[j];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        double part = 0.0;
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            part += x[r][c] * v[c];
+        sh.vec[tj][ti * B + r] = part;
+    }
+    __syncthreads();
+    if (tid < m) {
+        double sum = 0.0;
+        for (int q = 0; q < 16; ++q)
+            sum += sh.vec[q][tid];
+        sh.vec[0][tid] = sum;
+    }
+    __syncthreads();
+}
+
+template <int B>
+__device__ inline void write_T(const CalcArgs& a, int node, const double (&x)[B][B], double scale, double shift, FactorShared& sh)
+{
+    // T[e1][e2] = shift*delta + scale*X(e1, e2) for e1 < md, then relaxation
+    // to prior T <- (T - I) alpha + I (transforms.c:814-828) and FP32 cast.
+    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+    const int m = a.m, md = a.md;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const int e1 = ti * B + r;
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            const int e2 = tj * B + c;
+            if (e1 < md && e2 < m) {
+                const double dlt = (e1 == e2) ? 1.0 : 0.0;
+                double t = scale * x[r][c] + shift * dlt;
+                t = (t - dlt) * a.alpha + dlt;
+                const size_t o = ((size_t)node * md + e1) * m + e2;
+                a.T[o] = (float)t;
+                if (a.T64 != nullptr)
+                    a.T64[o] = t;
+            }
+        }
+    }
+}
+
+template <int B, bool ETKF>
+__global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
+{
+    __shared__ FactorShared sh;
+    const int bnode = blockIdx.x;
+    const int node = a.node0 + bnode;
+    const int tid = threadIdx.x;
+    const int m = a.m, md = a.md;
+    const bool dotT = (a.mode == ENKFG_MODE_ENKF || a.mode == ENKFG_MODE_HYBRID);
+    const int nt = a.ntypes;
+
+    const long long off = a.offsets[(size_t)node * nt];
+    const int p = (int)(a.offsets[(size_t)(node + 1) * nt] - off);
+
+    if (tid == 0)
+        sh.flag = 0;
+    if (p == 0) {
+        // empty column: T = I, w = 0, diagnostics 0 (transforms.c:766-780)
+        if (dotT)
+            for (int k = tid; k < md * m; k += 256) {
+                const int e1 = k / m, e2 = k % m;
+                a.T[(size_t)node * md * m + k] = (e1 == e2) ? 1.0f : 0.0f;
+                if (a.T64 != nullptr)
+                    a.T64[(size_t)node * md * m + k] = (e1 == e2) ? 1.0 : 0.0;
+            }
+        for (int e = tid; e < m; e += 256) {
+            a.w[(size_t)node * m + e] = 0.0f;
+            if (a.w64 != nullptr)
+                a.w64[(size_t)node * m + e] = 0.0;
+        }
+        for (int t = tid; t <= nt; t += 256) {
+            a.pnlobs[(size_t)t * a.nnode + node] = 0;
+            a.pdfs[(size_t)t * a.nnode + node] = 0.0f;
+            a.psrf[(size_t)t * a.nnode + node] = 0.0f;
+        }
+        return;
+    }
+
+    double x[B][B];
+    const double trG = load_gram<B>(a, bnode, -1, 1.0, x, sh);
+    const double logdetA = gj_invert<B>(x, m, sh);
+    // w = M^-1 S^T s
+    matvec_sts<B>(a, bnode, x, sh);
+    if (tid < m) {
+        const double wv = sh.vec[0][tid];
+        a.w[(size_t)node * m + tid] = (float)wv;
+        if (a.w64 != nullptr)
+            a.w64[(size_t)node * m + tid] = wv;
+    }
+    __syncthreads();
+    // dfs = tr(M^-1 G0), srf = sqrt(tr(G0) / (float) dfs) - 1
+    const double dfs = dot_gram<B>(a, bnode, -1, x, sh);
+    if (tid == 0) {
+        const float fd = (float)dfs;
+        a.pnlobs[(size_t)nt * a.nnode + node] = p;
+        a.pdfs[(size_t)nt * a.nnode + node] = fd;
+        a.psrf[(size_t)nt * a.nnode + node] = (float)(sqrt(trG / (double)fd) - 1.0);
+    }
+
+    if (dotT) {
+        if constexpr (!ETKF) {
+            write_T<B>(a, node, x, 0.5, 0.5, sh);
+        } else {
+            // scaled coupled Denman-Beavers: Y -> M^1/2, Z -> M^-1/2; x holds
+            // M^-1 = Y0^-1 already.  Z lives in shared-by-owner registers.
+            double y[B][B], z[B][B], u[B][B];
+            load_gram<B>(a, bnode, -1, 1.0, y, sh);
+            const int ti = tid >> 4, tj = tid & 15;
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c)
+                    z[r][c] = (ti * B + r == tj * B + c && ti * B + r < m) ? 1.0 : 0.0;
+            double logdetY = logdetA;
+            double logdetZ = 0.0;
+            bool last = false;
+            for (int it = 0; it < 60; ++it) {
+                // x = Y^-1 (for it == 0 already computed)
+                if (it > 0) {
+#pragma unroll
+                    for (int r = 0; r < B; ++r)
+#pragma unroll
+                        for (int c = 0; c < B; ++c)
+                            x[r][c] = y[r][c];
+                    logdetY = gj_invert<B>(x, m, sh);
+                }
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+#pragma unroll
+                    for (int c = 0; c < B; ++c)
+                        u[r][c] = z[r][c];
+                if (it > 0)
+                    logdetZ = gj_invert<B>(u, m, sh);
+                // (for it == 0, Z = I is its own inverse)
+                const double mu = last ? 1.0 : exp(-(logdetY + logdetZ) / (2.0 * m));
+                const double hm = 0.5 * mu, hi = 0.5 / mu;
+                double dn = 0.0, nn = 0.0;
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+#pragma unroll
+                    for (int c = 0; c < B; ++c) {
+                        const double yn = hm * y[r][c] + hi * u[r][c];
+                        const double zn = hm * z[r][c] + hi * x[r][c];
+                        dn += (zn - z[r][c]) * (zn - z[r][c]);
+                        nn += zn * zn;
+                        y[r][c] = yn;
+                        z[r][c] = zn;
+                    }
+                dn = block_sum(dn, sh);
+                nn = block_sum(nn, sh);
+                if (last)
+                    break;
+                if (dn <= 1e-20 * nn)
+                    last = true;   // converged quadratically: one more unscaled step
+            }
+            // symmetrise Z into T: Z(e1,e2) and Z(e2,e1) live in different
+            // threads; exchange through shared memory row by row is avoided by
+            // writing Z as is (symmetric to rounding).
+            write_T<B>(a, node, z, 1.0, 0.0, sh);
+        }
+    }
+
+    // per-type diagnostics: re-solve with the obs of each type only
+    if (a.pertype) {
+        for (int t = 0; t < nt; ++t) {
+            const int pt = a.counts[(size_t)node * nt + t];
+            if (tid == 0)
+                a.pnlobs[(size_t)t * a.nnode + node] = pt;
+            if (pt == 0) {
+                if (tid == 0) {
+                    a.pdfs[(size_t)t * a.nnode + node] = 0.0f;
+                    a.psrf[(size_t)t * a.nnode + node] = 0.0f;
+                }
+                continue;
+            }
+            if (pt == p) {
+                if (tid == 0) {
+                    a.pdfs[(size_t)t * a.nnode + node] = a.pdfs[(size_t)nt * a.nnode + node];
+                    const float fd = (float)dfs;
+                    a.psrf[(size_t)t * a.nnode + node] = ((double)fd > 1.0e-6) ? a.psrf[(size_t)nt * a.nnode + node] : 0.0f;
+                }
+                continue;
+            }
+            const double trt = load_gram<B>(a, bnode, t, 1.0, x, sh);
+            gj_invert<B>(x, m, sh);
+            const double dt = dot_gram<B>(a, bnode, t, x, sh);
+            if (tid == 0) {
+                const float fd = (float)dt;
+                a.pdfs[(size_t)t * a.nnode + node] = fd;
+                a.psrf[(size_t)t * a.nnode + node] = ((double)fd > 1.0e-6) ? (float)(sqrt(trt / (double)fd) - 1.0) : 0.0f;
+            }
+        }
+    } else if (tid < nt) {
+        a.pnlobs[(size_t)tid * a.nnode + node] = a.counts[(size_t)node * nt + tid];
+        a.pdfs[(size_t)tid * a.nnode + node] = 0.0f;
+        a.psrf[(size_t)tid * a.nnode + node] = 0.0f;
+    }
+    __syncthreads();
+    if (tid == 0 && sh.flag)
+        atomicExch(a.err, ENKFG_ERR_LINALG);
+}
+
+}  // namespace
+
+hipError_t launch_standardise(hipStream_t s, int nmem, int nobs, const short* type, const float* estd,
+                              const double* sqrt_rfactor, float* S_obsmajor, double* sv)
+{
+    const long long n = (long long)nmem * nobs;
+    if (n == 0)
+        return hipSuccess;
+    const double mult = sqrt((double)nmem - 1);
+    long long nb = (n + 255) / 256;
+    if (nb > 65536)
+        nb = 65536;
+    hipLaunchKernelGGL(standardise_kernel, dim3((unsigned)nb), dim3(256), 0, s, nmem, nobs, type, estd, sqrt_rfactor,
+                       mult, S_obsmajor, sv);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_f32(hipStream_t s, const float* in, float* out, long long rows, long long cols)
+{
+    if (rows == 0 || cols == 0)
+        return hipSuccess;
+    dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+    if (grid.y > 65535)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, s, in, out, rows, cols);
+    return hipGetLastError();
+}
+
+hipError_t launch_gram(hipStream_t s, const CalcArgs& a)
+{
+    const int MT = (a.m + 15) / 16;
+    dim3 grid((unsigned)(a.nbatch * a.nseg));
+    if (a.nbatch == 0)
+        return hipSuccess;
+    switch (MT) {
+    case 1: hipLaunchKernelGGL(gram_kernel<1>, grid, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(gram_kernel<2>, grid, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(gram_kernel<3>, grid, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(gram_kernel<4>, grid, dim3(256), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(gram_kernel<5>, grid, dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(gram_kernel<6>, grid, dim3(256), 0, s, a); break;
+    case 7: hipLaunchKernelGGL(gram_kernel<7>, grid, dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(gram_kernel<8>, grid, dim3(256), 0, s, a); break;
+    case 9: hipLaunchKernelGGL(gram_kernel<9>, grid, dim3(256), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(gram_kernel<10>, grid, dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int B>
+void launch_factor_b(hipStream_t s, const CalcArgs& a, bool etkf)
+{
+    const dim3 grid((unsigned)a.nbatch);
+    if (etkf)
+        hipLaunchKernelGGL((factor_kernel<B, true>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((factor_kernel<B, false>), grid, dim3(256), 0, s, a);
+}
+
+hipError_t launch_factor(hipStream_t s, const CalcArgs& a)
+{
+    const int B = (a.m + 15) / 16;
+    const bool etkf = a.scheme == ENKFG_SCHEME_ETKF && a.mode != ENKFG_MODE_ENOI;
+    if (a.nbatch == 0)
+        return hipSuccess;
+    switch (B) {
+    case 1: launch_factor_b<1>(s, a, etkf); break;
+    case 2: launch_factor_b<2>(s, a, etkf); break;
+    case 3: launch_factor_b<3>(s, a, etkf); break;
+    case 4: launch_factor_b<4>(s, a, etkf); break;
+    case 5: launch_factor_b<5>(s, a, etkf); break;
+    case 6: launch_factor_b<6>(s, a, etkf); break;
+    case 7: launch_factor_b<7>(s, a, etkf); break;
+    case 8: launch_factor_b<8>(s, a, etkf); break;
+    case 9: launch_factor_b<9>(s, a, etkf); break;
+    case 10: launch_factor_b<10>(s, a, etkf); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace enkfg

@@ -1,0 +1,978 @@
+// enkfg_abi.cpp -- the C ABI of include/enkfg.h: context, inputs, the
+// host-side kd-tree construction, device memory and kernel sequencing.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "enkfg.h"
+#include "enkfg_internal.h"
+
+using namespace enkfg;
+
+namespace {
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n)
+    {
+        if (n <= bytes && p != nullptr)
+            return hipSuccess;
+        if (p != nullptr) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess)
+                return e;
+        }
+        if (n == 0)
+            n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess)
+            bytes = n;
+        return e;
+    }
+    void release()
+    {
+        if (p != nullptr)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
+constexpr double kREarth = 6371.0;            // common/global.h:119
+constexpr double kDeg2Rad = M_PI / 180.0;     // common/global.h:120
+
+// common/utils.c:1449-1458, evaluated with the host libm as the reference does.
+// The reference is built with gcc -O2, which merges the sin and cos of the
+// same argument into one glibc sincos() call; sincos() and separate sin()/cos()
+// differ in the last bit for a fraction of inputs, so sincos() is called
+// explicitly to obtain the reference's coordinates bit for bit.
+void ll2xyz(double lon_deg, double lat_deg, double* out)
+{
+    const double lon = lon_deg * kDeg2Rad;
+    const double lat = lat_deg * kDeg2Rad;
+    double sinlat, coslat, sinlon, coslon;
+    sincos(lat, &sinlat, &coslat);
+    sincos(lon, &sinlon, &coslon);
+    out[0] = kREarth * coslon * coslat;
+    out[1] = kREarth * sinlon * coslat;
+    out[2] = kREarth * sinlat;
+}
+
+}  // namespace
+
+struct enkfg_ctx {
+    enkfg_config cfg{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[10] = {};
+    double timings[4] = {0, 0, 0, 0};
+
+    // obstypes and obs
+    std::vector<enkfg_obstype> types;
+    int nobs = 0;
+    std::vector<short> otype, ostatus;
+    std::vector<float> olon, olat, oestd;
+    bool obs_set = false;
+    DevBuf d_nodes, d_types, d_obsids, d_type, d_status, d_estd, d_sqrtrf;
+    std::vector<TypeInfo> tinfo;
+
+    // S and s
+    DevBuf d_S, d_s, d_tmp;
+    bool S_set = false;
+
+    // localisation
+    int nnode = 0;
+    DevBuf d_qxyz, d_glob, d_vis, d_counts, d_offsets, d_scantmp, d_ids, d_lc;
+    bool lists_valid = false;
+
+    // transforms and diagnostics
+    int ni_s = 0, nj_s = 0, stride = 0;
+    DevBuf d_T, d_w, d_T64, d_w64, d_pnlobs, d_pdfs, d_psrf, d_gram, d_sts, d_err;
+    bool transforms_valid = false, fp64_valid = false, diag_valid = false;
+    enkfg_stats stats{};
+
+    // update
+    DevBuf d_E, d_infl, d_nlev, d_flevel, d_fsurf, d_flog, d_finfl, d_fratio;
+    float* attached_E = nullptr;
+    enkfg_grid att_grid{};
+    int att_nfield = 0;
+    std::vector<enkfg_field> att_fields;
+    std::vector<int> att_nlevels;
+};
+
+namespace {
+
+int fail(enkfg_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c != nullptr)
+        c->err = buf;
+    return code;
+}
+
+#define HIPCHECK(c, expr)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return fail((c), (e_ == hipErrorOutOfMemory) ? ENKFG_ERR_NOMEM : ENKFG_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                                 \
+    } while (0)
+
+// obs_createkdtrees for one obstype: sequential insertion of the type's OK obs
+// in ascending index order (common/kdtree.c:194-274, observations.c:1695-1720).
+void build_tree(const std::vector<double>& xyz, int ndim, const std::vector<int>& ids, std::vector<KdNode>& out)
+{
+    const size_t base = out.size();
+    for (size_t k = 0; k < ids.size(); ++k) {
+        const double* c = &xyz[3 * k];
+        if (!std::isfinite(c[0]))
+            continue;   // kd_insertnode skips non-finite points
+        KdNode n{};
+        n.x = c[0];
+        n.y = c[1];
+        n.z = (ndim == 3) ? c[2] : 0.0;
+        n.left = n.right = -1;
+        n.obsid = ids[k];
+        const int id = (int)(out.size() - base);
+        if (id == 0) {
+            n.parent = -1;
+            n.dir = 0;
+            out.push_back(n);
+            continue;
+        }
+        int cur = 0;
+        for (;;) {
+            KdNode& p = out[base + cur];
+            const double pc = (p.dir == 0) ? p.x : (p.dir == 1 ? p.y : p.z);
+            int& slot = (c[p.dir] < pc) ? p.left : p.right;
+            if (slot < 0) {
+                slot = id;
+                n.parent = cur;
+                n.dir = (p.dir + 1) % ndim;
+                break;
+            }
+            cur = slot;
+        }
+        out.push_back(n);
+    }
+}
+
+int check_ctx(enkfg_ctx* c)
+{
+    return c == nullptr ? ENKFG_ERR_ARG : ENKFG_OK;
+}
+
+int upload_types(enkfg_ctx* c)
+{
+    HIPCHECK(c, c->d_types.reserve(sizeof(TypeInfo) * std::max<size_t>(c->tinfo.size(), 1)));
+    if (!c->tinfo.empty())
+        HIPCHECK(c, hipMemcpy(c->d_types.p, c->tinfo.data(), sizeof(TypeInfo) * c->tinfo.size(), hipMemcpyHostToDevice));
+    return ENKFG_OK;
+}
+
+int node_inputs(enkfg_ctx* c, const enkfg_nodes* nodes)
+{
+    if (nodes == nullptr || nodes->ni <= 0 || nodes->nj <= 0 || nodes->lon == nullptr || nodes->lat == nullptr)
+        return fail(c, ENKFG_ERR_ARG, "enkfg_nodes: invalid node grid");
+    const long long nn = (long long)nodes->ni * nodes->nj;
+    if (nn > 0x7fffffffLL)
+        return fail(c, ENKFG_ERR_ARG, "too many nodes");
+    c->nnode = (int)nn;
+    c->ni_s = nodes->ni;
+    c->nj_s = nodes->nj;
+    c->stride = nodes->stride;
+    std::vector<double> q(3 * (size_t)nn);
+    std::vector<unsigned char> g(nn);
+    for (long long k = 0; k < nn; ++k) {
+        const double lon = nodes->lon[k], lat = nodes->lat[k];
+        g[k] = std::isnan(lon + lat) ? 1 : 0;   // obs_getglobal path
+        if (g[k]) {
+            q[3 * k] = q[3 * k + 1] = q[3 * k + 2] = 0.0;
+        } else if (nodes->geographic) {
+            ll2xyz(lon, lat, &q[3 * k]);
+        } else {
+            q[3 * k] = lon;
+            q[3 * k + 1] = lat;
+            q[3 * k + 2] = std::numeric_limits<double>::quiet_NaN();
+        }
+    }
+    HIPCHECK(c, c->d_qxyz.reserve(q.size() * sizeof(double)));
+    HIPCHECK(c, c->d_glob.reserve(nn));
+    HIPCHECK(c, hipMemcpy(c->d_qxyz.p, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_glob.p, g.data(), nn, hipMemcpyHostToDevice));
+    const int nt = (int)c->types.size();
+    if (nodes->visible != nullptr) {
+        HIPCHECK(c, c->d_vis.reserve(std::max(nt, 1)));
+        HIPCHECK(c, hipMemcpy(c->d_vis.p, nodes->visible, nt, hipMemcpyHostToDevice));
+    }
+    return ENKFG_OK;
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess)
+        return -1.0;
+    return ms;
+}
+
+int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
+{
+    if (!c->obs_set)
+        return fail(c, ENKFG_ERR_STATE, "observations not set");
+    for (const auto& t : c->types)
+        if (t.nlobsmax != ENKFG_NLOBSMAX_NONE)
+            return fail(c, ENKFG_ERR_UNSUPPORTED, "NLOBSMAX is not supported by the GPU search yet");
+    int r = node_inputs(c, nodes);
+    if (r != ENKFG_OK)
+        return r;
+    const int nt = (int)c->types.size();
+    const long long ncnt = (long long)c->nnode * nt;
+    HIPCHECK(c, c->d_counts.reserve(std::max<long long>(ncnt, 1) * sizeof(int)));
+    HIPCHECK(c, c->d_offsets.reserve((ncnt + 1) * sizeof(long long)));
+    const unsigned char* vis = nodes->visible ? c->d_vis.as<unsigned char>() : nullptr;
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    HIPCHECK(c, launch_loc_count(c->stream, c->nnode, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
+                                 c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(),
+                                 c->d_counts.as<int>()));
+    size_t tmpb = 0;
+    HIPCHECK(c, launch_scan_counts(c->stream, nullptr, nullptr, ncnt, nullptr, &tmpb));
+    HIPCHECK(c, c->d_scantmp.reserve(tmpb));
+    tmpb = c->d_scantmp.bytes;
+    HIPCHECK(c, launch_scan_counts(c->stream, c->d_counts.as<int>(), c->d_offsets.as<long long>(), ncnt,
+                                   c->d_scantmp.p, &tmpb));
+    long long total = 0;
+    HIPCHECK(c, hipMemcpyAsync(&total, c->d_offsets.as<long long>() + ncnt, sizeof total, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, c->d_ids.reserve(std::max<long long>(total, 1) * sizeof(int)));
+    HIPCHECK(c, c->d_lc.reserve(std::max<long long>(total, 1) * sizeof(double)));
+    HIPCHECK(c, launch_loc_fill(c->stream, c->nnode, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
+                                c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(),
+                                c->d_obsids.as<int>(), c->d_offsets.as<long long>(), c->d_ids.as<int>(),
+                                c->d_lc.as<double>()));
+    HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->timings[0] = elapsed(c->ev[0], c->ev[1]);
+    c->lists_valid = true;
+    return ENKFG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int enkfg_create(enkfg_ctx** out, const enkfg_config* cfg)
+{
+    if (out == nullptr || cfg == nullptr)
+        return ENKFG_ERR_ARG;
+    *out = nullptr;
+    if (cfg->nmem < 2 || cfg->nmem > 160 || cfg->nmem_dynamic < 1 || cfg->nmem_dynamic > cfg->nmem)
+        return ENKFG_ERR_ARG;
+    if (cfg->mode != ENKFG_MODE_ENKF && cfg->mode != ENKFG_MODE_ENOI && cfg->mode != ENKFG_MODE_HYBRID)
+        return ENKFG_ERR_ARG;
+    if (cfg->scheme != ENKFG_SCHEME_DENKF && cfg->scheme != ENKFG_SCHEME_ETKF)
+        return ENKFG_ERR_ARG;
+    if (cfg->precision != ENKFG_PREC_FP64)
+        return ENKFG_ERR_UNSUPPORTED;
+    if (cfg->mode != ENKFG_MODE_HYBRID && cfg->nmem_dynamic != cfg->nmem)
+        return ENKFG_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return ENKFG_ERR_HIP;
+    if (cfg->device < 0 || cfg->device >= ndev)
+        return ENKFG_ERR_ARG;
+    if (hipSetDevice(cfg->device) != hipSuccess)
+        return ENKFG_ERR_HIP;
+    enkfg_ctx* c = new enkfg_ctx();
+    c->cfg = *cfg;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ENKFG_ERR_HIP;
+    }
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return ENKFG_ERR_HIP;
+        }
+    if (c->d_err.reserve(sizeof(int)) != hipSuccess) {
+        delete c;
+        return ENKFG_ERR_NOMEM;
+    }
+    *out = c;
+    return ENKFG_OK;
+}
+
+void enkfg_destroy(enkfg_ctx* c)
+{
+    if (c == nullptr)
+        return;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream != nullptr)
+        (void)hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->d_nodes, &c->d_types, &c->d_obsids, &c->d_type, &c->d_status, &c->d_estd, &c->d_sqrtrf,
+                      &c->d_S, &c->d_s, &c->d_tmp, &c->d_qxyz, &c->d_glob, &c->d_vis, &c->d_counts, &c->d_offsets,
+                      &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
+                      &c->d_pdfs, &c->d_psrf, &c->d_gram, &c->d_sts, &c->d_err, &c->d_E, &c->d_infl, &c->d_nlev,
+                      &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio};
+    for (DevBuf* b : bufs)
+        b->release();
+    for (auto& e : c->ev)
+        if (e != nullptr)
+            (void)hipEventDestroy(e);
+    if (c->stream != nullptr)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* enkfg_last_error(const enkfg_ctx* c)
+{
+    return c == nullptr ? "null context" : c->err.c_str();
+}
+
+int enkfg_sync(enkfg_ctx* c)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return ENKFG_OK;
+}
+
+int enkfg_set_obstypes(enkfg_ctx* c, int ntypes, const enkfg_obstype* types)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (ntypes < 0 || (ntypes > 0 && types == nullptr) || ntypes > 32767)
+        return fail(c, ENKFG_ERR_ARG, "invalid obstypes");
+    for (int t = 0; t < ntypes; ++t)
+        if (types[t].nlocrad < 1 || types[t].nlocrad > ENKFG_MAXLOCRAD || !(types[t].rfactor > 0.0))
+            return fail(c, ENKFG_ERR_ARG, "obstype %d: invalid nlocrad or rfactor", t);
+    c->types.assign(types, types + ntypes);
+    c->obs_set = false;
+    c->lists_valid = false;
+    return ENKFG_OK;
+}
+
+int enkfg_set_obs(enkfg_ctx* c, int nobs, const short* type, const short* status, const float* lon, const float* lat,
+                  const float* estd)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (nobs < 0 || (nobs > 0 && (!type || !status || !lon || !lat || !estd)))
+        return fail(c, ENKFG_ERR_ARG, "invalid obs arrays");
+    const int nt = (int)c->types.size();
+    for (int i = 0; i < nobs; ++i)
+        if (type[i] < 0 || type[i] >= nt)
+            return fail(c, ENKFG_ERR_ARG, "obs %d: type %d out of range", i, (int)type[i]);
+    c->nobs = nobs;
+    c->otype.assign(type, type + nobs);
+    c->ostatus.assign(status, status + nobs);
+    c->olon.assign(lon, lon + nobs);
+    c->olat.assign(lat, lat + nobs);
+    c->oestd.assign(estd, estd + nobs);
+
+    // trees (obs_createkdtrees)
+    std::vector<KdNode> nodes;
+    std::vector<int> obsids;
+    c->tinfo.assign(nt, TypeInfo{});
+    for (int t = 0; t < nt; ++t) {
+        const enkfg_obstype& ot = c->types[t];
+        TypeInfo& ti = c->tinfo[t];
+        ti.ndim = ot.geographic ? 3 : 2;
+        ti.nlocrad = ot.nlocrad;
+        ti.nlobsmax = ot.nlobsmax;
+        double rmax = 0.0;
+        for (int k = 0; k < ot.nlocrad; ++k) {
+            ti.locrad[k] = ot.locrad[k];
+            ti.locweight[k] = ot.locweight[k];
+            if (ot.locrad[k] > rmax)
+                rmax = ot.locrad[k];   // obstype_getmaxlocrad
+        }
+        ti.range = rmax;
+        ti.node0 = (int)nodes.size();
+        ti.obsid0 = (int)obsids.size();
+        if (ot.statsonly)
+            continue;
+        std::vector<int> ids;
+        for (int i = 0; i < nobs; ++i)
+            if (type[i] == t && status[i] == ENKFG_STATUS_OK)
+                ids.push_back(i);
+        std::vector<double> xyz(3 * ids.size());
+        for (size_t k = 0; k < ids.size(); ++k) {
+            const int i = ids[k];
+            if (ot.geographic)
+                ll2xyz((double)lon[i], (double)lat[i], &xyz[3 * k]);
+            else {
+                xyz[3 * k] = (double)lon[i];
+                xyz[3 * k + 1] = (double)lat[i];
+                xyz[3 * k + 2] = 0.0;
+            }
+        }
+        build_tree(xyz, ti.ndim, ids, nodes);
+        ti.nnode = (int)(nodes.size() - ti.node0);
+        // obs_getglobal lists obsids in index order, including obs the tree
+        // skipped for non-finite coordinates
+        obsids.insert(obsids.end(), ids.begin(), ids.end());
+        ti.nids = (int)ids.size();
+        ti.searchable = ids.empty() ? 0 : 1;
+    }
+    HIPCHECK(c, c->d_nodes.reserve(std::max<size_t>(nodes.size(), 1) * sizeof(KdNode)));
+    if (!nodes.empty())
+        HIPCHECK(c, hipMemcpy(c->d_nodes.p, nodes.data(), nodes.size() * sizeof(KdNode), hipMemcpyHostToDevice));
+    HIPCHECK(c, c->d_obsids.reserve(std::max<size_t>(obsids.size(), 1) * sizeof(int)));
+    if (!obsids.empty())
+        HIPCHECK(c, hipMemcpy(c->d_obsids.p, obsids.data(), obsids.size() * sizeof(int), hipMemcpyHostToDevice));
+    int r = upload_types(c);
+    if (r != ENKFG_OK)
+        return r;
+    HIPCHECK(c, c->d_type.reserve(std::max(nobs, 1) * sizeof(short)));
+    HIPCHECK(c, c->d_status.reserve(std::max(nobs, 1) * sizeof(short)));
+    HIPCHECK(c, c->d_estd.reserve(std::max(nobs, 1) * sizeof(float)));
+    if (nobs > 0) {
+        HIPCHECK(c, hipMemcpy(c->d_type.p, type, nobs * sizeof(short), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_status.p, status, nobs * sizeof(short), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_estd.p, estd, nobs * sizeof(float), hipMemcpyHostToDevice));
+    }
+    std::vector<double> srf(std::max(nt, 1));
+    for (int t = 0; t < nt; ++t)
+        srf[t] = sqrt(c->types[t].rfactor);
+    HIPCHECK(c, c->d_sqrtrf.reserve(srf.size() * sizeof(double)));
+    HIPCHECK(c, hipMemcpy(c->d_sqrtrf.p, srf.data(), srf.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->obs_set = true;
+    c->S_set = false;
+    c->lists_valid = false;
+    return ENKFG_OK;
+}
+
+int enkfg_set_obs_status(enkfg_ctx* c, const short* status)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->obs_set || status == nullptr)
+        return fail(c, ENKFG_ERR_STATE, "observations not set");
+    c->ostatus.assign(status, status + c->nobs);
+    if (c->nobs > 0)
+        HIPCHECK(c, hipMemcpy(c->d_status.p, status, c->nobs * sizeof(short), hipMemcpyHostToDevice));
+    c->lists_valid = false;
+    return ENKFG_OK;
+}
+
+int enkfg_set_S(enkfg_ctx* c, const float* S, const double* s, int standardised)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->obs_set)
+        return fail(c, ENKFG_ERR_STATE, "observations not set");
+    const int m = c->cfg.nmem;
+    const size_t n = (size_t)m * c->nobs;
+    if (c->nobs > 0 && (S == nullptr || s == nullptr))
+        return fail(c, ENKFG_ERR_ARG, "S or s is NULL");
+    for (int i = 0; i < c->nobs; ++i)
+        if (!std::isfinite(s[i]))
+            return fail(c, ENKFG_ERR_NONFINITE, "obs %d: non-finite innovation", i);
+    HIPCHECK(c, c->d_S.reserve(std::max<size_t>(n, 1) * sizeof(float)));
+    HIPCHECK(c, c->d_tmp.reserve(std::max<size_t>(n, 1) * sizeof(float)));
+    HIPCHECK(c, c->d_s.reserve(std::max(c->nobs, 1) * sizeof(double)));
+    if (c->nobs > 0) {
+        HIPCHECK(c, hipMemcpyAsync(c->d_tmp.p, S, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(c->d_s.p, s, c->nobs * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        // [m][nobs] -> [nobs][m]
+        HIPCHECK(c, launch_members_to_cells(c->stream, c->d_tmp.as<float>(), c->d_S.as<float>(), m, c->nobs));
+        if (!standardised)
+            HIPCHECK(c, launch_standardise(c->stream, m, c->nobs, c->d_type.as<short>(), c->d_estd.as<float>(),
+                                           c->d_sqrtrf.as<double>(), c->d_S.as<float>(), c->d_s.as<double>()));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    c->S_set = true;
+    return ENKFG_OK;
+}
+
+int enkfg_get_S(enkfg_ctx* c, float* S, double* s)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->S_set)
+        return fail(c, ENKFG_ERR_STATE, "S not set");
+    const int m = c->cfg.nmem;
+    const size_t n = (size_t)m * c->nobs;
+    if (c->nobs == 0)
+        return ENKFG_OK;
+    if (S != nullptr) {
+        HIPCHECK(c, launch_cells_to_members(c->stream, c->d_S.as<float>(), c->d_tmp.as<float>(), m, c->nobs));
+        HIPCHECK(c, hipMemcpyAsync(S, c->d_tmp.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (s != nullptr)
+        HIPCHECK(c, hipMemcpyAsync(s, c->d_s.p, c->nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return ENKFG_OK;
+}
+
+int enkfg_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    return do_localise(c, nodes);
+}
+
+int enkfg_get_local_counts(enkfg_ctx* c, int* counts)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->lists_valid || counts == nullptr)
+        return fail(c, ENKFG_ERR_STATE, "no local lists");
+    const int nt = (int)c->types.size();
+    std::vector<long long> off(c->nnode + 1);
+    for (int k = 0; k <= c->nnode; ++k)
+        HIPCHECK(c, hipMemcpy(&off[k], c->d_offsets.as<long long>() + (size_t)k * nt, sizeof(long long),
+                              hipMemcpyDeviceToHost));
+    for (int k = 0; k < c->nnode; ++k)
+        counts[k] = (int)(off[k + 1] - off[k]);
+    return ENKFG_OK;
+}
+
+int enkfg_get_localobs(enkfg_ctx* c, int node, int cap, int* ids, double* lc)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->lists_valid)
+        return fail(c, ENKFG_ERR_STATE, "no local lists");
+    if (node < 0 || node >= c->nnode)
+        return fail(c, ENKFG_ERR_ARG, "node out of range");
+    const int nt = (int)c->types.size();
+    long long o[2];
+    HIPCHECK(c, hipMemcpy(&o[0], c->d_offsets.as<long long>() + (size_t)node * nt, sizeof(long long),
+                          hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(&o[1], c->d_offsets.as<long long>() + (size_t)(node + 1) * nt, sizeof(long long),
+                          hipMemcpyDeviceToHost));
+    const long long p = o[1] - o[0];
+    if (p > cap)
+        return fail(c, ENKFG_ERR_ARG, "cap %d < %lld local obs", cap, p);
+    if (p > 0) {
+        if (ids != nullptr)
+            HIPCHECK(c, hipMemcpy(ids, c->d_ids.as<int>() + o[0], p * sizeof(int), hipMemcpyDeviceToHost));
+        if (lc != nullptr)
+            HIPCHECK(c, hipMemcpy(lc, c->d_lc.as<double>() + o[0], p * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return (int)ENKFG_OK;
+}
+
+int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->S_set)
+        return fail(c, ENKFG_ERR_STATE, "S not set");
+    int r = do_localise(c, nodes);
+    if (r != ENKFG_OK)
+        return r;
+    const int m = c->cfg.nmem, md = c->cfg.nmem_dynamic;
+    const int nt = (int)c->types.size();
+    const int nn = c->nnode;
+    const bool dotT = c->cfg.mode != ENKFG_MODE_ENOI;
+    const bool keep64 = (c->cfg.flags & ENKFG_FLAG_KEEP_FP64) != 0;
+    const bool pertype = (c->cfg.flags & ENKFG_FLAG_PERTYPE_DIAG) != 0 && nt > 0;
+    if (c->cfg.scheme == ENKFG_SCHEME_ETKF && dotT && m > 96)
+        return fail(c, ENKFG_ERR_UNSUPPORTED, "ETKF on the GPU needs nmem <= 96 (got %d)", m);
+    HIPCHECK(c, c->d_T.reserve(std::max<size_t>(dotT ? (size_t)nn * md * m : 1, 1) * sizeof(float)));
+    HIPCHECK(c, c->d_w.reserve((size_t)nn * m * sizeof(float)));
+    if (keep64) {
+        HIPCHECK(c, c->d_T64.reserve(std::max<size_t>(dotT ? (size_t)nn * md * m : 1, 1) * sizeof(double)));
+        HIPCHECK(c, c->d_w64.reserve((size_t)nn * m * sizeof(double)));
+    }
+    HIPCHECK(c, c->d_pnlobs.reserve((size_t)(nt + 1) * nn * sizeof(int)));
+    HIPCHECK(c, c->d_pdfs.reserve((size_t)(nt + 1) * nn * sizeof(float)));
+    HIPCHECK(c, c->d_psrf.reserve((size_t)(nt + 1) * nn * sizeof(float)));
+
+    const int mp = 16 * ((m + 15) / 16);
+    const int nseg = pertype ? nt : 1;
+    const size_t per_node = (size_t)nseg * mp * mp * sizeof(double);
+    const size_t budget = (size_t)2 << 30;
+    int nbatch = (int)std::max<size_t>(1, std::min<size_t>(budget / per_node, (size_t)nn));
+    if (nbatch > 65536)
+        nbatch = 65536;
+    HIPCHECK(c, c->d_gram.reserve((size_t)nbatch * per_node));
+    HIPCHECK(c, c->d_sts.reserve((size_t)nbatch * nseg * mp * sizeof(double)));
+    int zero = 0;
+    HIPCHECK(c, hipMemcpy(c->d_err.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+
+    CalcArgs a{};
+    a.m = m;
+    a.md = md;
+    a.mode = c->cfg.mode;
+    a.scheme = c->cfg.scheme;
+    a.ntypes = nt;
+    a.pertype = pertype ? 1 : 0;
+    a.alpha = c->cfg.alpha;
+    a.nseg = nseg;
+    a.counts = c->d_counts.as<int>();
+    a.offsets = c->d_offsets.as<long long>();
+    a.ids = c->d_ids.as<int>();
+    a.lc = c->d_lc.as<double>();
+    a.S = c->d_S.as<float>();
+    a.sv = c->d_s.as<double>();
+    a.gram = c->d_gram.as<double>();
+    a.sts = c->d_sts.as<double>();
+    a.T = dotT ? c->d_T.as<float>() : nullptr;
+    a.w = c->d_w.as<float>();
+    a.T64 = (keep64 && dotT) ? c->d_T64.as<double>() : nullptr;
+    a.w64 = keep64 ? c->d_w64.as<double>() : nullptr;
+    a.pnlobs = c->d_pnlobs.as<int>();
+    a.pdfs = c->d_pdfs.as<float>();
+    a.psrf = c->d_psrf.as<float>();
+    a.nnode = nn;
+    a.err = c->d_err.as<int>();
+    double tg = 0.0, tf = 0.0;
+    for (int n0 = 0; n0 < nn; n0 += nbatch) {
+        a.node0 = n0;
+        a.nbatch = std::min(nbatch, nn - n0);
+        HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+        HIPCHECK(c, launch_gram(c->stream, a));
+        HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
+        HIPCHECK(c, launch_factor(c->stream, a));
+        HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
+        HIPCHECK(c, hipEventSynchronize(c->ev[4]));
+        tg += elapsed(c->ev[2], c->ev[3]);
+        tf += elapsed(c->ev[3], c->ev[4]);
+    }
+    c->timings[1] = tg;
+    c->timings[2] = tf;
+    int err = 0;
+    HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
+    if (err != 0)
+        return fail(c, ENKFG_ERR_LINALG, "non-positive pivot in the factorisation of I + S^T S");
+
+    // calcstats (transforms.c:44-50) from the counts
+    std::vector<long long> off((size_t)nn + 1);
+    {
+        std::vector<long long> all((size_t)nn * nt + 1);
+        HIPCHECK(c, hipMemcpy(all.data(), c->d_offsets.p, all.size() * sizeof(long long), hipMemcpyDeviceToHost));
+        for (int k = 0; k <= nn; ++k)
+            off[k] = all[(size_t)k * nt];
+    }
+    enkfg_stats st{};
+    for (int k = 0; k < nn; ++k) {
+        const int p = (int)(off[k + 1] - off[k]);
+        st.nlobs_sum += p;
+        if (p > st.nlobs_max)
+            st.nlobs_max = p;
+        st.ncell++;
+        if (p > 0) {
+            if (p >= m)
+                st.n_inv_ens++;
+            else
+                st.n_inv_obs++;
+        }
+    }
+    c->stats = st;
+    c->transforms_valid = true;
+    c->fp64_valid = keep64;
+    c->diag_valid = true;
+    return ENKFG_OK;
+}
+
+int enkfg_get_transforms(enkfg_ctx* c, float* T, float* w, double* T64, double* w64)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->transforms_valid)
+        return fail(c, ENKFG_ERR_STATE, "no transforms");
+    const size_t nn = (size_t)c->nnode, m = c->cfg.nmem, md = c->cfg.nmem_dynamic;
+    const bool dotT = c->cfg.mode != ENKFG_MODE_ENOI;
+    if (T != nullptr && dotT)
+        HIPCHECK(c, hipMemcpy(T, c->d_T.p, nn * md * m * sizeof(float), hipMemcpyDeviceToHost));
+    if (w != nullptr)
+        HIPCHECK(c, hipMemcpy(w, c->d_w.p, nn * m * sizeof(float), hipMemcpyDeviceToHost));
+    if ((T64 != nullptr || w64 != nullptr) && !c->fp64_valid)
+        return fail(c, ENKFG_ERR_STATE, "FP64 transforms not kept (ENKFG_FLAG_KEEP_FP64)");
+    if (T64 != nullptr && dotT)
+        HIPCHECK(c, hipMemcpy(T64, c->d_T64.p, nn * md * m * sizeof(double), hipMemcpyDeviceToHost));
+    if (w64 != nullptr)
+        HIPCHECK(c, hipMemcpy(w64, c->d_w64.p, nn * m * sizeof(double), hipMemcpyDeviceToHost));
+    return ENKFG_OK;
+}
+
+int enkfg_get_diag(enkfg_ctx* c, int* pnlobs, float* pdfs, float* psrf)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->diag_valid)
+        return fail(c, ENKFG_ERR_STATE, "no diagnostics");
+    const size_t n = (c->types.size() + 1) * (size_t)c->nnode;
+    if (pnlobs != nullptr)
+        HIPCHECK(c, hipMemcpy(pnlobs, c->d_pnlobs.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    if (pdfs != nullptr)
+        HIPCHECK(c, hipMemcpy(pdfs, c->d_pdfs.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (psrf != nullptr)
+        HIPCHECK(c, hipMemcpy(psrf, c->d_psrf.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return ENKFG_OK;
+}
+
+int enkfg_get_stats(enkfg_ctx* c, enkfg_stats* st)
+{
+    if (check_ctx(c) || st == nullptr)
+        return ENKFG_ERR_ARG;
+    if (!c->diag_valid)
+        return fail(c, ENKFG_ERR_STATE, "no statistics");
+    *st = c->stats;
+    return ENKFG_OK;
+}
+
+int enkfg_get_timings(enkfg_ctx* c, double* ms, int n)
+{
+    if (check_ctx(c) || ms == nullptr)
+        return ENKFG_ERR_ARG;
+    for (int k = 0; k < n && k < 4; ++k)
+        ms[k] = c->timings[k];
+    return ENKFG_OK;
+}
+
+// ---------------------------------------------------------------- update
+
+int enkfg_set_transforms(enkfg_ctx* c, int nj_s, int ni_s, int stride, const float* T, const float* w)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    const bool dotT = c->cfg.mode != ENKFG_MODE_ENOI;
+    if (nj_s <= 0 || ni_s <= 0 || stride <= 0 || w == nullptr || (dotT && T == nullptr))
+        return fail(c, ENKFG_ERR_ARG, "invalid transforms");
+    const size_t nn = (size_t)nj_s * ni_s, m = c->cfg.nmem, md = c->cfg.nmem_dynamic;
+    HIPCHECK(c, c->d_w.reserve(nn * m * sizeof(float)));
+    HIPCHECK(c, hipMemcpy(c->d_w.p, w, nn * m * sizeof(float), hipMemcpyHostToDevice));
+    if (dotT) {
+        HIPCHECK(c, c->d_T.reserve(nn * md * m * sizeof(float)));
+        HIPCHECK(c, hipMemcpy(c->d_T.p, T, nn * md * m * sizeof(float), hipMemcpyHostToDevice));
+    }
+    c->nj_s = nj_s;
+    c->ni_s = ni_s;
+    c->stride = stride;
+    c->nnode = (int)nn;
+    c->transforms_valid = true;
+    c->fp64_valid = false;
+    c->diag_valid = false;
+    return ENKFG_OK;
+}
+
+namespace {
+
+int check_grid(enkfg_ctx* c, const enkfg_grid* g)
+{
+    if (g == nullptr || g->mni <= 0 || g->mnj <= 0 || g->nlevels == nullptr)
+        return fail(c, ENKFG_ERR_ARG, "invalid grid");
+    if (!c->transforms_valid)
+        return fail(c, ENKFG_ERR_STATE, "no transforms for the update");
+    if (g->stride != c->stride)
+        return fail(c, ENKFG_ERR_ARG, "grid stride = %d is not equal to the stride of transforms = %d", g->stride,
+                    c->stride);
+    if ((g->mni - 1) / g->stride + 1 != c->ni_s || (g->mnj - 1) / g->stride + 1 != c->nj_s)
+        return fail(c, ENKFG_ERR_ARG, "grid %d x %d does not match the %d x %d transforms grid", g->mnj, g->mni,
+                    c->nj_s, c->ni_s);
+    return ENKFG_OK;
+}
+
+int upload_fields(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enkfg_field* f)
+{
+    std::vector<int> lev(nfield), sf(nfield), lg(nfield);
+    std::vector<float> inf(nfield);
+    std::vector<double> rat(nfield);
+    for (int k = 0; k < nfield; ++k) {
+        lev[k] = f[k].level;
+        sf[k] = f[k].issurfacevar;
+        lg[k] = f[k].applylog;
+        inf[k] = f[k].inflation;
+        rat[k] = f[k].inf_ratio;
+    }
+    HIPCHECK(c, c->d_flevel.reserve(nfield * sizeof(int)));
+    HIPCHECK(c, c->d_fsurf.reserve(nfield * sizeof(int)));
+    HIPCHECK(c, c->d_flog.reserve(nfield * sizeof(int)));
+    HIPCHECK(c, c->d_finfl.reserve(nfield * sizeof(float)));
+    HIPCHECK(c, c->d_fratio.reserve(nfield * sizeof(double)));
+    HIPCHECK(c, hipMemcpy(c->d_flevel.p, lev.data(), nfield * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_fsurf.p, sf.data(), nfield * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_flog.p, lg.data(), nfield * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_finfl.p, inf.data(), nfield * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_fratio.p, rat.data(), nfield * sizeof(double), hipMemcpyHostToDevice));
+    const size_t ncell = (size_t)g->mni * g->mnj;
+    HIPCHECK(c, c->d_nlev.reserve(ncell * sizeof(int)));
+    HIPCHECK(c, hipMemcpy(c->d_nlev.p, g->nlevels, ncell * sizeof(int), hipMemcpyHostToDevice));
+    return ENKFG_OK;
+}
+
+UpdArgs upd_args(enkfg_ctx* c, const enkfg_grid* g, int nfield, int j1, int jsize, int flags)
+{
+    UpdArgs a{};
+    a.m = c->cfg.nmem;
+    a.md = c->cfg.nmem_dynamic;
+    a.mode = c->cfg.mode;
+    a.mni = g->mni;
+    a.mnj = g->mnj;
+    a.stride = g->stride;
+    a.periodic_i = g->periodic_i;
+    a.surfk = g->surfk;
+    a.ni_s = c->ni_s;
+    a.nj_s = c->nj_s;
+    a.j1 = j1;
+    a.jsize = jsize;
+    a.nfield = nfield;
+    a.outputinc = (flags & ENKFG_UPD_OUTPUTINC) ? 1 : 0;
+    a.nlevels = c->d_nlev.as<int>();
+    a.T = c->d_T.as<float>();
+    a.w = c->d_w.as<float>();
+    a.flevel = c->d_flevel.as<int>();
+    a.fsurf = c->d_fsurf.as<int>();
+    a.fapplylog = c->d_flog.as<int>();
+    a.finfl = c->d_finfl.as<float>();
+    a.finfratio = c->d_fratio.as<double>();
+    a.ncell = (long long)jsize * g->mni;
+    return a;
+}
+
+}  // namespace
+
+int enkfg_update_fields(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enkfg_field* fields, int j1, int j2,
+                        float** fb, float* infl, int flags)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (c->cfg.mode == ENKFG_MODE_ENOI)
+        return fail(c, ENKFG_ERR_ARG, "das_updatefields needs EnKF or hybrid mode");
+    int r = check_grid(c, g);
+    if (r != ENKFG_OK)
+        return r;
+    if (nfield <= 0 || fields == nullptr || fb == nullptr || j1 < 0 || j2 < j1 || j2 >= g->mnj || j1 % g->stride)
+        return fail(c, ENKFG_ERR_ARG, "invalid field batch");
+    r = upload_fields(c, g, nfield, fields);
+    if (r != ENKFG_OK)
+        return r;
+    const int m = c->cfg.nmem;
+    const int jsize = j2 - j1 + 1;
+    const size_t ncell = (size_t)jsize * g->mni;
+    HIPCHECK(c, c->d_E.reserve((size_t)nfield * ncell * m * sizeof(float)));
+    HIPCHECK(c, c->d_tmp.reserve(ncell * m * sizeof(float)));
+    for (int f = 0; f < nfield; ++f) {
+        if (fb[f] == nullptr)
+            return fail(c, ENKFG_ERR_ARG, "field buffer %d is NULL", f);
+        HIPCHECK(c, hipMemcpyAsync(c->d_tmp.p, fb[f], ncell * m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, launch_members_to_cells(c->stream, c->d_tmp.as<float>(), c->d_E.as<float>() + (size_t)f * ncell * m,
+                                            m, (long long)ncell));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    UpdArgs a = upd_args(c, g, nfield, j1, jsize, flags);
+    a.E = c->d_E.as<float>();
+    if (infl != nullptr) {
+        HIPCHECK(c, c->d_infl.reserve((size_t)nfield * ncell * sizeof(float)));
+        a.infl = c->d_infl.as<float>();
+    }
+    if ((flags & ENKFG_UPD_HYBRID_CONDITION) && c->cfg.mode == ENKFG_MODE_HYBRID)
+        HIPCHECK(c, launch_hybrid_condition(c->stream, a, c->cfg.gamma));
+    HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+    HIPCHECK(c, launch_update(c->stream, a));
+    HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
+    HIPCHECK(c, hipEventSynchronize(c->ev[6]));
+    c->timings[3] = elapsed(c->ev[5], c->ev[6]);
+    for (int f = 0; f < nfield; ++f) {
+        HIPCHECK(c, launch_cells_to_members(c->stream, c->d_E.as<float>() + (size_t)f * ncell * m, c->d_tmp.as<float>(),
+                                            m, (long long)ncell));
+        HIPCHECK(c, hipMemcpyAsync(fb[f], c->d_tmp.p, ncell * m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    if (infl != nullptr)
+        HIPCHECK(c, hipMemcpy(infl, c->d_infl.p, (size_t)nfield * ncell * sizeof(float), hipMemcpyDeviceToHost));
+    return ENKFG_OK;
+}
+
+int enkfg_update_bg(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enkfg_field* fields, int j1, int j2,
+                    float** fb)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    int r = check_grid(c, g);
+    if (r != ENKFG_OK)
+        return r;
+    if (nfield <= 0 || fields == nullptr || fb == nullptr || j1 < 0 || j2 < j1 || j2 >= g->mnj || j1 % g->stride)
+        return fail(c, ENKFG_ERR_ARG, "invalid field batch");
+    r = upload_fields(c, g, nfield, fields);
+    if (r != ENKFG_OK)
+        return r;
+    const int m1 = c->cfg.nmem + 1;
+    const int jsize = j2 - j1 + 1;
+    const size_t ncell = (size_t)jsize * g->mni;
+    HIPCHECK(c, c->d_E.reserve((size_t)nfield * ncell * m1 * sizeof(float)));
+    HIPCHECK(c, c->d_tmp.reserve(ncell * m1 * sizeof(float)));
+    for (int f = 0; f < nfield; ++f) {
+        HIPCHECK(c, hipMemcpyAsync(c->d_tmp.p, fb[f], ncell * m1 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, launch_members_to_cells(c->stream, c->d_tmp.as<float>(),
+                                            c->d_E.as<float>() + (size_t)f * ncell * m1, m1, (long long)ncell));
+    }
+    UpdArgs a = upd_args(c, g, nfield, j1, jsize, 0);
+    a.E = c->d_E.as<float>();
+    HIPCHECK(c, launch_update_bg(c->stream, a));
+    for (int f = 0; f < nfield; ++f) {
+        HIPCHECK(c, launch_cells_to_members(c->stream, c->d_E.as<float>() + (size_t)f * ncell * m1,
+                                            c->d_tmp.as<float>(), m1, (long long)ncell));
+        HIPCHECK(c, hipMemcpyAsync(fb[f], c->d_tmp.p, ncell * m1 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    return ENKFG_OK;
+}
+
+int enkfg_ens_attach(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enkfg_field* fields, float* dev_E)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    int r = check_grid(c, g);
+    if (r != ENKFG_OK)
+        return r;
+    if (nfield <= 0 || fields == nullptr || dev_E == nullptr)
+        return fail(c, ENKFG_ERR_ARG, "invalid ensemble");
+    r = upload_fields(c, g, nfield, fields);
+    if (r != ENKFG_OK)
+        return r;
+    c->attached_E = dev_E;
+    c->att_grid = *g;
+    c->att_grid.nlevels = nullptr;
+    c->att_nfield = nfield;
+    c->att_fields.assign(fields, fields + nfield);
+    return ENKFG_OK;
+}
+
+int enkfg_ens_update(enkfg_ctx* c, int flags, float* dev_infl)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (c->attached_E == nullptr)
+        return fail(c, ENKFG_ERR_STATE, "no ensemble attached");
+    if (c->cfg.mode == ENKFG_MODE_ENOI)
+        return fail(c, ENKFG_ERR_ARG, "das_updatefields needs EnKF or hybrid mode");
+    UpdArgs a = upd_args(c, &c->att_grid, c->att_nfield, 0, c->att_grid.mnj, flags);
+    a.E = c->attached_E;
+    a.infl = dev_infl;
+    if ((flags & ENKFG_UPD_HYBRID_CONDITION) && c->cfg.mode == ENKFG_MODE_HYBRID)
+        HIPCHECK(c, launch_hybrid_condition(c->stream, a, c->cfg.gamma));
+    HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+    HIPCHECK(c, launch_update(c->stream, a));
+    HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
+    HIPCHECK(c, hipEventSynchronize(c->ev[6]));
+    c->timings[3] = elapsed(c->ev[5], c->ev[6]);
+    return ENKFG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// enkfg_internal.h -- device data structures and kernel launchers shared by
+// the host side (enkfg_abi.cpp) and the HIP kernels (*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "enkfg.h"
+
+namespace enkfg {
+
+// One kd-tree node, array-of-structures so that a visit touches one 64-byte
+// line.  The tree is the one obs_createkdtrees builds (sequential insertion,
+// common/kdtree.c:194-274); parent links allow a stackless depth-first walk.
+struct alignas(16) KdNode {
+    double x, y, z;   // coordinates (z unused for 2D trees)
+    int left, right;  // child node indices within the type's tree, -1 = none
+    int parent;       // -1 for the root
+    int dir;          // split dimension
+    int obsid;        // global observation index (obsids[data])
+    int pad;
+};
+static_assert(sizeof(KdNode) == 48, "KdNode layout");
+
+// Per-obstype search parameters (obstype fields of common/obstypes.h)
+struct TypeInfo {
+    int node0;        // first node of this type's tree in the node array
+    int nnode;        // nodes in the tree (== OK obs at build time)
+    int ndim;         // 3 geographic, 2 otherwise
+    int searchable;   // tree present and not stats-only
+    int nlocrad;
+    int nlobsmax;
+    int obsid0;       // first entry of this type in the obsids array
+    int nids;         // OK obs of this type at tree build (obs_getglobal list)
+    double range;     // obstype_getmaxlocrad()
+    double locrad[ENKFG_MAXLOCRAD];
+    double locweight[ENKFG_MAXLOCRAD];
+};
+
+// ---- launchers (defined in the .hip files); all return hipError_t
+
+// localisation (loc_kernels.hip)
+hipError_t launch_loc_count(hipStream_t s, int nnode, const double* qxyz, const unsigned char* isglobal,
+                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
+                            const short* status, int* counts);
+hipError_t launch_loc_fill(hipStream_t s, int nnode, const double* qxyz, const unsigned char* isglobal,
+                           const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
+                           const short* status, const int* obsids, const long long* offsets, int* ids,
+                           double* lc);
+hipError_t launch_scan_counts(hipStream_t s, const int* counts, long long* offsets, long long n, void* tmp,
+                              size_t* tmp_bytes);
+
+// standardisation and layout (calc_kernels.hip)
+hipError_t launch_standardise(hipStream_t s, int nmem, int nobs, const short* type, const float* estd,
+                              const double* sqrt_rfactor, float* S_obsmajor, double* sv);
+hipError_t launch_transpose_f32(hipStream_t s, const float* in, float* out, long long rows, long long cols);
+
+// per-column transforms (calc_kernels.hip)
+struct CalcArgs {
+    int m, md, mode, scheme, ntypes, pertype;
+    double alpha;
+    int node0, nbatch;             // batch of nodes [node0, node0 + nbatch)
+    int nseg;                      // segments per node in the Gram scratch (ntypes or 1)
+    const int* counts;             // [nnode][ntypes]
+    const long long* offsets;      // [nnode*ntypes + 1]
+    const int* ids;
+    const double* lc;
+    const float* S;                // [nobs][m]
+    const double* sv;              // [nobs]
+    double* gram;                  // [nbatch][nseg][mp][mp]
+    double* sts;                   // [nbatch][nseg][mp]
+    float* T;                      // [nnode][md][m]
+    float* w;                      // [nnode][m]
+    double* T64;                   // optional
+    double* w64;                   // optional
+    int* pnlobs;                   // [ntypes+1][nnode]
+    float* pdfs;
+    float* psrf;
+    int nnode;
+    int* err;                      // device error flag
+};
+hipError_t launch_gram(hipStream_t s, const CalcArgs& a);
+hipError_t launch_factor(hipStream_t s, const CalcArgs& a);
+
+// update (update_kernels.hip)
+struct UpdArgs {
+    int m, md, mode;
+    int mni, mnj, stride, periodic_i, surfk;
+    int ni_s, nj_s;
+    int j1, jsize;                 // rows j1 .. j1+jsize-1 of the grid
+    int nfield;
+    int outputinc;
+    const int* nlevels;            // [mnj][mni]
+    const float* T;                // [nj_s][ni_s][md][m]
+    const float* w;                // [nj_s][ni_s][m]
+    const int* flevel;             // [nfield]
+    const int* fsurf;
+    const int* fapplylog;
+    const float* finfl;
+    const double* finfratio;
+    float* E;                      // [nfield][ncell][m], cell = (j - j1) * mni + i
+    long long ncell;               // cells per field in E
+    float* infl;                   // optional [nfield][ncell]
+};
+hipError_t launch_update(hipStream_t s, const UpdArgs& a);
+hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a);
+hipError_t launch_hybrid_condition(hipStream_t s, const UpdArgs& a, double gamma);
+// [nmem][rows][cols] <-> [rows*cols][nmem]
+hipError_t launch_members_to_cells(hipStream_t s, const float* in, float* out, int nmem, long long ncell);
+hipError_t launch_cells_to_members(hipStream_t s, const float* in, float* out, int nmem, long long ncell);
+
+}  // namespace enkfg

@@ -1,0 +1,307 @@
+// loc_kernels.hip -- GPU localisation: for every node of the stride grid, the
+// ordered list of (obs id, taper coefficient) that obs_findlocal
+// (common/observations.c:1837-1927) returns.
+//
+// Each thread walks the per-type kd-trees built on the host by sequential
+// insertion (the trees of obs_createkdtrees).  The walk is the preorder of
+// _kd_findnodeswithinrange (common/kdtree.c:516-540) -- node, then the child
+// on the query's side (left iff dx <= 0), then the other child iff |dx| <
+// range -- done without a stack by following parent links: on the way up, a
+// node is re-examined to decide whether the far child is still to be
+// visited, which depends only on that node and the query.  Lists are
+// therefore identical, element for element, to the reference's, including
+// their order.  Squared distances, tapers and the inclusion test use the
+// reference's operation order in IEEE double with no contraction (the
+// library is compiled with -ffp-contract=off), so the taper bits match too.
+//
+// Two passes: count (per node and type), exclusive scan, fill.
+#include "enkfg_internal.h"
+
+namespace enkfg {
+
+namespace {
+
+constexpr int kStatusOK = 0;
+
+__device__ inline double taper_gc(double x)
+{
+    // common/utils.c:1425-1443
+    if (x >= 1.0)
+        return 0.0;
+    x *= 2.0;
+    double x2 = x * x;
+    double x3 = x2 * x;
+    double f;
+    if (x < 1.0)
+        f = 1.0 + x2 * (-x3 / 4.0 + x2 / 2.0) + x3 * (5.0 / 8.0) - x2 * (5.0 / 3.0);
+    else
+        f = x2 * (x3 / 12.0 - x2 / 2.0) + x3 * (5.0 / 8.0) + x2 * (5.0 / 3.0) - x * 5.0 + 4.0 - (2.0 / 3.0) / x;
+    return (f >= 0.0) ? f : 0.0;
+}
+
+__device__ inline double calclcoeff(const TypeInfo& t, double dist)
+{
+    // common/obstypes.c:800-810
+    double sum = 0.0;
+    for (int i = 0; i < t.nlocrad; ++i)
+        if (dist <= t.locrad[i])
+            sum += t.locweight[i] * taper_gc(dist / t.locrad[i]);
+    return sum;
+}
+
+__device__ inline double coord(const KdNode& n, int d)
+{
+    return d == 0 ? n.x : (d == 1 ? n.y : n.z);
+}
+
+// Depth-first range search over one tree; calls emit(obsid, distsq) for every
+// hit in the reference's visiting order.
+template <class Emit>
+__device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, const double* q, double range, Emit&& emit)
+{
+    const double r2 = range * range;
+    int node = 0;
+    int from = -1;
+    bool down = true;
+
+    while (node >= 0) {
+        const KdNode nd = tree[node];
+        const double c = coord(nd, nd.dir);
+        const double dx = q[nd.dir] - c;
+        const int near = (dx <= 0.0) ? nd.left : nd.right;
+        const int far = (dx <= 0.0) ? nd.right : nd.left;
+        if (down) {
+            double distsq = 0.0;
+            for (int i = 0; i < ndim; i++) {
+                double d = coord(nd, i) - q[i];
+                distsq += d * d;
+            }
+            if (distsq <= r2)
+                emit(nd.obsid, distsq);
+            if (near >= 0) {
+                from = node;
+                node = near;
+                continue;
+            }
+            if (fabs(dx) < range && far >= 0) {
+                from = node;
+                node = far;
+                continue;
+            }
+            from = node;
+            node = nd.parent;
+            down = false;
+        } else {
+            // arrived from child `from`
+            if (from == near && fabs(dx) < range && far >= 0) {
+                from = node;
+                node = far;
+                down = true;
+                continue;
+            }
+            from = node;
+            node = nd.parent;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(128) loc_count_kernel(int nnode, const double* __restrict__ qxyz,
+                                                        const unsigned char* __restrict__ isglobal,
+                                                        const unsigned char* __restrict__ visible,
+                                                        const KdNode* __restrict__ nodes,
+                                                        const TypeInfo* __restrict__ types, int ntypes,
+                                                        const short* __restrict__ status, int* __restrict__ counts)
+{
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nnode)
+        return;
+    double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
+    const bool glob = isglobal[node] != 0;
+    for (int t = 0; t < ntypes; ++t) {
+        const TypeInfo ti = types[t];
+        int n = 0;
+        if (ti.searchable && (visible == nullptr || visible[t])) {
+            if (glob) {
+                n = ti.nids;   // obs_getglobal: all obs of the type, no status check
+            } else if (ti.nnode > 0) {
+                range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int obsid, double) {
+                    if (status[obsid] == kStatusOK)
+                        n++;
+                });
+            }
+        }
+        counts[(size_t)node * ntypes + t] = n;
+    }
+}
+
+__global__ void __launch_bounds__(128) loc_fill_kernel(int nnode, const double* __restrict__ qxyz,
+                                                       const unsigned char* __restrict__ isglobal,
+                                                       const unsigned char* __restrict__ visible,
+                                                       const KdNode* __restrict__ nodes,
+                                                       const TypeInfo* __restrict__ types, int ntypes,
+                                                       const short* __restrict__ status,
+                                                       const int* __restrict__ obsids,
+                                                       const long long* __restrict__ offsets, int* __restrict__ ids,
+                                                       double* __restrict__ lc)
+{
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nnode)
+        return;
+    double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
+    const bool glob = isglobal[node] != 0;
+    for (int t = 0; t < ntypes; ++t) {
+        const TypeInfo& ti = types[t];
+        if (!ti.searchable || !(visible == nullptr || visible[t]))
+            continue;
+        long long o = offsets[(size_t)node * ntypes + t];
+        if (glob) {
+            for (int i = 0; i < ti.nids; ++i) {
+                ids[o + i] = obsids[ti.obsid0 + i];
+                lc[o + i] = 1.0;
+            }
+        } else if (ti.nnode > 0) {
+            range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int obsid, double distsq) {
+                if (status[obsid] == kStatusOK) {
+                    ids[o] = obsid;
+                    lc[o] = calclcoeff(ti, sqrt(distsq));
+                    o++;
+                }
+            });
+        }
+    }
+}
+
+// ---- exclusive scan of int counts into long long offsets (n + 1 entries)
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__device__ inline long long block_exclusive_scan(long long v, long long* sh, long long* total)
+{
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        long long add = (tid >= off) ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += add;
+        __syncthreads();
+    }
+    long long incl = sh[tid];
+    *total = sh[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_partials_kernel(const int* __restrict__ in, long long n,
+                                                                    long long* __restrict__ partials)
+{
+    __shared__ long long sh[kScanThreads];
+    const long long base = (long long)blockIdx.x * kScanChunk;
+    long long sum = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        long long i = base + (long long)threadIdx.x * kScanItems + k;
+        if (i < n)
+            sum += in[i];
+    }
+    long long total;
+    block_exclusive_scan(sum, sh, &total);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_top_kernel(long long* __restrict__ partials, int nb)
+{
+    __shared__ long long sh[kScanThreads];
+    long long carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
+        int i = b0 + threadIdx.x;
+        long long v = (i < nb) ? partials[i] : 0;
+        long long total;
+        long long ex = block_exclusive_scan(v, sh, &total);
+        if (i < nb)
+            partials[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+        partials[nb] = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_final_kernel(const int* __restrict__ in, long long n,
+                                                                 const long long* __restrict__ partials,
+                                                                 long long* __restrict__ out, int nb)
+{
+    __shared__ long long sh[kScanThreads];
+    const long long base = (long long)blockIdx.x * kScanChunk;
+    long long vals[kScanItems];
+    long long sum = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        long long i = base + (long long)threadIdx.x * kScanItems + k;
+        vals[k] = (i < n) ? in[i] : 0;
+        sum += vals[k];
+    }
+    long long total;
+    long long ex = block_exclusive_scan(sum, sh, &total) + partials[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        long long i = base + (long long)threadIdx.x * kScanItems + k;
+        if (i < n)
+            out[i] = ex;
+        ex += vals[k];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0)
+        out[n] = partials[nb];
+}
+
+}  // namespace
+
+hipError_t launch_loc_count(hipStream_t s, int nnode, const double* qxyz, const unsigned char* isglobal,
+                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
+                            const short* status, int* counts)
+{
+    if (nnode <= 0)
+        return hipSuccess;
+    dim3 grid((nnode + 127) / 128);
+    hipLaunchKernelGGL(loc_count_kernel, grid, dim3(128), 0, s, nnode, qxyz, isglobal, visible, nodes, types,
+                       ntypes, status, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_loc_fill(hipStream_t s, int nnode, const double* qxyz, const unsigned char* isglobal,
+                           const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
+                           const short* status, const int* obsids, const long long* offsets, int* ids, double* lc)
+{
+    if (nnode <= 0)
+        return hipSuccess;
+    dim3 grid((nnode + 127) / 128);
+    hipLaunchKernelGGL(loc_fill_kernel, grid, dim3(128), 0, s, nnode, qxyz, isglobal, visible, nodes, types,
+                       ntypes, status, obsids, offsets, ids, lc);
+    return hipGetLastError();
+}
+
+// tmp must hold (nb + 1) long longs, nb = ceil(n / 2048); with tmp == nullptr
+// the required size is returned in *tmp_bytes.
+hipError_t launch_scan_counts(hipStream_t s, const int* counts, long long* offsets, long long n, void* tmp,
+                              size_t* tmp_bytes)
+{
+    const long long nb = (n + kScanChunk - 1) / kScanChunk;
+    const size_t need = (size_t)(nb + 1) * sizeof(long long);
+    if (tmp == nullptr) {
+        *tmp_bytes = need;
+        return hipSuccess;
+    }
+    if (*tmp_bytes < need)
+        return hipErrorInvalidValue;
+    long long* partials = static_cast<long long*>(tmp);
+    if (n == 0) {
+        // offsets[0] = 0 via the top kernel with nb = 0
+        hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanThreads), 0, s, offsets, 0);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(scan_partials_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, counts, n, partials);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanThreads), 0, s, partials, (int)nb);
+    hipLaunchKernelGGL(scan_final_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, counts, n, partials, offsets,
+                       (int)nb);
+    return hipGetLastError();
+}
+
+}  // namespace enkfg

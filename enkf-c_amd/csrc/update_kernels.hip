@@ -1,0 +1,427 @@
+// update_kernels.hip -- das_updatefields (update/update.c:92-564),
+// das_updatebg (update.c:568-814) and das_sethybridensemble
+// (common/dasystem.c:775-819) on a device-resident ensemble.
+//
+// Layout: E[field][cell][member], cell = j * mni + i (members contiguous, so
+// one cell of one field is a 4*m-byte contiguous record).  One block of
+// 64*ceil(m/64) threads handles one grid cell for all fields: thread e owns
+// member e.  The transform for the cell is interpolated from the stride grid
+// exactly as update.c:195-272 does (FP32, j first then i, no contraction),
+// once per cell, and kept in registers (thread e1 holds row e1 of T); it is
+// then applied to every field of the column.  The per-cell arithmetic follows
+// update.c:282-527: FP64 ensemble mean and increment sums, FP32 anomalies and
+// transform product, capped inflation, masks and missing values.
+#include "enkfg_internal.h"
+
+namespace enkfg {
+
+namespace {
+
+constexpr float kMaxObsVal = 999999.0f;   // MAXOBSVAL, common/global.h:114
+constexpr float kEpsF = 1.0e-6f;          // EPSF, update/update.c:41
+
+// cell index for block b: each of the 8 XCDs (blocks are dispatched to XCDs
+// round-robin) gets a contiguous range of cells, so neighbouring cells, which
+// share stride-grid transforms, hit the same L2.
+__device__ inline long long xcd_cell(long long ncell)
+{
+    const long long chunk = (ncell + 7) / 8;
+    return (long long)(blockIdx.x % 8) * chunk + blockIdx.x / 8;
+}
+
+template <int NW>
+struct RedShared {
+    double part[NW][2];
+};
+
+// deterministic block sum of two doubles (each wave reduces, then waves in order)
+template <int NW>
+__device__ inline void block_sum2(double& a, double& b, RedShared<NW>& sh)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        sh.part[w][0] = a;
+        sh.part[w][1] = b;
+    }
+    __syncthreads();
+    a = 0.0;
+    b = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        a += sh.part[k][0];
+        b += sh.part[k][1];
+    }
+}
+
+// bilinear stride-grid interpolation of one length-n row r of the node data
+// (rows of `stride_elems` floats per node) into out[0..n), update.c:195-272
+struct Interp {
+    int jj, jj2, ii, ii2;
+    bool jcopy, icopy;
+    float wj1, wj2, wi1, wi2;
+};
+
+__device__ inline Interp make_interp(int j, int i, int stride, int nj_s, int ni_s, int periodic_i)
+{
+    Interp p;
+    p.jj = j / stride;
+    const int stepj = j % stride;
+    p.ii = i / stride;
+    const int stepi = i % stride;
+    p.jj2 = (p.jj < nj_s - 1) ? p.jj + 1 : p.jj;
+    p.ii2 = (p.ii < ni_s - 1) ? p.ii + 1 : (periodic_i ? (p.ii + 1) % ni_s : p.ii);
+    p.jcopy = (stride == 1 || stepj == 0);
+    p.icopy = (stride == 1 || stepi == 0);
+    p.wj2 = (float)stepj / (float)stride;
+    p.wj1 = 1.0f - p.wj2;
+    p.wi2 = (float)stepi / (float)stride;
+    p.wi1 = 1.0f - p.wi2;
+    return p;
+}
+
+// value k of the interpolated record from node records base[(jj*ni_s+ii)*rec]
+__device__ inline float interp_at(const float* __restrict__ base, long long rec, int ni_s, const Interp& p, int k)
+{
+    const float a00 = base[((long long)p.jj * ni_s + p.ii) * rec + k];
+    float ta;
+    if (p.jcopy)
+        ta = a00;
+    else {
+        const float a10 = base[((long long)p.jj2 * ni_s + p.ii) * rec + k];
+        ta = a00 * p.wj1 + a10 * p.wj2;
+    }
+    if (p.icopy)
+        return ta;
+    const float a01 = base[((long long)p.jj * ni_s + p.ii2) * rec + k];
+    float tb;
+    if (p.jcopy)
+        tb = a01;
+    else {
+        const float a11 = base[((long long)p.jj2 * ni_s + p.ii2) * rec + k];
+        tb = a01 * p.wj1 + a11 * p.wj2;
+    }
+    return ta * p.wi1 + tb * p.wi2;
+}
+
+__device__ inline bool masked(int nl, int level, int issurf, int surfk, int surf_thresh)
+{
+    if (surfk == 0)
+        return nl <= level;
+    return nl <= (issurf ? surf_thresh : surfk - level);
+}
+
+template <int MT>
+__global__ void __launch_bounds__(64 * ((16 * MT + 63) / 64)) update_kernel(UpdArgs a)
+{
+    constexpr int MP = 16 * MT;
+    constexpr int NW = (MP + 63) / 64;
+    __shared__ float vfs[64 * NW];
+    __shared__ RedShared<NW> red;
+    __shared__ int badmin;
+
+    const long long cell = xcd_cell(a.ncell);
+    if (cell >= a.ncell)
+        return;
+    const int jl = (int)(cell / a.mni);
+    const int i = (int)(cell % a.mni);
+    const int j = a.j1 + jl;
+    const int e = threadIdx.x;
+    const int m = a.m, md = a.md;
+    const Interp ip = make_interp(j, i, a.stride, a.nj_s, a.ni_s, a.periodic_i);
+
+    // row e of the cell's transform (T is [node][md][m]) and w[e]
+    float trow[MP];
+    if (e < md) {
+        const float* base = a.T + (long long)e * m;
+#pragma unroll
+        for (int k = 0; k < MP; ++k)
+            trow[k] = (k < m) ? interp_at(base, (long long)md * m, a.ni_s, ip, k) : 0.0f;
+    }
+    const float wv = (e < m) ? interp_at(a.w, m, a.ni_s, ip, e) : 0.0f;
+    const int nl = a.nlevels[(long long)j * a.mni + i];
+    const double k_d = (md > 1) ? sqrt((double)(m - 1) / (double)(md - 1)) : 1.0;
+
+    for (int f = 0; f < a.nfield; ++f) {
+        float* rec = a.E + ((long long)f * a.ncell + cell) * m;
+        float* inflo = (a.infl != nullptr) ? a.infl + (long long)f * a.ncell + cell : nullptr;
+        if (masked(nl, a.flevel[f], a.fsurf[f], a.surfk, 0)) {
+            if (e < m)
+                rec[e] = 0.0f;
+            if (inflo != nullptr && e == 0)
+                *inflo = 0.0f;
+            continue;
+        }
+        float v = (e < m) ? rec[e] : 0.0f;
+        __syncthreads();   // the previous field's reads of badmin / vfs are done
+        if (e == 0)
+            badmin = m;
+        __syncthreads();
+        if (e < m && (!isfinite(v) || fabsf(v) > kMaxObsVal))
+            atomicMin(&badmin, e);
+        __syncthreads();
+        const int eb = badmin;
+        if (eb < md) {
+            if (e < m)
+                rec[e] = 0.0f;
+            if (inflo != nullptr && e == 0)
+                *inflo = 0.0f;
+            continue;
+        } else if (eb < m) {
+            if (e >= md && e < m) {
+                rec[e] = 0.0f;
+                v = 0.0f;
+            }
+            if (md == 0)
+                continue;
+        }
+        // x^f: FP64 mean over the dynamic members
+        double s1 = (e < md) ? (double)v : 0.0, s2 = 0.0;
+        block_sum2<NW>(s1, s2, red);
+        const double v1_f = s1 / (double)md;
+        float vf = (e < m) ? (float)((double)v - v1_f) : 0.0f;
+        vfs[e] = vf;
+        __syncthreads();
+        // A^a = A^f T (sgemv('T', m, md, 1, T, m, v_f, ...), update.c:399)
+        float va = 0.0f;
+        if (e < md) {
+#pragma unroll
+            for (int k = 0; k < MP; ++k)
+                va = fmaf(trow[k], vfs[k], va);
+        }
+        // x^a = x^f + A^f w (FP32 products, FP64 sum)
+        double d1 = (e < m) ? (double)(vf * wv) : 0.0, d2 = 0.0;
+        block_sum2<NW>(d1, d2, red);
+        const double v1_a = d1 + v1_f;
+        if (a.mode == ENKFG_MODE_HYBRID && e < md) {
+            va = (float)((double)va / k_d);
+            vf = (float)((double)vf / k_d);
+        }
+        const float inflation0 = a.finfl[f];
+        const double inf_ratio = a.finfratio[f];
+        float inflation = inflation0;
+        if (!isnan(inf_ratio)) {
+            double q1 = (e < md) ? (double)(vf * vf) : 0.0;
+            double q2 = (e < md) ? (double)(va * va) : 0.0;
+            block_sum2<NW>(q1, q2, red);
+            if (q2 > 0.0) {
+                inflation = (float)(sqrt(q1 / q2) * inf_ratio + 1.0 - inf_ratio);
+                if (inflation >= inflation0)
+                    inflation = inflation0;
+            }
+        }
+        if (inflation - 1.0f > kEpsF)
+            va = va * inflation;
+        else
+            inflation = 1.0f;
+        if (e < md) {
+            va = (float)((double)va + v1_a);
+            vf = (float)((double)vf + v1_f);
+            float out;
+            if (!a.outputinc)
+                out = va;
+            else if (!a.fapplylog[f])
+                out = va - vf;
+            else
+                out = isnormal(vf) ? (float)(exp10((double)va) - exp10((double)vf)) : 0.0f;
+            rec[e] = out;
+        }
+        if (inflo != nullptr && e == 0)
+            *inflo = inflation;
+    }
+}
+
+// das_updatebg: E is [field][cell][m + 1], member m the background
+template <int NWT>
+__global__ void __launch_bounds__(64 * NWT) update_bg_kernel(UpdArgs a)
+{
+    __shared__ float xs[64 * NWT];
+    __shared__ float xmean_s;
+    __shared__ int bad;
+    const long long cell = xcd_cell(a.ncell);
+    if (cell >= a.ncell)
+        return;
+    const int jl = (int)(cell / a.mni);
+    const int i = (int)(cell % a.mni);
+    const int j = a.j1 + jl;
+    const int e = threadIdx.x;
+    const int m = a.m;
+    const Interp ip = make_interp(j, i, a.stride, a.nj_s, a.ni_s, a.periodic_i);
+    const float wv = (e < m) ? interp_at(a.w, m, a.ni_s, ip, e) : 0.0f;
+    const int nl = a.nlevels[(long long)j * a.mni + i];
+    for (int f = 0; f < a.nfield; ++f) {
+        float* rec = a.E + ((long long)f * a.ncell + cell) * (m + 1);
+        // the surface threshold is 1 here (update.c:730), 0 in das_updatefields
+        if (masked(nl, a.flevel[f], a.fsurf[f], a.surfk, 1)) {
+            if (e == 0)
+                rec[m] = 0.0f;
+            continue;
+        }
+        const float bg = rec[m];
+        if (fabsf(bg) > kMaxObsVal) {
+            if (e == 0)
+                rec[m] = 0.0f;
+            continue;
+        }
+        const float v = (e < m) ? rec[e] : 0.0f;
+        __syncthreads();   // the previous field's reads of shared data are done
+        if (e == 0)
+            bad = 0;
+        __syncthreads();
+        if (e < m && fabsf(v) > kMaxObsVal)
+            bad = 1;
+        xs[e] = v;
+        __syncthreads();
+        if (bad) {
+            if (e == 0)
+                rec[m] = 0.0f;
+            continue;
+        }
+        // float mean and float accumulation in member order (update.c:751-761)
+        if (e == 0) {
+            float xm = 0.0f;
+            for (int k = 0; k < m; ++k)
+                xm += xs[k];
+            xm /= (float)m;
+            xmean_s = xm;
+        }
+        __syncthreads();
+        const float xmean = xmean_s;
+        const float contrib = (e < m) ? (v - xmean) * wv : 0.0f;
+        __syncthreads();
+        xs[e] = contrib;
+        __syncthreads();
+        if (e == 0) {
+            float b = bg;
+            for (int k = 0; k < m; ++k)
+                b += xs[k];
+            rec[m] = b;
+        }
+    }
+}
+
+// das_sethybridensemble on E[field][cell][m]
+__global__ void hybrid_condition_kernel(UpdArgs a, double k_d, double k_s)
+{
+    const long long n = (long long)a.nfield * a.ncell;
+    for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
+        float* rec = a.E + r * a.m;
+        double mean = 0.0;
+        for (int e = 0; e < a.md; ++e)
+            mean += (double)rec[e];
+        mean /= (double)a.md;
+        for (int e = 0; e < a.md; ++e)
+            rec[e] = (float)(((double)rec[e] - mean) * k_d + mean);
+        for (int e = a.md; e < a.m; ++e)
+            rec[e] = (float)((double)rec[e] * k_s + mean);
+    }
+}
+
+__global__ void transpose_tiles_kernel(const float* __restrict__ in, float* __restrict__ out, long long rows,
+                                       long long cols)
+{
+    __shared__ float tile[32][33];
+    const long long ntc = (cols + 31) / 32;
+    const long long t = blockIdx.x;
+    const long long r0 = (t / ntc) * 32, c0 = (t % ntc) * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        long long r = r0 + k, c = c0 + threadIdx.x;
+        if (r < rows && c < cols)
+            tile[k][threadIdx.x] = in[r * cols + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        long long c = c0 + k, r = r0 + threadIdx.x;
+        if (r < rows && c < cols)
+            out[c * rows + r] = tile[threadIdx.x][k];
+    }
+}
+
+hipError_t transpose(hipStream_t s, const float* in, float* out, long long rows, long long cols)
+{
+    if (rows == 0 || cols == 0)
+        return hipSuccess;
+    const long long ntiles = ((rows + 31) / 32) * ((cols + 31) / 32);
+    if (ntiles > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(transpose_tiles_kernel, dim3((unsigned)ntiles), dim3(32, 8), 0, s, in, out, rows, cols);
+    return hipGetLastError();
+}
+
+unsigned cell_blocks(long long ncell)
+{
+    return (unsigned)(8 * ((ncell + 7) / 8));
+}
+
+}  // namespace
+
+hipError_t launch_update(hipStream_t s, const UpdArgs& a)
+{
+    if (a.ncell == 0 || a.nfield == 0)
+        return hipSuccess;
+    const int MT = (a.m + 15) / 16;
+    const dim3 grid(cell_blocks(a.ncell));
+    switch (MT) {
+    case 1: hipLaunchKernelGGL(update_kernel<1>, grid, dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(update_kernel<2>, grid, dim3(64), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(update_kernel<3>, grid, dim3(64), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(update_kernel<4>, grid, dim3(64), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(update_kernel<5>, grid, dim3(128), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(update_kernel<6>, grid, dim3(128), 0, s, a); break;
+    case 7: hipLaunchKernelGGL(update_kernel<7>, grid, dim3(128), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(update_kernel<8>, grid, dim3(128), 0, s, a); break;
+    case 9: hipLaunchKernelGGL(update_kernel<9>, grid, dim3(192), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(update_kernel<10>, grid, dim3(192), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a)
+{
+    if (a.ncell == 0 || a.nfield == 0)
+        return hipSuccess;
+    const dim3 grid(cell_blocks(a.ncell));
+    if (a.m < 64)
+        hipLaunchKernelGGL(update_bg_kernel<1>, grid, dim3(64), 0, s, a);
+    else if (a.m < 128)
+        hipLaunchKernelGGL(update_bg_kernel<2>, grid, dim3(128), 0, s, a);
+    else if (a.m < 192)
+        hipLaunchKernelGGL(update_bg_kernel<3>, grid, dim3(192), 0, s, a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_hybrid_condition(hipStream_t s, const UpdArgs& a, double gamma)
+{
+    const int ms = a.m - a.md;
+    if (ms < 2 || a.md < 1)
+        return hipErrorInvalidValue;
+    const double k_d = (a.md > 1) ? sqrt((double)(a.m - 1) / (double)(a.md - 1)) : 0.0;
+    const double k_s = sqrt(gamma * (double)(a.m - 1) / (double)(ms - 1));
+    const long long n = (long long)a.nfield * a.ncell;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536)
+        nb = 65536;
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(hybrid_condition_kernel, dim3((unsigned)nb), dim3(256), 0, s, a, k_d, k_s);
+    return hipGetLastError();
+}
+
+hipError_t launch_members_to_cells(hipStream_t s, const float* in, float* out, int nmem, long long ncell)
+{
+    return transpose(s, in, out, nmem, ncell);
+}
+
+hipError_t launch_cells_to_members(hipStream_t s, const float* in, float* out, int nmem, long long ncell)
+{
+    return transpose(s, in, out, ncell, nmem);
+}
+
+}  // namespace enkfg

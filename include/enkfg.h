@@ -1,0 +1,219 @@
+/*
+ * enkfg.h -- C ABI of the MI355X (gfx950) EnKF-C analysis engine.
+ *
+ * The engine replaces two seams of enkf-c v2.47.3:
+ *   calc:   void das_calctransforms(dasystem* das)      calc/transforms.c:605
+ *           (declared common/dasystem.h:143), which in turn calls
+ *           obs_findlocal (common/observations.c:1837), kd_findnodeswithinrange
+ *           (common/kdtree.c:558) and calc_wT_denkf / calc_wT_etkf / calc_G /
+ *           calc_w / traceprod (calc/calcs.h:18-22);
+ *   update: static void das_updatefields(dasystem*, int nfield,
+ *           void** fieldbuffer, field fields[])          update/update.c:92
+ *           and das_updatebg (update/update.c:568), das_sethybridensemble
+ *           (common/dasystem.c:775).
+ *
+ * Conventions
+ *   - Every call returns an int status, ENKFG_OK (0) on success; the engine
+ *     never aborts.  enkfg_last_error() describes the last failure.  Where the
+ *     reference calls enkf_quit() (LAPACK info != 0 at calc/calcs.c:110-118,
+ *     339-343; stride mismatch at update/update.c:152-158) the engine returns
+ *     ENKFG_ERR_LINALG / ENKFG_ERR_ARG instead.
+ *   - The caller owns host buffers; the engine owns its device buffers, except
+ *     buffers passed through an *_attach call, which stay caller-owned.
+ *   - One host thread per context; each context has one HIP stream.  All
+ *     calls are synchronous unless named *_async.
+ *   - Array layouts follow the reference: S is [nmem][nobs] (das->S,
+ *     common/dasystem.h:95), T is [node][nmem_dynamic][nmem] and w is
+ *     [node][nmem] (the "T" and "w" variables of transforms.nc,
+ *     calc/transforms.c:54-94), field buffers are [nmem][jsize][mni]
+ *     (update/update.c:1764-1797).
+ */
+#ifndef ENKFG_H
+#define ENKFG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes */
+#define ENKFG_OK 0
+#define ENKFG_ERR_ARG 1          /* invalid argument or inconsistent sizes */
+#define ENKFG_ERR_HIP 2          /* HIP runtime error */
+#define ENKFG_ERR_NOMEM 3        /* device or host allocation failed */
+#define ENKFG_ERR_LINALG 4       /* factorisation failed (non-SPD matrix) */
+#define ENKFG_ERR_STATE 5        /* call out of order (e.g. no obs set) */
+#define ENKFG_ERR_UNSUPPORTED 6  /* valid input this engine does not handle */
+#define ENKFG_ERR_NONFINITE 7    /* non-finite innovation (calc/ensobs.c:609) */
+
+/* ---- modes and schemes: values of common/global.h:64-73 */
+#define ENKFG_MODE_ENKF 1
+#define ENKFG_MODE_ENOI 2
+#define ENKFG_MODE_HYBRID 3
+#define ENKFG_SCHEME_DENKF 1
+#define ENKFG_SCHEME_ETKF 2
+
+/* ---- arithmetic of the calc and update kernels */
+#define ENKFG_PREC_FP64 0        /* FP64 algebra on FP32 S, FP32/FP64 update */
+#define ENKFG_PREC_MIXED 1       /* reserved: BF16/FP16 MFMA products */
+
+/* ---- config flags */
+#define ENKFG_FLAG_PERTYPE_DIAG 1  /* per-obstype dfs/srf as the default
+                                      USE_MPIQUEUE build computes them
+                                      (calc/transforms.c:853-879) */
+#define ENKFG_FLAG_KEEP_FP64 2     /* keep pre-cast FP64 T and w */
+
+#define ENKFG_NLOBSMAX_NONE 2147483647
+#define ENKFG_MAXLOCRAD 8
+#define ENKFG_STATUS_OK 0
+
+typedef struct enkfg_ctx enkfg_ctx;
+
+/* dasystem fields used on the path (common/dasystem.h:48-127) */
+typedef struct {
+    int mode;            /* ENKFG_MODE_* (das->mode) */
+    int scheme;          /* ENKFG_SCHEME_* (das->scheme) */
+    int precision;       /* ENKFG_PREC_* */
+    int nmem;            /* das->nmem */
+    int nmem_dynamic;    /* das->nmem_dynamic (== nmem unless hybrid) */
+    double alpha;        /* relaxation to prior (das->alpha, ALPHA) */
+    double gamma;        /* hybrid static weight (das->gamma, GAMMA) */
+    int device;          /* HIP device ordinal */
+    int flags;           /* ENKFG_FLAG_* */
+} enkfg_config;
+
+/* obstype fields used on the path (common/obstypes.h:29-118) */
+typedef struct {
+    int nlocrad;
+    double locrad[ENKFG_MAXLOCRAD];     /* km */
+    double locweight[ENKFG_MAXLOCRAD];  /* normalised to sum 1 (obstypes.c:533-541) */
+    double rfactor;                     /* already multiplied by RFACTOR base */
+    int geographic;                     /* 3D chord coordinates if non-zero */
+    int statsonly;
+    int nlobsmax;                       /* ENKFG_NLOBSMAX_NONE = no limit */
+} enkfg_obstype;
+
+/* the stride sub-grid of one model grid (calc/transforms.c:665-684) */
+typedef struct {
+    int ni, nj;          /* ni_s = (mni-1)/stride+1, nj_s likewise */
+    int stride;
+    int geographic;      /* grid_isgeographic() */
+    const double* lon;   /* [nj][ni] node longitudes (grid_ij2xy), NaN = global */
+    const double* lat;   /* [nj][ni] node latitudes */
+    const unsigned char* visible;  /* [ntypes] 1 if the obstype is visible from
+                                      this grid's domain, NULL = all */
+} enkfg_nodes;
+
+/* calcstats, calc/transforms.c:44-50 */
+typedef struct {
+    long long nlobs_sum;
+    int nlobs_max;
+    int n_inv_obs;
+    int n_inv_ens;
+    int ncell;
+} enkfg_stats;
+
+/* model grid as the update sees it (update/update.c:96-104) */
+typedef struct {
+    int mni, mnj;        /* model grid size (grid_getsize) */
+    int stride;          /* grid_getstride(); must match the transforms */
+    int periodic_i;      /* grid_isperiodic_i() */
+    int surfk;           /* grid_getsurflayerid() */
+    const int* nlevels;  /* [mnj][mni] grid_getnumlevels() */
+} enkfg_grid;
+
+/* one horizontal field (common/dasystem.h:36-45 plus model variable data) */
+typedef struct {
+    int level;           /* field level */
+    int issurfacevar;
+    float inflation;     /* model_getvarinflation(): inflation0 */
+    double inf_ratio;    /* NaN = PLAIN inflation */
+    int applylog;        /* model_getvarislog() */
+} enkfg_field;
+
+/* update flags (common/global.h:92-104) */
+#define ENKFG_UPD_OUTPUTINC 32   /* UPDATE_OUTPUTINC */
+#define ENKFG_UPD_HYBRID_CONDITION 1024  /* apply das_sethybridensemble first */
+
+/* ---------------------------------------------------------------- context */
+int enkfg_create(enkfg_ctx** ctx, const enkfg_config* cfg);
+void enkfg_destroy(enkfg_ctx* ctx);
+const char* enkfg_last_error(const enkfg_ctx* ctx);
+int enkfg_sync(enkfg_ctx* ctx);
+
+/* ---------------------------------------------------------------- calc inputs */
+/* obstypes_set() output; replaces the das->obs->obstypes array. */
+int enkfg_set_obstypes(enkfg_ctx* ctx, int ntypes, const enkfg_obstype* types);
+
+/* obs_read() output plus obs_createkdtrees() (observations.c:1616): per
+ * non-stats-only obstype, obs with status OK are inserted, in ascending index
+ * order, into a kd-tree over 3D chord coordinates (geographic types) or
+ * (lon, lat).  lon/lat are the float obs coordinates of observations.h:67-68. */
+int enkfg_set_obs(enkfg_ctx* ctx, int nobs, const short* type, const short* status, const float* lon, const float* lat, const float* estd);
+
+/* Search-time obs status (observations.c:1906): obs marked bad after the trees
+ * were built (e.g. by obs_markbadbatches) are skipped by the local search. */
+int enkfg_set_obs_status(enkfg_ctx* ctx, const short* status);
+
+/* das->S ([nmem][nobs], float) and das->s_f ([nobs], double).  With
+ * standardised == 0 they are HA^f and the innovations, and the engine applies
+ * das_standardise (calc/ensobs.c:849-898) on the device. */
+int enkfg_set_S(enkfg_ctx* ctx, const float* S, const double* s, int standardised);
+
+/* The standardised S and s_f (the reference standardises in place; callers
+ * such as das_updateHE read them back). */
+int enkfg_get_S(enkfg_ctx* ctx, float* S, double* s);
+
+/* ---------------------------------------------------------------- calc */
+/* das_calctransforms for one grid: local search, gather and taper, per-column
+ * transform, relaxation to prior, FP32 cast, diagnostics.  Results stay on
+ * the device until fetched with enkfg_get_*. */
+int enkfg_calc_transforms(enkfg_ctx* ctx, const enkfg_nodes* nodes);
+
+/* T [node][nmem_dynamic][nmem] float, w [node][nmem] float, and optional
+ * pre-cast FP64 copies (need ENKFG_FLAG_KEEP_FP64).  Any pointer may be NULL. */
+int enkfg_get_transforms(enkfg_ctx* ctx, float* T, float* w, double* T64, double* w64);
+
+/* enkf_diag.nc contents: arrays [ntypes + 1][node]; slot ntypes holds the
+ * totals (nlobs, dfs, srf), slots 0..ntypes-1 the per-type values
+ * (calc/transforms.c:843-879).  Any pointer may be NULL. */
+int enkfg_get_diag(enkfg_ctx* ctx, int* pnlobs, float* pdfs, float* psrf);
+
+int enkfg_get_stats(enkfg_ctx* ctx, enkfg_stats* stats);
+
+/* Localisation only (obs_findlocal for every node); lists are kept on the
+ * device.  counts: [node]; a node's list is read with enkfg_get_localobs. */
+int enkfg_localise(enkfg_ctx* ctx, const enkfg_nodes* nodes);
+int enkfg_get_local_counts(enkfg_ctx* ctx, int* counts);
+int enkfg_get_localobs(enkfg_ctx* ctx, int node, int cap, int* ids, double* lc);
+
+/* ---------------------------------------------------------------- update */
+/* The transforms.nc contents for the update (T may be NULL in EnOI mode).
+ * Not needed when enkfg_calc_transforms ran in the same context. */
+int enkfg_set_transforms(enkfg_ctx* ctx, int nj_s, int ni_s, int stride, const float* T, const float* w);
+
+/* das_updatefields for rows j1..j2 of a structured grid (j1 % stride == 0):
+ * fieldbuffer[f] is float [nmem][j2-j1+1][mni], updated in place; infl
+ * (optional) receives [nfield][j2-j1+1][mni] inflation factors. */
+int enkfg_update_fields(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const enkfg_field* fields, int j1, int j2, float** fieldbuffer, float* infl, int flags);
+
+/* das_updatebg (EnOI): fieldbuffer[f] is [nmem + 1][jsize][mni]; member nmem
+ * is the background, updated in place. */
+int enkfg_update_bg(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const enkfg_field* fields, int j1, int j2, float** fieldbuffer);
+
+/* Device-resident ensemble: a caller-owned device buffer laid out
+ * [field][cell][member] (cell = j * mni + i, members contiguous) holding all
+ * nfield fields of the whole grid.  The update then runs in place with no
+ * host traffic. */
+int enkfg_ens_attach(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const enkfg_field* fields, float* dev_E);
+int enkfg_ens_update(enkfg_ctx* ctx, int flags, float* dev_infl);
+
+/* hipEvent-measured duration (ms) of the last calc / update kernels, per
+ * phase: [0] localisation, [1] Gram, [2] factorisation, [3] update. */
+int enkfg_get_timings(enkfg_ctx* ctx, double* ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

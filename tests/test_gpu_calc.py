@@ -1,0 +1,144 @@
+"""GPU parity of the calc path (localisation, per-column transforms,
+diagnostics) against the CPU oracle, through the C ABI.
+
+Tolerances: local lists (ids and taper coefficients) are compared bit for
+bit; pre-cast FP64 T and w within 1e-10 relative to max|T|, max|w| (the
+BASELINE plain-FP64 bar); the FP32 outputs within 2 ulp of the oracle's cast;
+dfs/srf (stored as float by the reference) within 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+import enkfg
+import oracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def problem(**kw):
+    P = synth.make_problem(**kw)
+    return P
+
+
+def oracle_obs(P, status=None, status_build=None):
+    return O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd,
+                       status=status if status is not None else P.obs_status, status_at_build=status_build)
+
+
+def engine_for(P, mode, scheme, nmem_dynamic=None, alpha=1.0, pertype=True):
+    E = enkfg.Engine(P.nmem, nmem_dynamic=nmem_dynamic, mode=mode, scheme=scheme, alpha=alpha,
+                     pertype_diag=pertype, keep_fp64=True)
+    E.set_obstypes(P.obstypes)
+    E.set_obs(P.obs_type, P.obs_status, P.obs_lon, P.obs_lat, P.obs_estd)
+    return E
+
+
+@pytest.mark.parametrize("seed,curv,profiles", [(0, False, 0), (1, True, 40)])
+def test_localisation_bitwise(seed, curv, profiles):
+    print("case localisation seed=%d curv=%s profiles=%d" % (seed, curv, profiles), flush=True)
+    P = problem(seed=seed, curvilinear=curv, profiles=profiles, nobs=2500, locrad=80.0)
+    E = engine_for(P, enkfg.MODE_ENKF, enkfg.SCHEME_DENKF)
+    E.localise(P.node_lon, P.node_lat, stride=P.stride)
+    counts = E.local_counts()
+    ob = oracle_obs(P)
+    for k in range(P.node_lon.size):
+        ids_o, lc_o = ob.findlocal(P.node_lon.flat[k], P.node_lat.flat[k])
+        assert counts[k] == len(ids_o), "node %d count" % k
+        ids_g, lc_g = E.localobs(k, counts[k])
+        assert np.array_equal(ids_g[:counts[k]], ids_o), "node %d ids" % k
+        assert np.array_equal(lc_g[:counts[k]].view(np.uint64), lc_o.view(np.uint64)), "node %d taper bits" % k
+    E.close()
+
+
+def test_localisation_status_visibility_global():
+    print("case localisation status/visibility/global", flush=True)
+    P = problem(seed=3, profiles=30, nobs=2000, locrad=70.0)
+    rng = np.random.default_rng(5)
+    status = P.obs_status.copy()
+    status[rng.random(P.nobs) < 0.1] = 7          # bad after the trees were built
+    E = engine_for(P, enkfg.MODE_ENKF, enkfg.SCHEME_DENKF)
+    E.set_obs_status(status)
+    lon = P.node_lon.copy()
+    lat = P.node_lat.copy()
+    lon[0, 0] = np.nan                              # a node without coordinates -> all obs
+    vis = np.array([1, 0], np.uint8)
+    E.localise(lon, lat, stride=P.stride, visible=vis)
+    counts = E.local_counts()
+    ob = oracle_obs(P, status=status, status_build=P.obs_status)
+    for k in range(lon.size):
+        ids_o, lc_o = ob.findlocal(lon.flat[k], lat.flat[k], visible=vis)
+        assert counts[k] == len(ids_o)
+        ids_g, lc_g = E.localobs(k, counts[k])
+        assert np.array_equal(ids_g[:counts[k]], ids_o)
+        assert np.array_equal(lc_g[:counts[k]].view(np.uint64), lc_o.view(np.uint64))
+    E.close()
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def ulps32(a, b):
+    ai = a.view(np.int32).astype(np.int64)
+    bi = b.view(np.int32).astype(np.int64)
+    return np.abs(ai - bi).max()
+
+
+CASES = [
+    # mode, scheme, m, md, alpha, profiles
+    (enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, 24, 24, 1.0, 0),
+    (enkfg.MODE_ENKF, enkfg.SCHEME_ETKF, 24, 24, 1.0, 0),
+    (enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, 96, 96, 0.5, 30),
+    (enkfg.MODE_ENKF, enkfg.SCHEME_ETKF, 96, 96, 1.0, 30),
+    (enkfg.MODE_ENOI, enkfg.SCHEME_DENKF, 40, 40, 1.0, 0),
+    (enkfg.MODE_HYBRID, enkfg.SCHEME_DENKF, 48, 32, 1.0, 20),
+]
+
+
+@pytest.mark.parametrize("mode,scheme,m,md,alpha,profiles", CASES)
+def test_calc_transforms_parity(mode, scheme, m, md, alpha, profiles):
+    print("case calc mode=%d scheme=%d m=%d md=%d alpha=%g" % (mode, scheme, m, md, alpha), flush=True)
+    P = problem(seed=m + md, nmem=m, profiles=profiles, nobs=2500, locrad=75.0, nj=45, ni=60)
+    ob = oracle_obs(P)
+    S, s = O.standardise(m, ob, P.HA, P.innov)
+    ref = O.calc_transforms(mode, scheme, m, md, alpha, ob, S, s, P.node_lon.ravel(), P.node_lat.ravel())
+    E = engine_for(P, mode, scheme, nmem_dynamic=md, alpha=alpha)
+    E.set_S(P.HA, P.innov, standardised=False)
+    Sg, sg = E.get_S()
+    assert np.array_equal(Sg, S), "standardised S"
+    assert np.array_equal(sg, s), "standardised s"
+    E.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
+    T, w, T64, w64 = E.transforms(fp64=True)
+    pn, pd, ps = E.diag()
+    st = E.stats()
+    assert st == ref["stats"]
+    assert np.array_equal(pn, ref["pnlobs"])
+    assert relerr(w64, ref["w64"]) < 1e-10
+    assert ulps32(w, ref["w"]) <= 2 or relerr(w, ref["w"]) < 1e-6
+    if mode != enkfg.MODE_ENOI:
+        assert relerr(T64, ref["T64"]) < 1e-10
+        assert np.abs(T.astype(np.float64) - ref["T"]).max() <= 2.4e-7 * np.abs(ref["T"]).max()
+    nz = ref["pdfs"] != 0
+    assert np.allclose(pd, ref["pdfs"], rtol=1e-6, atol=1e-7)
+    assert np.allclose(ps[nz], ref["psrf"][nz], rtol=1e-5, atol=1e-6)
+    E.close()
+
+
+def test_calc_empty_columns_and_stats():
+    print("case calc empty columns", flush=True)
+    P = problem(seed=11, nobs=40, locrad=20.0, nj=30, ni=30)
+    ob = oracle_obs(P)
+    S, s = O.standardise(P.nmem, ob, P.HA, P.innov)
+    ref = O.calc_transforms(1, 1, P.nmem, P.nmem, 1.0, ob, S, s, P.node_lon.ravel(), P.node_lat.ravel())
+    assert (ref["pnlobs"][-1] == 0).any(), "fixture must contain empty columns"
+    E = engine_for(P, enkfg.MODE_ENKF, enkfg.SCHEME_DENKF)
+    E.set_S(S, s, standardised=True)
+    E.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
+    T, w, T64, w64 = E.transforms(fp64=True)
+    empty = ref["pnlobs"][-1] == 0
+    eye = np.eye(P.nmem, dtype=np.float32)
+    assert all(np.array_equal(T[k], eye) for k in np.where(empty)[0])
+    assert not w[empty].any()
+    assert E.stats() == ref["stats"]
+    E.close()
