@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the calc + update hot path on one MI355X.
+
+Workload (default): the one-GPU tile -- 51 layers x 1500 x 1800 curvilinear
+grid, ensemble m = 96, 7M super-obs (60% SST uniform, 30% SLA along 500 km
+tracks, 10% T/S profiles over 51 levels), LOCRAD 250 km, STRIDE 3 (300,000
+local analyses), plain FP64 GPU path.  Synthetic data generated on the GPU
+(Gaussian random fields, 300 km correlation length).
+
+A step is one pass of the hot path: das_calctransforms (localisation, per-
+column transforms, diagnostics) for every stride node, then das_updatefields
+over all 103 fields of the tile, with every input already resident in HBM.
+`value` is local analyses per second of the calc pass; the update rate
+(cell-member updates per second) is reported beside it.
+
+Prints one JSON line (rank 0).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("enkf-c_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+METRIC = "local analyses/s + cell·member updates/s at m=96; mixed vs FP64 GPU path"
+PEAK_FP64 = 78.6e12     # MI355X dense FP64 (matrix and vector), flop/s
+PEAK_HBM = 8.0e12       # MI355X HBM3E, B/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(X, scheme_id, md, sample_target_s=15.0, max_nodes=4000):
+    """The reference kernels (oracle/_ref: calc/calcs.c + common/kdtree.c with
+    MKL, one thread) or, when they are not built, the oracle restatement, timed
+    on a contiguous sample of stride nodes from the middle of the tile."""
+    import numpy as np
+    import oracle as O
+    os.environ.setdefault("MKL_NUM_THREADS", "1")
+    kind = "reference" if O.ref_available() else "port"
+    try:
+        if kind == "reference":
+            O.ref()
+    except OSError:
+        kind = "port"
+    ob = O.OracleObs(X.obstypes, X.obs_type, X.obs_lon, X.obs_lat, X.obs_estd, build_trees=(kind == "port"))
+    S, s = O.standardise(X.nmem, ob, X.HA, X.innov)
+    handle = O.RefTrees(ob) if kind == "reference" else False
+    node0 = (X.nj_s // 2) * X.ni_s + X.ni_s // 4
+    lon = X.node_lon.ravel()[node0:node0 + max_nodes]
+    lat = X.node_lat.ravel()[node0:node0 + max_nodes]
+    # calibrate on a few nodes, then time a sample of about sample_target_s
+    t = time.perf_counter()
+    O.calc_transforms(1, scheme_id, X.nmem, md, 1.0, ob, S, s, lon[:8], lat[:8], use_ref=handle)
+    per = (time.perf_counter() - t) / 8
+    n = int(max(16, min(max_nodes, sample_target_s / max(per, 1e-6))))
+    t = time.perf_counter()
+    r = O.calc_transforms(1, scheme_id, X.nmem, md, 1.0, ob, S, s, lon[:n], lat[:n], use_ref=handle)
+    dt = time.perf_counter() - t
+    pbar = r["stats"]["nlobs_sum"] / max(r["stats"]["ncell"], 1)
+    what = ("reference calc/calcs.c + common/kdtree.c (MKL, 1 thread) driven by the restated "
+            "das_calctransforms loop" if kind == "reference" else "oracle restatement (plain C, 1 thread)")
+    return dict(value=n / dt, unit="local analyses/s", cores=1, kind=kind,
+                sample="%d consecutive stride nodes of the middle row band (mean %.0f local obs), %s, %.1f s"
+                       % (n, pbar, what, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scheme", choices=["denkf", "etkf"], default="denkf")
+    ap.add_argument("--size", choices=["full", "small"], default="full")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        log("bench.py: this build runs on one GPU; the path shards by stride-row bands "
+            "(see DESIGN.md), but no multi-GPU run is part of it.  Use --gpus 1.")
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import enkfg
+    import synth_gpu
+
+    if args.size == "full":
+        tile = dict(nj=1500, ni=1800, nk=51, nmem=96, n_sst=4_200_000, n_sla=2_100_000, n_prof=6_900)
+    else:
+        tile = dict(nj=300, ni=360, nk=10, nmem=96, n_sst=170_000, n_sla=84_000, n_prof=280)
+    scheme_id = enkfg.SCHEME_DENKF if args.scheme == "denkf" else enkfg.SCHEME_ETKF
+    log("generating the synthetic tile %s on the GPU ..." % tile)
+    t0 = time.perf_counter()
+    X = synth_gpu.make_tile(**tile, seed=0)
+    log("  done in %.1f s: %d obs, %d x %d nodes, %d fields" % (time.perf_counter() - t0, X.nobs, X.nj_s, X.ni_s,
+                                                                 X.nfield))
+    E = enkfg.Engine(X.nmem, mode=enkfg.MODE_ENKF, scheme=scheme_id, pertype_diag=True, keep_fp64=False)
+    E.set_obstypes(X.obstypes)
+    t0 = time.perf_counter()
+    E.set_obs(X.obs_type, X.obs_status, X.obs_lon, X.obs_lat, X.obs_estd)
+    t_tree = time.perf_counter() - t0
+    log("  obs + kd-trees set in %.1f s" % t_tree)
+    E.set_S(X.HA, X.innov, standardised=False)
+    log("  S uploaded and standardised")
+
+    def step(attach=False):
+        E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
+        tc = E.timings()
+        if attach:
+            E.ens_attach(X.nlevels, X.stride, 0, 0, X.fields, X.E.data_ptr())
+        E.ens_update()
+        tu = E.timings()
+        return tc[:3] + [tu[3]]
+
+    for w in range(args.warmup):
+        ph = step(attach=(w == 0))
+        log("  warmup %d: phases (ms) loc %.1f gram %.1f factor %.1f update %.1f" % (w, *ph))
+    if args.warmup == 0:
+        E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
+        E.ens_attach(X.nlevels, X.stride, 0, 0, X.fields, X.E.data_ptr())
+    torch.cuda.synchronize()
+    E.sync()
+    phases = np.zeros(4)
+    tcalc = tupd = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        a = time.perf_counter()
+        E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
+        b = time.perf_counter()
+        ph = E.timings()
+        E.ens_update()
+        c = time.perf_counter()
+        ph[3] = E.timings()[3]
+        phases += np.array(ph)
+        tcalc += b - a
+        tupd += c - b
+        log("  step %d: calc %.1f ms update %.1f ms (loc %.1f gram %.1f factor %.1f update-kernel %.1f)"
+            % (k, 1e3 * (b - a), 1e3 * (c - b), *ph))
+    E.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    K = args.steps
+    phases /= K
+    st = E.stats()
+    n_an = X.nj_s * X.ni_s
+    m = X.nmem
+    md = m
+    nfield, ncell = X.nfield, X.ncell
+    calc_rate = n_an / (tcalc / K)
+    upd_rate = nfield * ncell * md / (tupd / K)
+
+    # algorithmic work (per step)
+    p_sum = st["nlobs_sum"]
+    F_gram = p_sum * (m * (m + 1) + 2 * m)              # S^T S upper triangle (SYRK count) + S^T s
+    lev = X.nlevels
+    wet = 0
+    for f in X.fields:
+        thr = 0 if f.get("issurfacevar") else f["level"]
+        wet += int((lev > thr).sum())
+    B_upd = 4 * m * (2 * wet + (nfield * ncell - wet)) + 4 * n_an * (m * md + m)
+    kern = {"gram": phases[1], "update": phases[3], "loc": phases[0], "factor": phases[2]}
+    dom = max(kern, key=kern.get)
+    if dom == "update":
+        ach = B_upd / (phases[3] * 1e-3)
+        roof = dict(bound="hbm", achieved=ach / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=ach / PEAK_HBM,
+                    traffic=None, kernel="update_kernel<6>",
+                    algorithmic="4*m*(2*wet cell-fields + dry cell-fields) + 4*N_an*(m*m_d+m) = %.3e B per launch"
+                                % B_upd)
+    elif dom == "gram":
+        ach = F_gram / (phases[1] * 1e-3)
+        roof = dict(bound="mfma", achieved=ach / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s", frac=ach / PEAK_FP64,
+                    traffic=None, kernel="gram_kernel<6> (FP64 MFMA)",
+                    algorithmic="sum_nodes p*(m(m+1)+2m) = %.3e flop per step (all launches)" % F_gram)
+    else:
+        roof = dict(bound="latency", achieved=None, peak=None, unit=None, frac=None, traffic=None,
+                    kernel=dom, algorithmic="dominant phase is %s" % dom)
+
+    res = dict(metric=METRIC, value=calc_rate, unit="local analyses/s", n_gpus=1, steps=K, warmup=args.warmup,
+               ms_per_step=1e3 * wall / K, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               data="synthetic (GPU-generated Gaussian random fields; no files)",
+               config=dict(workload="tile %dx%dx%d, m=%d, %d obs, %s, FP64, stride %d, locrad 250 km, "
+                                    "%d local analyses, %d fields" % (X.nk, X.nj, X.ni, m, X.nobs,
+                                                                      args.scheme.upper(), X.stride, n_an, nfield),
+                           grid=[X.nk, X.nj, X.ni], nmem=m, nobs=int(X.nobs), n_an=n_an, nfield=nfield,
+                           precision="fp64", scheme=args.scheme, mean_local_obs=p_sum / n_an),
+               update_cell_member_per_s=upd_rate,
+               phases_ms=dict(localise=phases[0], gram=phases[1], factor=phases[2], update=phases[3],
+                              calc_wall=1e3 * tcalc / K, update_wall=1e3 * tupd / K),
+               kd_tree_build_s=t_tree,
+               roofline=roof)
+    if not args.no_cpu_baseline:
+        log("CPU baseline ...")
+        E.close()
+        del X.E
+        torch.cuda.empty_cache()
+        res["cpu_baseline"] = cpu_baseline(X, scheme_id, md)
+        log("  %s" % res["cpu_baseline"])
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -85,14 +85,131 @@ __host__ __device__ constexpr int tri_J(int MT, int f)
     return I + f;
 }
 
+// Chunks of KC local obs are staged in LDS as tapered FP64 rows (double
+// buffered: the gathers of chunk c+1 are in flight while chunk c feeds the
+// MFMAs).  Row stride RS (doubles) is congruent to 16 mod 32 so that the four
+// rows a k-step reads fall in disjoint LDS bank halves.
+constexpr int KC = 32;
+template <int MT>
+struct GramLds {
+    static constexpr int MP = 16 * MT;
+    static constexpr int RS = MP + ((16 - MP % 32) + 32) % 32;
+    double rows[2][KC][RS];   // chunk c in rows[c % 2]
+    double sl[3][KC];         // s_o * lc_o of the chunk's rows, chunk c in [c % 3]
+    double lc[3][KC];         // taper coefficients of the chunk's rows
+    int id[3][KC];            // obs ids of the chunk's rows (-1 past the end)
+};
+
+// per-thread share of the coalesced chunk load: elements q = tid + 256 j of
+// the KC x m chunk (row q / m, column q % m)
+template <int MT>
+struct ChunkLoad {
+    static constexpr int NQ = (KC * 16 * MT + 255) / 256;
+    float v[NQ];
+};
+
+// ids, tapers and s*lc of chunk k0.. into LDS buffer buf (threads < KC)
+__device__ inline void chunk_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c,
+                                       double& sl)
+{
+    const int k = k0 + (int)threadIdx.x;
+    if (threadIdx.x < KC && k < cnt) {
+        id = a.ids[off + k];
+        c = a.lc[off + k];
+        sl = a.sv[id] * c;
+    } else {
+        id = -1;
+        c = 0.0;
+        sl = 0.0;
+    }
+}
+
+template <int MT>
+__device__ inline void chunk_meta_store(GramLds<MT>& sh, int buf, int id, double c, double sl)
+{
+    if (threadIdx.x < KC) {
+        sh.id[buf][threadIdx.x] = id;
+        sh.lc[buf][threadIdx.x] = c;
+        sh.sl[buf][threadIdx.x] = sl;
+    }
+}
+
+// gathers of the chunk whose ids are in LDS buffer buf
+template <int MT>
+__device__ inline void chunk_vals(const CalcArgs& a, const GramLds<MT>& sh, int buf, ChunkLoad<MT>& L)
+{
+    const int m = a.m;
+#pragma unroll
+    for (int j = 0; j < ChunkLoad<MT>::NQ; ++j) {
+        const int q = threadIdx.x + 256 * j;
+        const int row = q / m, colm = q - row * m;
+        const int id = (row < KC) ? sh.id[buf][row] : -1;
+        L.v[j] = (id >= 0) ? a.S[(size_t)id * m + colm] : 0.0f;
+    }
+}
+
+// rows of a chunk into row buffer rbuf, tapered with the coefficients in
+// meta buffer mbuf
+template <int MT>
+__device__ inline void chunk_store(GramLds<MT>& sh, int rbuf, int mbuf, int m, const ChunkLoad<MT>& L)
+{
+#pragma unroll
+    for (int j = 0; j < ChunkLoad<MT>::NQ; ++j) {
+        const int q = threadIdx.x + 256 * j;
+        const int row = q / m, colm = q - row * m;
+        if (row < KC)
+            sh.rows[rbuf][row][colm] = (double)L.v[j] * sh.lc[mbuf][row];
+    }
+}
+
+// acc[T] += frag[I] frag[J]^T for the tiles (I, J) of wave W, with the tile
+// coordinates compile-time constants (register arrays stay in registers)
+template <int MT, int W, int T>
+__device__ inline void mfma_tiles(const double (&frag)[MT], dbl4* acc)
+{
+    if constexpr (W + 4 * T < tri_count(MT)) {
+        constexpr int f = W + 4 * T;
+        constexpr int I = tri_I(MT, f), J = tri_J(MT, f);
+        acc[T] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[I], frag[J], acc[T], 0, 0, 0);
+        mfma_tiles<MT, W, T + 1>(frag, acc);
+    }
+}
+
+template <int MT, int W>
+__device__ inline void gram_wave_chunk(const GramLds<MT>& sh, int buf, int mbuf, dbl4 (&acc)[(tri_count(MT) - W + 3) / 4 > 0 ? (tri_count(MT) - W + 3) / 4 : 1], double (&sts)[MT])
+{
+    constexpr int NT = tri_count(MT);
+    constexpr int NTW = (NT - W + 3) / 4;
+    const int lane = threadIdx.x & 63;
+    const int kq = lane >> 4;
+    const int col = lane & 15;
+#pragma unroll 2
+    for (int s = 0; s < KC / 4; ++s) {
+        const int r = 4 * s + kq;
+        double frag[MT];
+#pragma unroll
+        for (int X = 0; X < MT; ++X)
+            frag[X] = sh.rows[buf][r][16 * X + col];
+        (void)NTW;
+        mfma_tiles<MT, W, 0>(frag, acc);
+        if (W == 0) {
+            const double sl = sh.sl[mbuf][r];
+#pragma unroll
+            for (int X = 0; X < MT; ++X)
+                sts[X] += frag[X] * sl;
+        }
+    }
+}
+
 template <int MT, int W>
 __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, double* __restrict__ gout,
-                                 double* __restrict__ stsout)
+                                 double* __restrict__ stsout, GramLds<MT>& sh)
 {
     constexpr int NT = tri_count(MT);
     constexpr int NTW = (NT - W + 3) / 4;   // tiles of this wave
     constexpr int MP = 16 * MT;
-    const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const int kq = lane >> 4;   // which of the 4 obs of a k-step
     const int col = lane & 15;
     const int m = a.m;
@@ -106,33 +223,37 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
     for (int X = 0; X < MT; ++X)
         sts[X] = 0.0;
 
-    for (int k0 = 0; k0 < cnt; k0 += 4) {
-        const int k = k0 + kq;
-        double c = 0.0, sl = 0.0;
-        const float* row = a.S;
-        if (k < cnt) {
-            const int id = a.ids[off + k];
-            c = a.lc[off + k];
-            row = a.S + (size_t)id * m;
-            if (W == 0)
-                sl = a.sv[id] * c;
-        }
-        double frag[MT];
-#pragma unroll
-        for (int X = 0; X < MT; ++X) {
-            const int e = 16 * X + col;
-            frag[X] = (k < cnt && e < m) ? (double)row[e] * c : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < NTW; ++t) {
-            const int f = W + 4 * t;
-            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[tri_I(MT, f)], frag[tri_J(MT, f)], acc[t], 0, 0, 0);
-        }
-        if (W == 0) {
-#pragma unroll
-            for (int X = 0; X < MT; ++X)
-                sts[X] += frag[X] * sl;
-        }
+    // zero both row buffers once: padding columns stay zero
+    for (int q = tid; q < 2 * KC * GramLds<MT>::RS; q += 256)
+        (&sh.rows[0][0][0])[q] = 0.0;
+    ChunkLoad<MT> L;
+    const int nchunk = (cnt + KC - 1) / KC;
+    int mid;
+    double mc, msl;
+    // prologue: meta of chunks 0 and 1, rows of chunk 0
+    chunk_meta_load(a, off, cnt, 0, mid, mc, msl);
+    chunk_meta_store<MT>(sh, 0, mid, mc, msl);
+    chunk_meta_load(a, off, cnt, KC, mid, mc, msl);
+    chunk_meta_store<MT>(sh, 1, mid, mc, msl);
+    __syncthreads();
+    chunk_vals<MT>(a, sh, 0, L);
+    chunk_store<MT>(sh, 0, 0, m, L);
+    __syncthreads();
+    // iteration c: gathers of chunk c+1 and meta of chunk c+2 are in flight
+    // while chunk c feeds the MFMAs; one barrier per chunk
+    for (int c = 0; c < nchunk; ++c) {
+        const bool more = c + 1 < nchunk;
+        const bool more2 = c + 2 < nchunk;
+        if (more)
+            chunk_vals<MT>(a, sh, (c + 1) % 3, L);
+        if (more2)
+            chunk_meta_load(a, off, cnt, (c + 2) * KC, mid, mc, msl);
+        gram_wave_chunk<MT, W>(sh, c & 1, c % 3, acc, sts);
+        if (more)
+            chunk_store<MT>(sh, (c + 1) & 1, (c + 1) % 3, m, L);
+        if (more2)
+            chunk_meta_store<MT>(sh, (c + 2) % 3, mid, mc, msl);
+        __syncthreads();
     }
     // Operand layouts of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l
     // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register
@@ -180,13 +301,14 @@ __global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
     }
     if (cnt == 0)
         return;
+    __shared__ GramLds<MT> sh;
     double* gout = a.gram + (size_t)b * MP * MP;
     double* stsout = a.sts + (size_t)b * MP;
     switch (threadIdx.x >> 6) {
-    case 0: gram_wave<MT, 0>(a, off, cnt, gout, stsout); break;
-    case 1: gram_wave<MT, 1>(a, off, cnt, gout, stsout); break;
-    case 2: gram_wave<MT, 2>(a, off, cnt, gout, stsout); break;
-    default: gram_wave<MT, 3>(a, off, cnt, gout, stsout); break;
+    case 0: gram_wave<MT, 0>(a, off, cnt, gout, stsout, sh); break;
+    case 1: gram_wave<MT, 1>(a, off, cnt, gout, stsout, sh); break;
+    case 2: gram_wave<MT, 2>(a, off, cnt, gout, stsout, sh); break;
+    default: gram_wave<MT, 3>(a, off, cnt, gout, stsout, sh); break;
     }
 }
 

@@ -174,6 +174,37 @@ void build_tree(const std::vector<double>& xyz, int ndim, const std::vector<int>
         }
         out.push_back(n);
     }
+    // Relabel the nodes in preorder (root first, then the left subtree, then
+    // the right one) so that a subtree occupies a contiguous stretch of
+    // memory; the tree itself, and so every walk over it, is unchanged.
+    const int n = (int)(out.size() - base);
+    if (n > 2) {
+        std::vector<int> order;
+        order.reserve(n);
+        std::vector<int> stack(1, 0);
+        while (!stack.empty()) {
+            const int v = stack.back();
+            stack.pop_back();
+            order.push_back(v);
+            const KdNode& nd = out[base + v];
+            if (nd.right >= 0)
+                stack.push_back(nd.right);
+            if (nd.left >= 0)
+                stack.push_back(nd.left);
+        }
+        std::vector<int> pos(n);
+        for (int k = 0; k < n; ++k)
+            pos[order[k]] = k;
+        std::vector<KdNode> tmp(n);
+        for (int k = 0; k < n; ++k) {
+            KdNode nd = out[base + order[k]];
+            nd.left = (nd.left >= 0) ? pos[nd.left] : -1;
+            nd.right = (nd.right >= 0) ? pos[nd.right] : -1;
+            nd.parent = (nd.parent >= 0) ? pos[nd.parent] : -1;
+            tmp[k] = nd;
+        }
+        std::copy(tmp.begin(), tmp.end(), out.begin() + base);
+    }
 }
 
 int check_ctx(enkfg_ctx* c)

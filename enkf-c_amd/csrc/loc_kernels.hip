@@ -112,26 +112,28 @@ __global__ void __launch_bounds__(128) loc_count_kernel(int nnode, const double*
                                                         const TypeInfo* __restrict__ types, int ntypes,
                                                         const short* __restrict__ status, int* __restrict__ counts)
 {
-    const int node = blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= nnode)
+    // one thread per (node, type); consecutive threads take neighbouring nodes
+    // of the same type so that a wave walks similar paths of one tree
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)nnode * ntypes)
         return;
+    const int t = (int)(gid / nnode);
+    const int node = (int)(gid % nnode);
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
-    for (int t = 0; t < ntypes; ++t) {
-        const TypeInfo ti = types[t];
-        int n = 0;
-        if (ti.searchable && (visible == nullptr || visible[t])) {
-            if (glob) {
-                n = ti.nids;   // obs_getglobal: all obs of the type, no status check
-            } else if (ti.nnode > 0) {
-                range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int obsid, double) {
-                    if (status[obsid] == kStatusOK)
-                        n++;
-                });
-            }
+    const TypeInfo& ti = types[t];
+    int n = 0;
+    if (ti.searchable && (visible == nullptr || visible[t])) {
+        if (glob) {
+            n = ti.nids;   // obs_getglobal: all obs of the type, no status check
+        } else if (ti.nnode > 0) {
+            range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int obsid, double) {
+                if (status[obsid] == kStatusOK)
+                    n++;
+            });
         }
-        counts[(size_t)node * ntypes + t] = n;
     }
+    counts[(size_t)node * ntypes + t] = n;
 }
 
 __global__ void __launch_bounds__(128) loc_fill_kernel(int nnode, const double* __restrict__ qxyz,
@@ -144,30 +146,30 @@ __global__ void __launch_bounds__(128) loc_fill_kernel(int nnode, const double* 
                                                        const long long* __restrict__ offsets, int* __restrict__ ids,
                                                        double* __restrict__ lc)
 {
-    const int node = blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= nnode)
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)nnode * ntypes)
         return;
+    const int t = (int)(gid / nnode);
+    const int node = (int)(gid % nnode);
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
-    for (int t = 0; t < ntypes; ++t) {
-        const TypeInfo& ti = types[t];
-        if (!ti.searchable || !(visible == nullptr || visible[t]))
-            continue;
-        long long o = offsets[(size_t)node * ntypes + t];
-        if (glob) {
-            for (int i = 0; i < ti.nids; ++i) {
-                ids[o + i] = obsids[ti.obsid0 + i];
-                lc[o + i] = 1.0;
-            }
-        } else if (ti.nnode > 0) {
-            range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int obsid, double distsq) {
-                if (status[obsid] == kStatusOK) {
-                    ids[o] = obsid;
-                    lc[o] = calclcoeff(ti, sqrt(distsq));
-                    o++;
-                }
-            });
+    const TypeInfo& ti = types[t];
+    if (!ti.searchable || !(visible == nullptr || visible[t]))
+        return;
+    long long o = offsets[(size_t)node * ntypes + t];
+    if (glob) {
+        for (int i = 0; i < ti.nids; ++i) {
+            ids[o + i] = obsids[ti.obsid0 + i];
+            lc[o + i] = 1.0;
         }
+    } else if (ti.nnode > 0) {
+        range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int obsid, double distsq) {
+            if (status[obsid] == kStatusOK) {
+                ids[o] = obsid;
+                lc[o] = calclcoeff(ti, sqrt(distsq));
+                o++;
+            }
+        });
     }
 }
 
@@ -258,10 +260,11 @@ hipError_t launch_loc_count(hipStream_t s, int nnode, const double* qxyz, const 
                             const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                             const short* status, int* counts)
 {
-    if (nnode <= 0)
+    const long long nt = (long long)nnode * ntypes;
+    if (nt <= 0)
         return hipSuccess;
-    dim3 grid((nnode + 127) / 128);
-    hipLaunchKernelGGL(loc_count_kernel, grid, dim3(128), 0, s, nnode, qxyz, isglobal, visible, nodes, types,
+    dim3 grid((unsigned)((nt + 63) / 64));
+    hipLaunchKernelGGL(loc_count_kernel, grid, dim3(64), 0, s, nnode, qxyz, isglobal, visible, nodes, types,
                        ntypes, status, counts);
     return hipGetLastError();
 }
@@ -270,10 +273,11 @@ hipError_t launch_loc_fill(hipStream_t s, int nnode, const double* qxyz, const u
                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                            const short* status, const int* obsids, const long long* offsets, int* ids, double* lc)
 {
-    if (nnode <= 0)
+    const long long nt = (long long)nnode * ntypes;
+    if (nt <= 0)
         return hipSuccess;
-    dim3 grid((nnode + 127) / 128);
-    hipLaunchKernelGGL(loc_fill_kernel, grid, dim3(128), 0, s, nnode, qxyz, isglobal, visible, nodes, types,
+    dim3 grid((unsigned)((nt + 63) / 64));
+    hipLaunchKernelGGL(loc_fill_kernel, grid, dim3(64), 0, s, nnode, qxyz, isglobal, visible, nodes, types,
                        ntypes, status, obsids, offsets, ids, lc);
     return hipGetLastError();
 }

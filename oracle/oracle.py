@@ -120,7 +120,10 @@ def ref():
     if _ref is None:
         if not ref_available():
             raise RuntimeError("oracle/_ref not built (needs the reference tree and MKL): make -C oracle ref")
-        os.environ.setdefault("MKL_NUM_THREADS", "1")
+        # one thread, sequential MKL: threaded MKL sharing a process with
+        # torch's OpenMP runtime gives wrong results
+        os.environ["MKL_THREADING_LAYER"] = "SEQUENTIAL"
+        os.environ["MKL_NUM_THREADS"] = "1"
         R = C.CDLL(REF_PATH)
         R.ref_local_analysis.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                          _dp, _dp, _dp]
@@ -129,8 +132,11 @@ def ref():
         R.ref_kd_destroy.argtypes = [C.c_void_p]
         R.ref_kd_size.argtypes = [C.c_void_p]
         R.ref_kd_range.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, C.c_int, _ip, _dp]
-        R.ref_calc_transforms.argtypes = [C.POINTER(CalcSpec), C.POINTER(Obs), _sp, _fp, _dp, C.c_int, _dp,
-                                          _dp, C.c_int, _up, _fp, _fp, _dp, _dp, _ip, _fp, _fp,
+        R.ref_trees_create.restype = C.c_void_p
+        R.ref_trees_create.argtypes = [C.POINTER(Obs), _sp]
+        R.ref_trees_destroy.argtypes = [C.c_void_p]
+        R.ref_calc_transforms.argtypes = [C.POINTER(CalcSpec), C.POINTER(Obs), C.c_void_p, _fp, _dp, C.c_int,
+                                          _dp, _dp, C.c_int, _up, _fp, _fp, _dp, _dp, _ip, _fp, _fp,
                                           C.POINTER(CalcStats), C.c_int, C.c_int]
         _ref = R
     return _ref
@@ -237,7 +243,7 @@ def make_obstypes(specs):
 class OracleObs:
     """Observations plus per-type kd-trees, as obs_createkdtrees builds them."""
 
-    def __init__(self, otypes, type_, lon, lat, estd, status=None, status_at_build=None):
+    def __init__(self, otypes, type_, lon, lat, estd, status=None, status_at_build=None, build_trees=True):
         self.types = make_obstypes(otypes)
         self.ntypes = len(otypes)
         self.type = np.ascontiguousarray(type_, np.int16)
@@ -250,9 +256,10 @@ class OracleObs:
             status_at_build if status_at_build is not None else self.status, np.int16)
         self.o = Obs(n, self.ntypes, _ptr(self.type, _sp), _ptr(self.status, _sp), _ptr(self.lon, _fp),
                      _ptr(self.lat, _fp), _ptr(self.estd, _fp), self.types, None, None, None)
-        r = lib().or_obs_build_trees(C.byref(self.o), _ptr(self.status_build, _sp))
-        if r != 0:
-            raise RuntimeError("or_obs_build_trees: %d" % r)
+        if build_trees:
+            r = lib().or_obs_build_trees(C.byref(self.o), _ptr(self.status_build, _sp))
+            if r != 0:
+                raise RuntimeError("or_obs_build_trees: %d" % r)
 
     def findlocal(self, lon, lat, geographic=1, visible=None):
         cap = max(len(self.type), 1)
@@ -317,6 +324,24 @@ def local_analysis(mode, scheme, m, mout, S, s, use_ref=False):
 
 
 # ---------------------------------------------------------------- calc loop
+class RefTrees:
+    """The reference's per-type kd-trees (kd_insertnode, sequential)."""
+
+    def __init__(self, obs):
+        self.h = ref().ref_trees_create(C.byref(obs.o), _ptr(obs.status_build, _sp))
+
+    def close(self):
+        if self.h:
+            ref().ref_trees_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def calc_transforms(mode, scheme, nmem, nmem_dynamic, alpha, obs, S, s, lon, lat, geographic=1,
                     visible=None, node0=0, nnode_do=-1, use_ref=False, want_T=True):
     """Runs the das_calctransforms loop.  S: [nmem][nobs] float32 (already
@@ -340,11 +365,17 @@ def calc_transforms(mode, scheme, nmem, nmem_dynamic, alpha, obs, S, s, lon, lat
     spec = CalcSpec(mode, scheme, nmem, md, alpha)
     vis = None if visible is None else np.ascontiguousarray(visible, np.uint8)
     if use_ref:
-        r = ref().ref_calc_transforms(C.byref(spec), C.byref(obs.o), _ptr(obs.status_build, _sp), _ptr(S, _fp),
+        own = False
+        if not isinstance(use_ref, RefTrees):
+            use_ref = RefTrees(obs)
+            own = True
+        r = ref().ref_calc_transforms(C.byref(spec), C.byref(obs.o), use_ref.h, _ptr(S, _fp),
                                       _ptr(s, _dp), nnode, _ptr(lon, _dp), _ptr(lat, _dp), int(geographic),
                                       _ptr(vis, _up), _ptr(T, _fp), _ptr(w, _fp), _ptr(T64, _dp),
                                       _ptr(w64, _dp), _ptr(pnlobs, _ip), _ptr(pdfs, _fp), _ptr(psrf, _fp),
                                       C.byref(st), node0, nnode_do)
+        if own:
+            use_ref.close()
     else:
         r = lib().or_calc_transforms(C.byref(spec), C.byref(obs.o), _ptr(S, _fp), _ptr(s, _dp), nnode,
                                      _ptr(lon, _dp), _ptr(lat, _dp), int(geographic), _ptr(vis, _up),

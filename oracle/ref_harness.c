@@ -162,27 +162,44 @@ int ref_kd_range(void* tree, const double* q, double range, int ordered, int cap
 }
 
 /* ---------------------------------------------------------------------- */
-/* The calc loop over nodes node0 .. node0 + nnode_do - 1 with the reference
- * kernels (search, algebra).  Same arguments and outputs as
- * or_calc_transforms (oracle/enkf_oracle.h). */
-int ref_calc_transforms(const or_calcspec* spec, const or_obs* obs, const short* status_at_build, const float* S, const double* s, int nnode, const double* lon, const double* lat, int geographic, const unsigned char* visible, float* Tout, float* wout, double* T64, double* w64, int* pnlobs, float* pdfs, float* psrf, or_calcstats* stats, int node0, int nnode_do)
-{
-    int m = spec->nmem, md = spec->nmem_dynamic, nt = obs->ntypes;
-    int dotT = (spec->mode == OR_MODE_ENKF || spec->mode == OR_MODE_HYBRID);
-    kdtree** trees = calloc(nt, sizeof(kdtree*));
-    int** obsids = calloc(nt, sizeof(int*));
-    int* ntobs = calloc(nt, sizeof(int));
-    int* lobs = malloc((obs->nobs + 1) * sizeof(int));
-    double* lco = malloc((obs->nobs + 1) * sizeof(double));
-    double* w = malloc(m * sizeof(double));
-    double** T = alloc2d(m, m, sizeof(double));
-    size_t Mbytes = (size_t) m * (2 * m + 11) * sizeof(double) + (2 * m + 11) * sizeof(void*);
-    void* Mstore = malloc(Mbytes);
-    int t, i, node;
+/* Per-type trees as obs_createkdtrees (observations.c:1616-1733) builds them,
+ * with the reference kd_insertnode. */
+typedef struct {
+    int nt;
+    kdtree** trees;
+    int** obsids;
+    int* ntobs;
+} ref_trees;
 
-    if (stats != NULL)
-        memset(stats, 0, sizeof(*stats));
-    /* trees: obs_createkdtrees (observations.c:1616-1733) */
+void ref_trees_destroy(ref_trees* rt)
+{
+    int t;
+
+    if (rt == NULL)
+        return;
+    for (t = 0; t < rt->nt; ++t) {
+        if (rt->trees[t] != NULL)
+            kd_destroy(rt->trees[t]);
+        free(rt->obsids[t]);
+    }
+    free(rt->trees);
+    free(rt->obsids);
+    free(rt->ntobs);
+    free(rt);
+}
+
+ref_trees* ref_trees_create(const or_obs* obs, const short* status_at_build)
+{
+    int nt = obs->ntypes, t, i;
+    ref_trees* rt = calloc(1, sizeof(ref_trees));
+    kdtree** trees;
+    int** obsids;
+    int* ntobs;
+
+    rt->nt = nt;
+    rt->trees = trees = calloc(nt, sizeof(kdtree*));
+    rt->obsids = obsids = calloc(nt, sizeof(int*));
+    rt->ntobs = ntobs = calloc(nt, sizeof(int));
     for (t = 0; t < nt; ++t) {
         const or_obstype* ot = &obs->types[t];
         int n = 0;
@@ -212,6 +229,30 @@ int ref_calc_transforms(const or_calcspec* spec, const or_obs* obs, const short*
             }
         ntobs[t] = n;
     }
+    return rt;
+}
+
+/* The calc loop over nodes node0 .. node0 + nnode_do - 1 with the reference
+ * kernels (search, algebra).  Same arguments and outputs as
+ * or_calc_transforms (oracle/enkf_oracle.h), the trees from
+ * ref_trees_create(). */
+int ref_calc_transforms(const or_calcspec* spec, const or_obs* obs, const ref_trees* rt, const float* S, const double* s, int nnode, const double* lon, const double* lat, int geographic, const unsigned char* visible, float* Tout, float* wout, double* T64, double* w64, int* pnlobs, float* pdfs, float* psrf, or_calcstats* stats, int node0, int nnode_do)
+{
+    int m = spec->nmem, md = spec->nmem_dynamic, nt = obs->ntypes;
+    int dotT = (spec->mode == OR_MODE_ENKF || spec->mode == OR_MODE_HYBRID);
+    kdtree** trees = rt->trees;
+    int** obsids = rt->obsids;
+    int* ntobs = rt->ntobs;
+    int* lobs = malloc((obs->nobs + 1) * sizeof(int));
+    double* lco = malloc((obs->nobs + 1) * sizeof(double));
+    double* w = malloc(m * sizeof(double));
+    double** T = alloc2d(m, m, sizeof(double));
+    size_t Mbytes = (size_t) m * (2 * m + 11) * sizeof(double) + (2 * m + 11) * sizeof(void*);
+    void* Mstore = malloc(Mbytes);
+    int t, i, node;
+
+    if (stats != NULL)
+        memset(stats, 0, sizeof(*stats));
     if (nnode_do < 0)
         nnode_do = nnode - node0;
     for (node = node0; node < node0 + nnode_do; ++node) {
@@ -391,14 +432,6 @@ int ref_calc_transforms(const or_calcspec* spec, const or_obs* obs, const short*
         free(G);
         free(sloc);
     }
-    for (t = 0; t < nt; ++t) {
-        if (trees[t] != NULL)
-            kd_destroy(trees[t]);
-        free(obsids[t]);
-    }
-    free(trees);
-    free(obsids);
-    free(ntobs);
     free(lobs);
     free(lco);
     free(w);
