@@ -85,81 +85,102 @@ __host__ __device__ constexpr int tri_J(int MT, int f)
     return I + f;
 }
 
-// Chunks of KC local obs are staged in LDS as tapered FP64 rows (double
-// buffered: the gathers of chunk c+1 are in flight while chunk c feeds the
-// MFMAs).  Row stride RS (doubles) is congruent to 16 mod 32 so that the four
-// rows a k-step reads fall in disjoint LDS bank halves.
-constexpr int KC = 32;
+// Chunks of KC local obs are staged in LDS as the raw FP32 rows of S (double
+// buffered: the gathers of chunk c+1, and the list entries of chunk c+2, are
+// in flight while chunk c feeds the MFMAs); the taper is applied in FP64 when
+// a fragment is formed, (double) S * lc, as the reference forms Sloc.  Row
+// stride RS (floats) is congruent to 16 mod 32 and to 16 or 48 mod 64, so the
+// four rows a k-step reads fall in disjoint LDS banks.
+constexpr int KC = 64;
 template <int MT>
 struct GramLds {
     static constexpr int MP = 16 * MT;
     static constexpr int RS = MP + ((16 - MP % 32) + 32) % 32;
-    double rows[2][KC][RS];   // chunk c in rows[c % 2]
-    double sl[3][KC];         // s_o * lc_o of the chunk's rows, chunk c in [c % 3]
-    double lc[3][KC];         // taper coefficients of the chunk's rows
-    int id[3][KC];            // obs ids of the chunk's rows (-1 past the end)
+    float rows[2][KC][RS];    // chunk c in rows[c % 2]
+    double sl[2][KC];         // s_o * lc_o of the chunk's rows, chunk c in [c % 2]
+    double lc[3][KC];         // taper coefficients, chunk c in [c % 3]
+    int id[3][KC];            // obs ids (-1 past the end), chunk c in [c % 3]
 };
 
 // per-thread share of the coalesced chunk load: elements q = tid + 256 j of
-// the KC x m chunk (row q / m, column q % m)
+// the KC x MP chunk (row q / MP, column q % MP; columns >= m are skipped);
+// MP is a compile-time constant, so the index arithmetic is shifts and
+// multiplies.
 template <int MT>
 struct ChunkLoad {
     static constexpr int NQ = (KC * 16 * MT + 255) / 256;
     float v[NQ];
+    double s;                 // s of row tid (threads < KC)
 };
 
-// ids, tapers and s*lc of chunk k0.. into LDS buffer buf (threads < KC)
-__device__ inline void chunk_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c,
-                                       double& sl)
+// list entries (ids, tapers) of chunk k0.. (threads < KC): contiguous loads
+__device__ inline void chunk_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c)
 {
     const int k = k0 + (int)threadIdx.x;
     if (threadIdx.x < KC && k < cnt) {
         id = a.ids[off + k];
         c = a.lc[off + k];
-        sl = a.sv[id] * c;
     } else {
         id = -1;
         c = 0.0;
-        sl = 0.0;
     }
 }
 
 template <int MT>
-__device__ inline void chunk_meta_store(GramLds<MT>& sh, int buf, int id, double c, double sl)
+__device__ inline void chunk_meta_store(GramLds<MT>& sh, int buf, int id, double c)
 {
     if (threadIdx.x < KC) {
         sh.id[buf][threadIdx.x] = id;
         sh.lc[buf][threadIdx.x] = c;
-        sh.sl[buf][threadIdx.x] = sl;
     }
 }
 
-// gathers of the chunk whose ids are in LDS buffer buf
+// gathers of the chunk whose ids are in meta buffer mbuf: rows of S and s
 template <int MT>
-__device__ inline void chunk_vals(const CalcArgs& a, const GramLds<MT>& sh, int buf, ChunkLoad<MT>& L)
+__device__ inline void chunk_vals(const CalcArgs& a, const GramLds<MT>& sh, int mbuf, ChunkLoad<MT>& L)
 {
+    constexpr int MP = 16 * MT;
     const int m = a.m;
 #pragma unroll
     for (int j = 0; j < ChunkLoad<MT>::NQ; ++j) {
         const int q = threadIdx.x + 256 * j;
-        const int row = q / m, colm = q - row * m;
-        const int id = (row < KC) ? sh.id[buf][row] : -1;
-        L.v[j] = (id >= 0) ? a.S[(size_t)id * m + colm] : 0.0f;
+        const int row = q / MP, col = q - row * MP;
+        const int id = (row < KC && col < m) ? sh.id[mbuf][row] : -1;
+        L.v[j] = (id >= 0) ? a.S[(long long)id * m + col] : 0.0f;
+    }
+    L.s = 0.0;
+    if (threadIdx.x < KC) {
+        const int id = sh.id[mbuf][threadIdx.x];
+        if (id >= 0)
+            L.s = a.sv[id];
     }
 }
 
-// rows of a chunk into row buffer rbuf, tapered with the coefficients in
-// meta buffer mbuf
+// rows of a chunk into row buffer rbuf; s * lc with the tapers of meta
+// buffer mbuf
 template <int MT>
 __device__ inline void chunk_store(GramLds<MT>& sh, int rbuf, int mbuf, int m, const ChunkLoad<MT>& L)
 {
+    constexpr int MP = 16 * MT;
 #pragma unroll
     for (int j = 0; j < ChunkLoad<MT>::NQ; ++j) {
         const int q = threadIdx.x + 256 * j;
-        const int row = q / m, colm = q - row * m;
-        if (row < KC)
-            sh.rows[rbuf][row][colm] = (double)L.v[j] * sh.lc[mbuf][row];
+        const int row = q / MP, col = q - row * MP;
+        if (row < KC && col < m)
+            sh.rows[rbuf][row][col] = L.v[j];
     }
+    if (threadIdx.x < KC)
+        sh.sl[rbuf][threadIdx.x] = L.s * sh.lc[mbuf][threadIdx.x];
+}
+
+// Position of element (i, j) of an m x m matrix (m <= 16 B) in the Gram
+// scratch: the factor kernel's thread (i / B, j / B) of its 16 x 16 thread
+// grid owns the B x B block starting at (B (i / B), B (j / B)), stored as one
+// contiguous run of B * B doubles per thread.
+template <int B>
+__host__ __device__ constexpr size_t gram_index(int i, int j)
+{
+    return ((size_t)((i / B) * 16 + (j / B)) * B + (i % B)) * B + (j % B);
 }
 
 // acc[T] += frag[I] frag[J]^T for the tiles (I, J) of wave W, with the tile
@@ -178,22 +199,20 @@ __device__ inline void mfma_tiles(const double (&frag)[MT], dbl4* acc)
 template <int MT, int W>
 __device__ inline void gram_wave_chunk(const GramLds<MT>& sh, int buf, int mbuf, dbl4 (&acc)[(tri_count(MT) - W + 3) / 4 > 0 ? (tri_count(MT) - W + 3) / 4 : 1], double (&sts)[MT])
 {
-    constexpr int NT = tri_count(MT);
-    constexpr int NTW = (NT - W + 3) / 4;
     const int lane = threadIdx.x & 63;
     const int kq = lane >> 4;
     const int col = lane & 15;
 #pragma unroll 2
     for (int s = 0; s < KC / 4; ++s) {
         const int r = 4 * s + kq;
+        const double c = sh.lc[mbuf][r];
         double frag[MT];
 #pragma unroll
         for (int X = 0; X < MT; ++X)
-            frag[X] = sh.rows[buf][r][16 * X + col];
-        (void)NTW;
+            frag[X] = (double)sh.rows[buf][r][16 * X + col] * c;
         mfma_tiles<MT, W, 0>(frag, acc);
         if (W == 0) {
-            const double sl = sh.sl[mbuf][r];
+            const double sl = sh.sl[buf][r];
 #pragma unroll
             for (int X = 0; X < MT; ++X)
                 sts[X] += frag[X] * sl;
@@ -207,7 +226,6 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
 {
     constexpr int NT = tri_count(MT);
     constexpr int NTW = (NT - W + 3) / 4;   // tiles of this wave
-    constexpr int MP = 16 * MT;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int kq = lane >> 4;   // which of the 4 obs of a k-step
@@ -225,39 +243,41 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
 
     // zero both row buffers once: padding columns stay zero
     for (int q = tid; q < 2 * KC * GramLds<MT>::RS; q += 256)
-        (&sh.rows[0][0][0])[q] = 0.0;
+        (&sh.rows[0][0][0])[q] = 0.0f;
     ChunkLoad<MT> L;
     const int nchunk = (cnt + KC - 1) / KC;
     int mid;
-    double mc, msl;
-    // prologue: meta of chunks 0 and 1, rows of chunk 0
-    chunk_meta_load(a, off, cnt, 0, mid, mc, msl);
-    chunk_meta_store<MT>(sh, 0, mid, mc, msl);
-    chunk_meta_load(a, off, cnt, KC, mid, mc, msl);
-    chunk_meta_store<MT>(sh, 1, mid, mc, msl);
+    double mc;
+    // prologue: list entries of chunks 0 and 1, rows of chunk 0
+    chunk_meta_load(a, off, cnt, 0, mid, mc);
+    chunk_meta_store<MT>(sh, 0, mid, mc);
+    chunk_meta_load(a, off, cnt, KC, mid, mc);
+    chunk_meta_store<MT>(sh, 1, mid, mc);
     __syncthreads();
     chunk_vals<MT>(a, sh, 0, L);
     chunk_store<MT>(sh, 0, 0, m, L);
     __syncthreads();
-    // iteration c: gathers of chunk c+1 and meta of chunk c+2 are in flight
-    // while chunk c feeds the MFMAs; one barrier per chunk
+    // iteration c: gathers of chunk c+1 and list entries of chunk c+2 are in
+    // flight while chunk c feeds the MFMAs; one barrier per chunk
     for (int c = 0; c < nchunk; ++c) {
         const bool more = c + 1 < nchunk;
         const bool more2 = c + 2 < nchunk;
         if (more)
             chunk_vals<MT>(a, sh, (c + 1) % 3, L);
         if (more2)
-            chunk_meta_load(a, off, cnt, (c + 2) * KC, mid, mc, msl);
+            chunk_meta_load(a, off, cnt, (c + 2) * KC, mid, mc);
         gram_wave_chunk<MT, W>(sh, c & 1, c % 3, acc, sts);
         if (more)
             chunk_store<MT>(sh, (c + 1) & 1, (c + 1) % 3, m, L);
         if (more2)
-            chunk_meta_store<MT>(sh, (c + 2) % 3, mid, mc, msl);
+            chunk_meta_store<MT>(sh, (c + 2) % 3, mid, mc);
         __syncthreads();
     }
     // Operand layouts of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l
     // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register
     // i of lane l holds D[4 i + l / 16][l % 16].
+    // Stored in the factor kernel's register-block layout (gram_index), both
+    // triangles.
 #pragma unroll
     for (int t = 0; t < NTW; ++t) {
         const int f = W + 4 * t;
@@ -266,9 +286,9 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
         for (int i = 0; i < 4; ++i) {
             const int r = 16 * I + 4 * i + kq;
             const int cc = 16 * J + col;
-            gout[(size_t)r * MP + cc] = acc[t][i];
+            gout[gram_index<MT>(r, cc)] = acc[t][i];
             if (I != J)
-                gout[(size_t)cc * MP + r] = acc[t][i];
+                gout[gram_index<MT>(cc, r)] = acc[t][i];
         }
     }
     if (W == 0) {
@@ -287,7 +307,14 @@ template <int MT>
 __global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
 {
     constexpr int MP = 16 * MT;
-    const int b = blockIdx.x;
+    // blocks are dispatched to the 8 XCDs round-robin; give each XCD a
+    // contiguous range of (node, segment) work items so that neighbouring
+    // nodes, whose local obs mostly coincide, share that XCD's L2
+    const int nwork = a.nbatch * a.nseg;
+    const int per = (nwork + 7) / 8;
+    const int b = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (b >= nwork)
+        return;
     const int node = a.node0 + b / a.nseg;
     const int seg = b % a.nseg;
     long long off;
@@ -314,86 +341,163 @@ __global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
 
 // ------------------------------------------------------------------ factor
 // 256 threads as a 16 x 16 grid; thread (ti, tj) owns the B x B block of rows
-// ti*B.., columns tj*B.. of the m x m matrix (B = ceil(m / 16)).
+// ti*B.., columns tj*B.. of the padded 16B x 16B matrix (B = ceil(m / 16));
+// the padding is kept as an identity block, which inversion leaves intact.
+template <int B>
 struct FactorShared {
-    double prow[2][256];
-    double pcol[2][256];
-    double red[256];
-    double vec[16][256];
+    double piv[2][B * B];          // inverse of the pivot block (double buffered)
+    double rowb[2][16][B * B];     // scaled pivot block-row, by block column
+    double colb[2][16][B * B];     // pivot block-column before the step, by block row
+    double red[4];
+    double vec[16][16 * B + 1];
     int flag;
 };
 
-// In-place inversion of the SPD matrix held in blocks `x` by Gauss-Jordan
-// elimination without pivoting (stable for SPD matrices, whose pivots are
-// the positive Schur-complement diagonals).  Returns sum(log(pivot)) (the log
-// of the determinant); sets sh.flag if a pivot is not positive.
-template <int B>
-__device__ inline double gj_invert(double (&x)[B][B], int m, FactorShared& sh)
+// Inverse of a small SPD block held by one thread (scalar Gauss-Jordan,
+// compile-time indices).  Returns sum(log(pivots)); sets *bad on a pivot <= 0.
+template <int B, bool LOGDET>
+__device__ inline double small_invert(double (&p)[B][B], int& bad)
 {
+    double logdet = 0.0;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+        const double d = p[k][k];
+        if (!(d > 0.0))
+            bad = 1;
+        if constexpr (LOGDET)
+            logdet += log(d);
+        const double rd = 1.0 / d;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            if (i == k)
+                continue;
+            const double f = p[i][k] * rd;
+#pragma unroll
+            for (int j = 0; j < B; ++j)
+                if (j != k)
+                    p[i][j] = fma(-f, p[k][j], p[i][j]);
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            if (j != k)
+                p[k][j] *= rd;
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+            if (i != k)
+                p[i][k] = -p[i][k] * rd;
+        p[k][k] = rd;
+    }
+    return logdet;
+}
+
+// In-place inversion of the SPD matrix held in blocks `x` by blocked
+// Gauss-Jordan elimination without pivoting (stable for SPD matrices, whose
+// pivot blocks are the SPD Schur complements).  Block step K:
+//   P = A_KK^-1;  A_KJ <- P A_KJ;  A_IJ <- A_IJ - A_IK A_KJ;  A_IK <- -A_IK P;
+//   A_KK <- P.
+// Returns sum(log(pivots)) when LOGDET; sets sh.flag on a non-positive pivot.
+template <int B, bool LOGDET = false>
+__device__ inline double gj_invert(double (&x)[B][B], int m, FactorShared<B>& sh)
+{
+    (void)m;
     const int tid = threadIdx.x;
     const int ti = tid >> 4, tj = tid & 15;
     double logdet = 0.0;
-    for (int k = 0; k < m; ++k) {
-        const int buf = k & 1;
-        const int kb = k / B, kr = k % B;
-        // (register arrays are only indexed with compile-time indices)
-        if (ti == kb) {
+    for (int K = 0; K < 16; ++K) {
+        const int buf = K & 1;
+        if (ti == K && tj == K) {
+            int bad = 0;
+            logdet = small_invert<B, LOGDET>(x, bad);
+            if (bad)
+                sh.flag = 1;
 #pragma unroll
-            for (int c = 0; c < B; ++c) {
-                double v = 0.0;
+            for (int r = 0; r < B; ++r)
 #pragma unroll
-                for (int rr = 0; rr < B; ++rr)
-                    v = (rr == kr) ? x[rr][c] : v;
-                sh.prow[buf][tj * B + c] = v;
-            }
-        }
-        if (tj == kb) {
-#pragma unroll
-            for (int r = 0; r < B; ++r) {
-                double v = 0.0;
-#pragma unroll
-                for (int cc = 0; cc < B; ++cc)
-                    v = (cc == kr) ? x[r][cc] : v;
-                sh.pcol[buf][ti * B + r] = v;
-            }
+                for (int c = 0; c < B; ++c)
+                    sh.piv[buf][r * B + c] = x[r][c];
         }
         __syncthreads();
-        const double d = sh.prow[buf][k];
-        if (!(d > 0.0) && tid == 0)
-            sh.flag = 1;
-        const double rd = 1.0 / d;
-        logdet += log(d);
-        double pc[B], pr[B];
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-            pc[r] = sh.pcol[buf][ti * B + r];
-#pragma unroll
-        for (int c = 0; c < B; ++c)
-            pr[c] = sh.prow[buf][tj * B + c];
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const int i = ti * B + r;
+        if (ti == K && tj != K) {
+            // A_KJ <- P A_KJ, one column at a time
 #pragma unroll
             for (int c = 0; c < B; ++c) {
-                const int j = tj * B + c;
-                double v = x[r][c];
-                if (i == k && j == k)
-                    v = rd;
-                else if (i == k)
-                    v = pr[c] * rd;
-                else if (j == k)
-                    v = -pc[r] * rd;
-                else
-                    v = v - (pc[r] * rd) * pr[c];
-                x[r][c] = v;
+                double t[B];
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int q = 0; q < B; ++q)
+                        s = fma(sh.piv[buf][r * B + q], x[q][c], s);
+                    t[r] = s;
+                }
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    x[r][c] = t[r];
+                    sh.rowb[buf][tj][r * B + c] = t[r];
+                }
+            }
+        }
+        if (tj == K && ti != K) {
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c)
+                    sh.colb[buf][ti][r * B + c] = x[r][c];
+        }
+        __syncthreads();
+        if (ti != K && tj != K) {
+            // A_IJ <- A_IJ - A_IK A_KJ as B rank-1 updates
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+                double aq[B], bq[B];
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+                    aq[r] = sh.colb[buf][ti][r * B + q];
+#pragma unroll
+                for (int c = 0; c < B; ++c)
+                    bq[c] = sh.rowb[buf][tj][q * B + c];
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+#pragma unroll
+                    for (int c = 0; c < B; ++c)
+                        x[r][c] = fma(-aq[r], bq[c], x[r][c]);
+            }
+        } else if (ti != K && tj == K) {
+            // A_IK <- -A_IK P, one row at a time
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                double t[B];
+#pragma unroll
+                for (int c = 0; c < B; ++c) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int q = 0; q < B; ++q)
+                        s = fma(x[r][q], sh.piv[buf][q * B + c], s);
+                    t[c] = -s;
+                }
+#pragma unroll
+                for (int c = 0; c < B; ++c)
+                    x[r][c] = t[c];
             }
         }
     }
     __syncthreads();
+    if constexpr (LOGDET) {
+        // the log-determinant accumulates in the diagonal-block owners; sum it
+        double v = (ti == tj) ? logdet : 0.0;
+        for (int o = 32; o > 0; o >>= 1)
+            v += __shfl_xor(v, o);
+        if ((tid & 63) == 0)
+            sh.red[tid >> 6] = v;
+        __syncthreads();
+        logdet = sh.red[0] + sh.red[1] + sh.red[2] + sh.red[3];
+        __syncthreads();
+    }
     return logdet;
 }
 
-__device__ inline double block_sum(double v, FactorShared& sh)
+template <int B>
+__device__ inline double block_sum(double v, FactorShared<B>& sh)
 {
     // deterministic block reduction: wave reduction then 4 partials in order
     for (int o = 32; o > 0; o >>= 1)
@@ -410,7 +514,7 @@ __device__ inline double block_sum(double v, FactorShared& sh)
 // Loads G0 = sum of the node's segment Gram matrices (one segment if seg >= 0)
 // into x (plus `diag` on the diagonal); returns the trace of G0.
 template <int B>
-__device__ inline double load_gram(const CalcArgs& a, int bnode, int seg, double diag, double (&x)[B][B], FactorShared& sh)
+__device__ inline double load_gram(const CalcArgs& a, int bnode, int seg, double diag, double (&x)[B][B], FactorShared<B>& sh)
 {
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
     const int mp = 16 * ((a.m + 15) / 16);
@@ -436,7 +540,7 @@ __device__ inline double load_gram(const CalcArgs& a, int bnode, int seg, double
             for (int c = 0; c < B; ++c) {
                 const int j = tj * B + c;
                 if (i < m && j < m)
-                    x[r][c] += g[(size_t)i * mp + j];
+                    x[r][c] += g[gram_index<B>(i, j)];
             }
         }
     }
@@ -449,6 +553,8 @@ __device__ inline double load_gram(const CalcArgs& a, int bnode, int seg, double
             if (i == j && i < m) {
                 tr += x[r][c];
                 x[r][c] += diag;
+            } else if (i == j) {
+                x[r][c] = 1.0;   // identity padding, left intact by the inversion
             }
         }
     }
@@ -457,7 +563,7 @@ __device__ inline double load_gram(const CalcArgs& a, int bnode, int seg, double
 
 // <X, G0> for the inverse in x against the node's Gram matrices
 template <int B>
-__device__ inline double dot_gram(const CalcArgs& a, int bnode, int seg, const double (&x)[B][B], FactorShared& sh)
+__device__ inline double dot_gram(const CalcArgs& a, int bnode, int seg, const double (&x)[B][B], FactorShared<B>& sh)
 {
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
     const int mp = 16 * ((a.m + 15) / 16);
@@ -478,7 +584,7 @@ __device__ inline double dot_gram(const CalcArgs& a, int bnode, int seg, const d
             for (int c = 0; c < B; ++c) {
                 const int j = tj * B + c;
                 if (i < m && j < m)
-                    acc += x[r][c] * g[(size_t)i * mp + j];
+                    acc += x[r][c] * g[gram_index<B>(i, j)];
             }
         }
     }
@@ -487,7 +593,7 @@ __device__ inline double dot_gram(const CalcArgs& a, int bnode, int seg, const d
 
 // y = X v for the node's S^T s (summed over segments); result in sh.vec[0][0..m)
 template <int B>
-__device__ inline void matvec_sts(const CalcArgs& a, int bnode, const double (&x)[B][B], FactorShared& sh)
+__device__ inline void matvec_sts(const CalcArgs& a, int bnode, const double (&x)[B][B], FactorShared<B>& sh)
 {
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
     const int mp = 16 * ((a.m + 15) / 16);
@@ -528,7 +634,7 @@ __device__ inline void matvec_sts(const CalcArgs& a, int bnode, const double (&x
 }
 
 template <int B>
-__device__ inline void write_T(const CalcArgs& a, int node, const double (&x)[B][B], double scale, double shift, FactorShared& sh)
+__device__ inline void write_T(const CalcArgs& a, int node, const double (&x)[B][B], double scale, double shift, FactorShared<B>& sh)
 {
     // T[e1][e2] = shift*delta + scale*X(e1, e2) for e1 < md, then relaxation
     // to prior T <- (T - I) alpha + I (transforms.c:814-828) and FP32 cast.
@@ -556,7 +662,7 @@ __device__ inline void write_T(const CalcArgs& a, int node, const double (&x)[B]
 template <int B, bool ETKF>
 __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
 {
-    __shared__ FactorShared sh;
+    __shared__ FactorShared<B> sh;
     const int bnode = blockIdx.x;
     const int node = a.node0 + bnode;
     const int tid = threadIdx.x;
@@ -593,7 +699,7 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
 
     double x[B][B];
     const double trG = load_gram<B>(a, bnode, -1, 1.0, x, sh);
-    const double logdetA = gj_invert<B>(x, m, sh);
+    const double logdetA = gj_invert<B, ETKF>(x, m, sh);
     // w = M^-1 S^T s
     matvec_sts<B>(a, bnode, x, sh);
     if (tid < m) {
@@ -625,7 +731,7 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
             for (int r = 0; r < B; ++r)
 #pragma unroll
                 for (int c = 0; c < B; ++c)
-                    z[r][c] = (ti * B + r == tj * B + c && ti * B + r < m) ? 1.0 : 0.0;
+                    z[r][c] = (ti * B + r == tj * B + c) ? 1.0 : 0.0;
             double logdetY = logdetA;
             double logdetZ = 0.0;
             bool last = false;
@@ -637,7 +743,7 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
 #pragma unroll
                         for (int c = 0; c < B; ++c)
                             x[r][c] = y[r][c];
-                    logdetY = gj_invert<B>(x, m, sh);
+                    logdetY = gj_invert<B, true>(x, m, sh);
                 }
 #pragma unroll
                 for (int r = 0; r < B; ++r)
@@ -645,7 +751,7 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
                     for (int c = 0; c < B; ++c)
                         u[r][c] = z[r][c];
                 if (it > 0)
-                    logdetZ = gj_invert<B>(u, m, sh);
+                    logdetZ = gj_invert<B, true>(u, m, sh);
                 // (for it == 0, Z = I is its own inverse)
                 const double mu = last ? 1.0 : exp(-(logdetY + logdetZ) / (2.0 * m));
                 const double hm = 0.5 * mu, hi = 0.5 / mu;
@@ -746,7 +852,7 @@ hipError_t launch_transpose_f32(hipStream_t s, const float* in, float* out, long
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a)
 {
     const int MT = (a.m + 15) / 16;
-    dim3 grid((unsigned)(a.nbatch * a.nseg));
+    dim3 grid((unsigned)(8 * ((a.nbatch * a.nseg + 7) / 8)));
     if (a.nbatch == 0)
         return hipSuccess;
     switch (MT) {
