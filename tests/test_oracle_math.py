@@ -1,0 +1,340 @@
+"""The CPU oracle against what the mathematics fixes: published closed forms,
+textbook library routines (numpy.linalg), brute force on small inputs and
+invariants of the update.  Each check is chosen so that a dropped term, a
+wrong sign, a transposed operand or an off-by-one index fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import synth
+
+R_EARTH = 6371.0
+
+
+# ---------------------------------------------------------------- taper
+def gc_textbook(r, c):
+    """Gaspari & Cohn (1999), QJRMS 125, eq. (4.10), half-width c."""
+    z = abs(r) / c
+    if z <= 1.0:
+        return -0.25 * z ** 5 + 0.5 * z ** 4 + 0.625 * z ** 3 - 5.0 / 3.0 * z ** 2 + 1.0
+    if z <= 2.0:
+        return z ** 5 / 12.0 - 0.5 * z ** 4 + 0.625 * z ** 3 + 5.0 / 3.0 * z ** 2 - 5.0 * z + 4.0 - 2.0 / (3.0 * z)
+    return 0.0
+
+
+def test_taper_closed_forms():
+    assert O.taper_gc(0.0) == 1.0
+    assert O.taper_gc(1.0) == 0.0 and O.taper_gc(1.5) == 0.0
+    assert abs(O.taper_gc(0.5) - 5.0 / 24.0) < 1e-15
+    # both branches meet at x = 1/2 in value and slope; the taper and its
+    # slope vanish at x = 1
+    h = 1e-7
+    assert abs(O.taper_gc(0.5 - h) - O.taper_gc(0.5 + h)) < 1e-6
+    d_lo = (O.taper_gc(0.5 - h) - O.taper_gc(0.5 - 2 * h)) / h
+    d_hi = (O.taper_gc(0.5 + 2 * h) - O.taper_gc(0.5 + h)) / h
+    assert abs(d_lo - d_hi) < 1e-5
+    assert O.taper_gc(1.0 - h) < 1e-12
+    for x in np.linspace(0.0, 0.999, 301):
+        assert abs(O.taper_gc(x) - gc_textbook(x, 0.5)) < 2e-15, x
+
+
+def test_calclcoeff_weighted_sum():
+    """obstypes.c:800-822: sum over radii of weight * GC(dist / radius)."""
+    for d in (0.0, 10.0, 49.0, 120.0, 199.0, 260.0):
+        assert O.calclcoeff([200.0], [1.0], d) == O.taper_gc(d / 200.0)
+        two = O.calclcoeff([100.0, 250.0], [0.3, 0.7], d)
+        ref = 0.3 * gc_textbook(d, 50.0) + 0.7 * gc_textbook(d, 125.0)
+        assert abs(two - ref) < 1e-14
+
+
+# ---------------------------------------------------------------- geometry
+def test_ll2xyz_chord():
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        lo1, lo2 = rng.uniform(-180, 360, 2)
+        la1, la2 = rng.uniform(-89, 89, 2)
+        a = np.array(O.ll2xyz(lo1, la1))
+        b = np.array(O.ll2xyz(lo2, la2))
+        assert abs(np.linalg.norm(a) - R_EARTH) < 1e-9
+        p1, p2, dl = math.radians(la1), math.radians(la2), math.radians(lo2 - lo1)
+        hav = math.sin((p2 - p1) / 2) ** 2 + math.cos(p1) * math.cos(p2) * math.sin(dl / 2) ** 2
+        dsig = 2 * math.asin(math.sqrt(hav))
+        assert abs(np.linalg.norm(a - b) - 2 * R_EARTH * math.sin(dsig / 2)) < 1e-8
+
+
+# ---------------------------------------------------------------- kd-tree
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_kd_range_brute_force(ndim):
+    rng = np.random.default_rng(11 + ndim)
+    pts = np.round(rng.uniform(0, 100, (3000, ndim)), 2)
+    pts[500:520] = pts[499]                      # duplicates
+    t = O.OracleKd(pts)
+    for q in rng.uniform(-5, 105, (40, ndim)):
+        for r in (0.5, 7.0, 30.0):
+            d, d2 = t.range(q, r)
+            dd = ((pts - q) ** 2).sum(1)
+            want = np.flatnonzero(dd <= r * r)
+            assert np.array_equal(np.sort(d), want)
+            assert np.allclose(d2, dd[d], rtol=1e-15, atol=0)
+            do, d2o = t.range(q, r, ordered=True)
+            assert np.array_equal(np.sort(do), want)
+            assert np.all(np.diff(d2o) >= 0)
+    # common/kdtree.c:538 prunes the far child on |dx| < range (strict), so a
+    # zero-radius search at a duplicated point stops at the first copy: the
+    # later copies were inserted to its right (kd_insertnode goes left only
+    # on coord < node) and |dx| = 0 is not < 0
+    d, _ = t.range(pts[499], 0.0)
+    assert d.tolist() == [499]
+    d, _ = t.range(pts[499], 1e-9)
+    assert set(d.tolist()) == set(range(499, 520))
+
+
+# ---------------------------------------------------------------- local search
+def test_findlocal_brute_force():
+    """obs_findlocal (calc/obs.c:1064-1133): per type, every good observation
+    within the type's largest radius, tapered by calclcoeff."""
+    P = synth.make_problem(seed=2, nj=10, ni=12, nmem=12, nobs=1500, locrad=150.0, profiles=40)
+    P.obstypes[1]["locrad"] = [80.0, 220.0]
+    P.obstypes[1]["locweight"] = [2.0, 1.0]
+    status = P.obs_status.copy()
+    status[::13] = 2
+    ob = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd, status=status)
+    xyz = np.array([O.ll2xyz(a, b) for a, b in zip(P.obs_lon.astype(np.float64), P.obs_lat.astype(np.float64))])
+    for lon, lat in zip(P.node_lon.ravel()[::7], P.node_lat.ravel()[::7]):
+        ids, lc = ob.findlocal(lon, lat)
+        q = np.array(O.ll2xyz(lon, lat))
+        want_ids, want_lc = [], []
+        for t, ot in enumerate(P.obstypes):
+            rmax = max(ot["locrad"])
+            lw = np.array(ot.get("locweight", [1.0] * len(ot["locrad"])), float)
+            lw /= lw.sum()
+            d2 = ((xyz - q) ** 2).sum(1)
+            sel = np.flatnonzero((P.obs_type == t) & (status == 0) & (d2 <= rmax * rmax))
+            want_ids += sel.tolist()
+            for o in sel:
+                d = math.sqrt(d2[o])
+                want_lc.append(sum(w * gc_textbook(d, r / 2.0) for r, w in zip(ot["locrad"], lw)))
+        order = np.argsort(ids)
+        assert np.array_equal(ids[order], np.sort(want_ids))
+        wl = np.array(want_lc)[np.argsort(want_ids)]
+        assert np.allclose(lc[order], wl, rtol=0, atol=1e-12)
+        # types are listed in order
+        assert np.all(np.diff(P.obs_type[ids]) >= 0)
+
+
+# ---------------------------------------------------------------- per-column algebra
+def spd_inv_sqrt(M):
+    lam, U = np.linalg.eigh(M)
+    return (U / np.sqrt(lam)) @ U.T
+
+
+@pytest.mark.parametrize("m,mout,p", [(10, 10, 1), (10, 10, 4), (10, 10, 10), (10, 7, 25), (40, 40, 17),
+                                      (40, 32, 90)])
+def test_local_analysis_closed_forms(m, mout, p):
+    rng = np.random.default_rng(m * 1000 + p)
+    S = rng.standard_normal((p, m)) * 0.2
+    s = rng.standard_normal(p) * 0.3
+    M = np.eye(m) + S.T @ S
+    Minv = np.linalg.inv(M)
+    w_ref = np.linalg.solve(M, S.T @ s)
+    dfs_ref = np.trace(Minv @ S.T @ S)
+    srf_ref = math.sqrt(np.trace(S.T @ S) / dfs_ref) - 1.0
+    T, w, dfs, srf = O.local_analysis(O.MODE_ENKF, O.SCHEME_DENKF, m, mout, S, s)
+    assert np.allclose(w, w_ref, rtol=0, atol=1e-12)
+    assert np.allclose(T, (0.5 * (np.eye(m) + Minv))[:mout], rtol=0, atol=1e-12)
+    assert abs(dfs - dfs_ref) < 1e-11 and abs(srf - srf_ref) < 1e-11
+    T, w, _, _ = O.local_analysis(O.MODE_ENKF, O.SCHEME_ETKF, m, mout, S, s)
+    assert np.allclose(w, w_ref, rtol=0, atol=1e-12)
+    assert np.allclose(T, spd_inv_sqrt(M)[:mout], rtol=0, atol=1e-11)
+    _, w, _, _ = O.local_analysis(O.MODE_ENOI, O.SCHEME_DENKF, m, mout, S, s)
+    assert np.allclose(w, w_ref, rtol=0, atol=1e-12)
+
+
+def test_single_observation():
+    """p = 1: M = I + a a^T, so M^-1 = I - a a^T / (1 + |a|^2) (Sherman-
+    Morrison), dfs = |a|^2 / (1 + |a|^2), srf = sqrt(1 + |a|^2) - 1."""
+    m = 16
+    a = np.linspace(-0.4, 0.5, m)
+    s0 = 0.7
+    n2 = a @ a
+    T, w, dfs, srf = O.local_analysis(O.MODE_ENKF, O.SCHEME_DENKF, m, m, a[None, :], np.array([s0]))
+    assert np.allclose(w, a * s0 / (1 + n2), atol=1e-15)
+    assert np.allclose(T, np.eye(m) - 0.5 * np.outer(a, a) / (1 + n2), atol=1e-15)
+    assert abs(dfs - n2 / (1 + n2)) < 1e-15
+    assert abs(srf - (math.sqrt(1 + n2) - 1)) < 1e-14
+    T, _, _, _ = O.local_analysis(O.MODE_ENKF, O.SCHEME_ETKF, m, m, a[None, :], np.array([s0]))
+    assert np.allclose(T, np.eye(m) + (1 / math.sqrt(1 + n2) - 1) * np.outer(a, a) / n2, atol=1e-14)
+
+
+def test_calc_loop_relaxation_and_pertype_diagnostics():
+    """das_calctransforms (calc/transforms.c:605-985): T relaxed to the prior
+    as I + alpha (T - I), per-type diagnostics equal separate analyses of each
+    type's observations, empty columns give T = I, w = 0."""
+    P = synth.make_problem(seed=4, nj=8, ni=9, nmem=16, nobs=700, locrad=90.0, profiles=30)
+    ob = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd)
+    S, s = O.standardise(16, ob, P.HA, P.innov)
+    lon = np.concatenate([P.node_lon.ravel(), [P.node_lon.max() + 30.0]])
+    lat = np.concatenate([P.node_lat.ravel(), [P.node_lat.mean()]])
+    alpha = 0.6
+    r = O.calc_transforms(O.MODE_ENKF, O.SCHEME_ETKF, 16, 12, alpha, ob, S, s, lon, lat)
+    nt = len(P.obstypes)
+    for n in range(0, len(lon) - 1, 5):
+        ids, lc = ob.findlocal(lon[n], lat[n])
+        Sl = S[:, ids].T.astype(np.float64) * lc[:, None]
+        sl = s[ids] * lc
+        T, w, dfs, srf = O.local_analysis(O.MODE_ENKF, O.SCHEME_ETKF, 16, 12, Sl, sl)
+        Trel = np.eye(12, 16) + alpha * (T - np.eye(12, 16))
+        assert np.allclose(r["T64"][n], Trel, rtol=0, atol=1e-14)
+        assert np.array_equal(r["w64"][n], w)
+        assert r["pnlobs"][nt, n] == len(ids)
+        assert r["pdfs"][nt, n] == np.float32(dfs)
+        for t in range(nt):
+            sel = P.obs_type[ids] == t
+            assert r["pnlobs"][t, n] == sel.sum()
+            if sel.any():
+                _, _, dt, st = O.local_analysis(O.MODE_ENOI, O.SCHEME_DENKF, 16, 16, Sl[sel], sl[sel])
+                assert abs(r["pdfs"][t, n] - dt) <= 2e-7 * abs(dt)
+                assert abs(r["psrf"][t, n] - st) <= 2e-7 * max(abs(st), 1.0)
+    last = len(lon) - 1
+    assert np.array_equal(r["T64"][last], np.eye(12, 16)) and not r["w64"][last].any()
+    assert not r["pnlobs"][:, last].any()
+
+
+# ---------------------------------------------------------------- update
+def update_problem(nj=10, ni=11, m=8, md=8, stride=3, nk=2, seed=0):
+    rng = np.random.default_rng(seed)
+    nj_s, ni_s = (nj - 1) // stride + 1, (ni - 1) // stride + 1
+    T = (np.eye(md, m)[None, None] + 0.1 * rng.standard_normal((nj_s, ni_s, md, m))).astype(np.float32)
+    w = (0.1 * rng.standard_normal((nj_s, ni_s, m))).astype(np.float32)
+    nlev = np.full((nj, ni), nk, np.int32)
+    E = rng.standard_normal((m, nj, ni)).astype(np.float32) * 2.0 + 5.0
+    return T, w, nlev, E
+
+
+def brute_cell(v, T, w, md, mode=O.MODE_ENKF, inflation=1.0):
+    m = len(v)
+    v = v.astype(np.float64)
+    mean = v[:md].mean()
+    a = v - mean
+    va = T.astype(np.float64) @ a
+    mean_a = mean + a @ w.astype(np.float64)
+    if mode == O.MODE_HYBRID and md > 1:
+        va /= math.sqrt((m - 1) / (md - 1))
+    return va * inflation + mean_a
+
+
+def test_update_identity_transform_is_noop():
+    T, w, nlev, E = update_problem(md=8)
+    T[:] = np.eye(8, dtype=np.float32)
+    w[:] = 0.0
+    buf = E.copy()
+    O.update_fields(O.MODE_ENKF, 8, 8, 3, 0, 0, nlev, T, w, [dict(level=0)], [buf], 0, 9)
+    assert np.allclose(buf, E, rtol=2e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("mode,md", [(O.MODE_ENKF, 8), (O.MODE_HYBRID, 5)])
+def test_update_brute_force_at_nodes_and_between(mode, md):
+    """At stride nodes the transforms are used as stored; between nodes they
+    are the bilinear blend (update.c:195-272), j first, then i."""
+    m, stride = 8, 3
+    T, w, nlev, E = update_problem(m=m, md=md, stride=stride)
+    buf = E.copy()
+    O.update_fields(mode, m, md, stride, 0, 0, nlev, T, w, [dict(level=0)], [buf], 0, 9)
+    for j in range(10):
+        for i in range(11):
+            jj, ii = j // stride, i // stride
+            fj, fi = (j % stride) / stride, (i % stride) / stride
+            jj2 = min(jj + 1, T.shape[0] - 1)
+            ii2 = min(ii + 1, T.shape[1] - 1)
+            Tc = ((1 - fj) * ((1 - fi) * T[jj, ii] + fi * T[jj, ii2]) + fj * ((1 - fi) * T[jj2, ii] + fi * T[jj2, ii2]))
+            wc = ((1 - fj) * ((1 - fi) * w[jj, ii] + fi * w[jj, ii2]) + fj * ((1 - fi) * w[jj2, ii] + fi * w[jj2, ii2]))
+            want = brute_cell(E[:, j, i], Tc, wc, md, mode)
+            assert np.allclose(buf[:md, j, i], want, rtol=1e-5, atol=1e-5), (j, i)
+            assert np.array_equal(buf[md:, j, i], E[md:, j, i])
+
+
+def test_update_periodic_wrap():
+    m = 6
+    T, w, nlev, E = update_problem(ni=12, m=m, md=m, stride=3)   # ni_s = 4, last column 11 wraps
+    a = E.copy()
+    b = E.copy()
+    O.update_fields(O.MODE_ENKF, m, m, 3, 0, 0, nlev, T, w, [dict(level=0)], [a], 0, 9)
+    O.update_fields(O.MODE_ENKF, m, m, 3, 1, 0, nlev, T, w, [dict(level=0)], [b], 0, 9)
+    assert np.array_equal(a[:, :, :10], b[:, :, :10])
+    Tc = (1 / 3) * T[0, 3] + (2 / 3) * T[0, 0]
+    wc = (1 / 3) * w[0, 3] + (2 / 3) * w[0, 0]
+    assert np.allclose(b[:, 0, 11], brute_cell(E[:, 0, 11], Tc, wc, m), rtol=1e-5, atol=1e-5)
+    assert not np.allclose(a[:, 0, 11], b[:, 0, 11])
+
+
+def test_update_masks_missing_and_increments():
+    m, md = 8, 6
+    T, w, nlev, E = update_problem(m=m, md=md, stride=1, nj=4, ni=4)
+    nlev[1, 1] = 0
+    E[2, 2, 2] = np.nan            # dynamic member missing: whole cell zeroed
+    E[7, 3, 3] = 1e7               # static member out of range: statics zeroed
+    buf = E.copy()
+    O.update_fields(O.MODE_ENKF, m, md, 1, 0, 0, nlev, T, w, [dict(level=0)], [buf], 0, 3)
+    assert not buf[:, 1, 1].any() and not buf[:, 2, 2].any()
+    assert not buf[md:, 3, 3].any()
+    inc = E.copy()
+    O.update_fields(O.MODE_ENKF, m, md, 1, 0, 0, nlev, T, w, [dict(level=0)], [inc], 0, 3, outputinc=1)
+    ok = np.ones((4, 4), bool)
+    ok[1, 1] = ok[2, 2] = ok[3, 3] = False
+    assert np.allclose(inc[:md][:, ok], buf[:md][:, ok] - E[:md][:, ok], rtol=1e-5, atol=1e-5)
+
+
+def test_update_inflation_fixed_and_capped():
+    m = 8
+    T, w, nlev, E = update_problem(m=m, md=m, stride=1, nj=4, ni=4)
+    T = (0.7 * T).astype(np.float32)   # shrinks the spread, so the capped factor engages
+    fixed, capped = E.copy(), E.copy()
+    infl = O.update_fields(O.MODE_ENKF, m, m, 1, 0, 0, nlev, T, w,
+                           [dict(level=0, inflation=1.1, inf_ratio=float("nan"))], [fixed], 0, 3, want_infl=True)
+    assert np.all(infl == np.float32(1.1))
+    infl2 = O.update_fields(O.MODE_ENKF, m, m, 1, 0, 0, nlev, T, w,
+                            [dict(level=0, inflation=2.0, inf_ratio=0.5)], [capped], 0, 3, want_infl=True)
+    for j in range(4):
+        for i in range(4):
+            v = E[:, j, i].astype(np.float64)
+            a = v - v.mean()
+            va = T[j, i].astype(np.float64) @ a
+            want = min(math.sqrt((a @ a) / (va @ va)) * 0.5 + 0.5, 2.0)
+            assert abs(infl2[0, j, i] - want) < 1e-5
+            assert np.allclose(fixed[:, j, i], brute_cell(E[:, j, i], T[j, i], w[j, i], m, inflation=1.1),
+                               rtol=1e-5, atol=1e-5)
+            assert np.allclose(capped[:, j, i], brute_cell(E[:, j, i], T[j, i], w[j, i], m, inflation=want),
+                               rtol=1e-5, atol=1e-5)
+
+
+def test_update_bg_increment():
+    """das_updatebg (update.c:568-814): x^a = x^f + (E - mean(E)) w."""
+    m = 8
+    _, w, nlev, E = update_problem(m=m, md=m, stride=2, nj=7, ni=7)
+    bg = np.random.default_rng(1).standard_normal((1, 7, 7)).astype(np.float32)
+    buf = np.concatenate([E, bg])
+    O.update_bg(m, 2, 0, 0, nlev, w, [dict(level=0)], [buf], 0, 6)
+    for j in (0, 2, 4, 6):
+        for i in (0, 2, 4, 6):
+            v = E[:, j, i].astype(np.float64)
+            want = bg[0, j, i] + (v - v.mean()) @ w[j // 2, i // 2].astype(np.float64)
+            assert abs(buf[m, j, i] - want) < 1e-5 * max(1.0, abs(want))
+
+
+def test_sethybridensemble_closed_form():
+    """das_sethybridensemble (common/dasystem.c:775-819): dynamic anomalies
+    scaled by sqrt((m-1)/(md-1)) about the dynamic mean; static members
+    scaled by sqrt(gamma (m-1)/(ms-1)) and shifted by the dynamic mean."""
+    m, md, gamma = 10, 6, 0.3
+    v = np.random.default_rng(3).standard_normal((m, 50)).astype(np.float32)
+    out = v.copy()
+    O.sethybridensemble(m, md, gamma, out)
+    mean = v[:md].astype(np.float64).mean(0)
+    kd = math.sqrt((m - 1) / (md - 1))
+    ks = math.sqrt(gamma * (m - 1) / (m - md - 1))
+    assert np.allclose(out[:md], (v[:md] - mean) * kd + mean, rtol=1e-6, atol=1e-6)
+    assert np.allclose(out[md:], v[md:] * ks + mean, rtol=1e-6, atol=1e-6)
+    assert np.allclose(out[:md].astype(np.float64).mean(0), mean, atol=1e-6)
