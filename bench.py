@@ -34,6 +34,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(workload_tag, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
+    by the reading of a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_hbm_traffic.md" % workload_tag)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == "`%s`" % kernel:
+            try:
+                return float(cells[4]) * 1e9
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(X, scheme_id, md, sample_target_s=15.0, max_nodes=4000):
     """The reference kernels (oracle/_ref: calc/calcs.c + common/kdtree.c with
     MKL, one thread) or, when they are not built, the oracle restatement, timed
@@ -77,6 +94,8 @@ def main():
     ap.add_argument("--scheme", choices=["denkf", "etkf"], default="denkf")
     ap.add_argument("--size", choices=["full", "small"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "both"], default="both",
+                    help="both: the FP64 path (value) and then the mixed path, reported beside it")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -99,60 +118,76 @@ def main():
     X = synth_gpu.make_tile(**tile, seed=0)
     log("  done in %.1f s: %d obs, %d x %d nodes, %d fields" % (time.perf_counter() - t0, X.nobs, X.nj_s, X.ni_s,
                                                                  X.nfield))
-    E = enkfg.Engine(X.nmem, mode=enkfg.MODE_ENKF, scheme=scheme_id, pertype_diag=True, keep_fp64=False)
-    E.set_obstypes(X.obstypes)
-    t0 = time.perf_counter()
-    E.set_obs(X.obs_type, X.obs_status, X.obs_lon, X.obs_lat, X.obs_estd)
-    t_tree = time.perf_counter() - t0
-    log("  obs + kd-trees set in %.1f s" % t_tree)
-    E.set_S(X.HA, X.innov, standardised=False)
-    log("  S uploaded and standardised")
-
-    def step(attach=False):
-        E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
-        tc = E.timings()
-        if attach:
-            E.ens_attach(X.nlevels, X.stride, 0, 0, X.fields, X.E.data_ptr())
-        E.ens_update()
-        tu = E.timings()
-        return tc[:3] + [tu[3]]
-
-    for w in range(args.warmup):
-        ph = step(attach=(w == 0))
-        log("  warmup %d: phases (ms) loc %.1f gram %.1f factor %.1f update %.1f" % (w, *ph))
-    if args.warmup == 0:
-        E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
-        E.ens_attach(X.nlevels, X.stride, 0, 0, X.fields, X.E.data_ptr())
-    torch.cuda.synchronize()
-    E.sync()
-    phases = np.zeros(4)
-    tcalc = tupd = 0.0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        a = time.perf_counter()
-        E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
-        b = time.perf_counter()
-        ph = E.timings()
-        E.ens_update()
-        c = time.perf_counter()
-        ph[3] = E.timings()[3]
-        phases += np.array(ph)
-        tcalc += b - a
-        tupd += c - b
-        log("  step %d: calc %.1f ms update %.1f ms (loc %.1f gram %.1f factor %.1f update-kernel %.1f)"
-            % (k, 1e3 * (b - a), 1e3 * (c - b), *ph))
-    E.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    K = args.steps
-    phases /= K
-    st = E.stats()
-    n_an = X.nj_s * X.ni_s
     m = X.nmem
     md = m
+    n_an = X.nj_s * X.ni_s
     nfield, ncell = X.nfield, X.ncell
-    calc_rate = n_an / (tcalc / K)
-    upd_rate = nfield * ncell * md / (tupd / K)
+    # a band of stride rows in the middle of the tile, for the mixed-versus-FP64 comparison
+    jb = X.nj_s // 2
+    sample = (X.node_lon[jb:jb + 8], X.node_lat[jb:jb + 8])
+
+    def run(precision):
+        """warmup + timed steps of one precision; returns rates, phases and the
+        transforms of the comparison band"""
+        E = enkfg.Engine(m, mode=enkfg.MODE_ENKF, scheme=scheme_id, precision=precision, pertype_diag=True,
+                         keep_fp64=False)
+        E.set_obstypes(X.obstypes)
+        t0 = time.perf_counter()
+        E.set_obs(X.obs_type, X.obs_status, X.obs_lon, X.obs_lat, X.obs_estd)
+        t_tree = time.perf_counter() - t0
+        E.set_S(X.HA, X.innov, standardised=False)
+        log("  [%s] obs + kd-trees set in %.1f s, S uploaded and standardised" % (pname[precision], t_tree))
+        for w in range(max(args.warmup, 1)):
+            E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
+            ph = E.timings()
+            if w == 0:
+                E.ens_attach(X.nlevels, X.stride, 0, 0, X.fields, X.E.data_ptr())
+            if w < args.warmup:
+                E.ens_update()
+                ph[3] = E.timings()[3]
+                log("  [%s] warmup %d: phases (ms) loc %.1f gram %.1f factor %.1f update %.1f"
+                    % (pname[precision], w, *ph))
+        torch.cuda.synchronize()
+        E.sync()
+        phases = np.zeros(4)
+        tcalc = tupd = 0.0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            a = time.perf_counter()
+            E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
+            b = time.perf_counter()
+            ph = E.timings()
+            E.ens_update()
+            c = time.perf_counter()
+            ph[3] = E.timings()[3]
+            phases += np.array(ph)
+            tcalc += b - a
+            tupd += c - b
+            log("  [%s] step %d: calc %.1f ms update %.1f ms (loc %.1f gram %.1f factor %.1f update-kernel %.1f)"
+                % (pname[precision], k, 1e3 * (b - a), 1e3 * (c - b), *ph))
+        E.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        K = args.steps
+        st = E.stats()
+        nl = E.launches()
+        # transforms of the comparison band (after the timed region)
+        E.calc_transforms(*sample, stride=X.stride)
+        Tb, wb, _, _ = E.transforms()
+        pnb, pdb, _ = E.diag()
+        E.close()
+        return dict(calc_rate=n_an / (tcalc / K), upd_rate=nfield * ncell * md / (tupd / K), phases=phases / K,
+                    tcalc=tcalc / K, tupd=tupd / K, wall=wall, stats=st, launches=nl, t_tree=t_tree,
+                    band=(Tb, wb, pdb))
+
+    pname = {enkfg.PREC_FP64: "fp64", enkfg.PREC_MIXED: "mixed"}
+    R = run(enkfg.PREC_FP64)
+    Rm = run(enkfg.PREC_MIXED) if args.precision == "both" else None
+    K = args.steps
+    phases = R["phases"]
+    st = R["stats"]
+    calc_rate, upd_rate = R["calc_rate"], R["upd_rate"]
+    tag = "tile_fp64_%s" % args.scheme if args.size == "full" else "small_fp64_%s" % args.scheme
 
     # algorithmic work (per step)
     p_sum = st["nlobs_sum"]
@@ -167,21 +202,27 @@ def main():
     dom = max(kern, key=kern.get)
     if dom == "update":
         ach = B_upd / (phases[3] * 1e-3)
+        kname = "update_cell_kernel<%d>" % ((m + 15) // 16)
         roof = dict(bound="hbm", achieved=ach / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=ach / PEAK_HBM,
-                    traffic=None, kernel="update_kernel<6>",
+                    traffic=profiled_traffic(tag, kname), kernel=kname, launches_per_step=1,
+                    avg_launch_ms=phases[3],
                     algorithmic="4*m*(2*wet cell-fields + dry cell-fields) + 4*N_an*(m*m_d+m) = %.3e B per launch"
                                 % B_upd)
     elif dom == "gram":
         ach = F_gram / (phases[1] * 1e-3)
+        kname = "gram_kernel<%d>" % ((m + 15) // 16)
+        nl = R["launches"]
         roof = dict(bound="mfma", achieved=ach / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s", frac=ach / PEAK_FP64,
-                    traffic=None, kernel="gram_kernel<6> (FP64 MFMA)",
-                    algorithmic="sum_nodes p*(m(m+1)+2m) = %.3e flop per step (all launches)" % F_gram)
+                    traffic=profiled_traffic(tag, kname), kernel=kname + " (FP64 MFMA)",
+                    launches_per_step=nl, avg_launch_ms=phases[1] / max(nl, 1),
+                    algorithmic="sum_nodes p*(m(m+1)+2m) = %.3e flop per step over %d launches" % (F_gram, nl))
     else:
         roof = dict(bound="latency", achieved=None, peak=None, unit=None, frac=None, traffic=None,
                     kernel=dom, algorithmic="dominant phase is %s" % dom)
 
     res = dict(metric=METRIC, value=calc_rate, unit="local analyses/s", n_gpus=1, steps=K, warmup=args.warmup,
-               ms_per_step=1e3 * wall / K, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               ms_per_step=1e3 * (R["wall"]) / K, higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f64",
                data="synthetic (GPU-generated Gaussian random fields; no files)",
                config=dict(workload="tile %dx%dx%d, m=%d, %d obs, %s, FP64, stride %d, locrad 250 km, "
                                     "%d local analyses, %d fields" % (X.nk, X.nj, X.ni, m, X.nobs,
@@ -190,12 +231,30 @@ def main():
                            precision="fp64", scheme=args.scheme, mean_local_obs=p_sum / n_an),
                update_cell_member_per_s=upd_rate,
                phases_ms=dict(localise=phases[0], gram=phases[1], factor=phases[2], update=phases[3],
-                              calc_wall=1e3 * tcalc / K, update_wall=1e3 * tupd / K),
-               kd_tree_build_s=t_tree,
+                              calc_wall=1e3 * R["tcalc"], update_wall=1e3 * R["tupd"]),
+               kd_tree_build_s=R["t_tree"],
                roofline=roof)
+    if Rm is not None:
+        Tb, wb, pdb = R["band"]
+        Tm, wm, pdm = Rm["band"]
+        nz = pdb != 0
+        F_mix = 3 * p_sum * (m * (m + 1) + 2 * m)       # three split products
+        pm = Rm["phases"]
+        res["mixed"] = dict(
+            precision="S^T S as BF16 hi/lo MFMA (3 products, FP32 accumulation); factorisation FP64; update FP32",
+            local_analyses_per_s=Rm["calc_rate"], update_cell_member_per_s=Rm["upd_rate"],
+            calc_speedup_vs_fp64=Rm["calc_rate"] / calc_rate,
+            step_speedup_vs_fp64=(R["tcalc"] + R["tupd"]) / (Rm["tcalc"] + Rm["tupd"]),
+            phases_ms=dict(localise=pm[0], gram=pm[1], factor=pm[2], update=pm[3],
+                           calc_wall=1e3 * Rm["tcalc"], update_wall=1e3 * Rm["tupd"]),
+            gram_bf16_tflops=F_mix / (pm[1] * 1e-3) / 1e12,
+            error_vs_fp64=dict(
+                sample="8 stride rows (%d nodes) in the middle of the tile" % (8 * X.ni_s),
+                T=float(np.abs(Tm - Tb).max() / np.abs(Tb).max()),
+                w=float(np.abs(wm - wb).max() / max(np.abs(wb).max(), 1e-30)),
+                dfs=float(np.abs(pdm[nz] - pdb[nz]).max() / np.abs(pdb[nz]).max())))
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
-        E.close()
         del X.E
         torch.cuda.empty_cache()
         res["cpu_baseline"] = cpu_baseline(X, scheme_id, md)

@@ -21,7 +21,10 @@
 // The identities used are those of the User Guide (doc/enkf-userguide.tex:
 // 440-451, 490-522); the per-type diagnostics re-solve the analysis with only
 // the obs of one type, as the USE_MPIQUEUE build does (transforms.c:853-879).
+#include <algorithm>
+
 #include "enkfg_internal.h"
+#include "gram_layout.h"
 
 namespace enkfg {
 
@@ -62,28 +65,28 @@ __global__ void transpose_kernel(const float* __restrict__ in, float* __restrict
     }
 }
 
+// Gather: dst row r (stride `stride`) = src row perm[r] (stride `width`)
+// for r < nrow, zero padded to the stride, followed by one zero row.
+// Scatter: dst row perm[r] (stride `width`) = the first `width` elements of
+// src row r (stride `stride`).
+template <class T, bool SCATTER>
+__global__ void permute_rows_kernel(const T* __restrict__ src, const int* __restrict__ perm, T* __restrict__ dst,
+                                    long long nrow, int width, int stride)
+{
+    const long long n = (SCATTER ? nrow : nrow + 1) * stride;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        const long long r = k / stride;
+        const int c = (int)(k - r * stride);
+        if (SCATTER) {
+            if (c < width)
+                dst[(long long)perm[r] * width + c] = src[k];
+        } else {
+            dst[k] = (r < nrow && c < width) ? src[(long long)perm[r] * width + c] : T(0);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ Gram (MFMA)
-// Upper-triangular 16x16 tiles (I <= J) of an MT x MT tile grid, enumerated
-// row by row; wave w of the 4 waves owns tiles w, w + 4, w + 8, ...
-__host__ __device__ constexpr int tri_count(int MT) { return MT * (MT + 1) / 2; }
-__host__ __device__ constexpr int tri_I(int MT, int f)
-{
-    int I = 0;
-    while (f >= MT - I) {
-        f -= MT - I;
-        ++I;
-    }
-    return I;
-}
-__host__ __device__ constexpr int tri_J(int MT, int f)
-{
-    int I = 0;
-    while (f >= MT - I) {
-        f -= MT - I;
-        ++I;
-    }
-    return I + f;
-}
 
 // Chunks of KC local obs are staged in LDS as the raw FP32 rows of S (double
 // buffered: the gathers of chunk c+1, and the list entries of chunk c+2, are
@@ -91,40 +94,16 @@ __host__ __device__ constexpr int tri_J(int MT, int f)
 // a fragment is formed, (double) S * lc, as the reference forms Sloc.  Row
 // stride RS (floats) is congruent to 16 mod 32 and to 16 or 48 mod 64, so the
 // four rows a k-step reads fall in disjoint LDS banks.
-constexpr int KC = 64;
+constexpr int KC = kGramKC;
 template <int MT>
 struct GramLds {
     static constexpr int MP = 16 * MT;
     static constexpr int RS = MP + ((16 - MP % 32) + 32) % 32;
-    float rows[2][KC][RS];    // chunk c in rows[c % 2]
+    alignas(16) float rows[2][KC][RS];    // chunk c in rows[c % 2]
     double sl[2][KC];         // s_o * lc_o of the chunk's rows, chunk c in [c % 2]
     double lc[3][KC];         // taper coefficients, chunk c in [c % 3]
     int id[3][KC];            // obs ids (-1 past the end), chunk c in [c % 3]
 };
-
-// per-thread share of the coalesced chunk load: elements q = tid + 256 j of
-// the KC x MP chunk (row q / MP, column q % MP; columns >= m are skipped);
-// MP is a compile-time constant, so the index arithmetic is shifts and
-// multiplies.
-template <int MT>
-struct ChunkLoad {
-    static constexpr int NQ = (KC * 16 * MT + 255) / 256;
-    float v[NQ];
-    double s;                 // s of row tid (threads < KC)
-};
-
-// list entries (ids, tapers) of chunk k0.. (threads < KC): contiguous loads
-__device__ inline void chunk_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c)
-{
-    const int k = k0 + (int)threadIdx.x;
-    if (threadIdx.x < KC && k < cnt) {
-        id = a.ids[off + k];
-        c = a.lc[off + k];
-    } else {
-        id = -1;
-        c = 0.0;
-    }
-}
 
 template <int MT>
 __device__ inline void chunk_meta_store(GramLds<MT>& sh, int buf, int id, double c)
@@ -135,52 +114,22 @@ __device__ inline void chunk_meta_store(GramLds<MT>& sh, int buf, int id, double
     }
 }
 
-// gathers of the chunk whose ids are in meta buffer mbuf: rows of S and s
-template <int MT>
-__device__ inline void chunk_vals(const CalcArgs& a, const GramLds<MT>& sh, int mbuf, ChunkLoad<MT>& L)
-{
-    constexpr int MP = 16 * MT;
-    const int m = a.m;
-#pragma unroll
-    for (int j = 0; j < ChunkLoad<MT>::NQ; ++j) {
-        const int q = threadIdx.x + 256 * j;
-        const int row = q / MP, col = q - row * MP;
-        const int id = (row < KC && col < m) ? sh.id[mbuf][row] : -1;
-        L.v[j] = (id >= 0) ? a.S[(long long)id * m + col] : 0.0f;
-    }
-    L.s = 0.0;
-    if (threadIdx.x < KC) {
-        const int id = sh.id[mbuf][threadIdx.x];
-        if (id >= 0)
-            L.s = a.sv[id];
-    }
-}
-
 // rows of a chunk into row buffer rbuf; s * lc with the tapers of meta
 // buffer mbuf
 template <int MT>
-__device__ inline void chunk_store(GramLds<MT>& sh, int rbuf, int mbuf, int m, const ChunkLoad<MT>& L)
+__device__ inline void chunk_store(const CalcArgs& a, GramLds<MT>& sh, int rbuf, int mbuf, const GramRegs<MT>& L)
 {
-    constexpr int MP = 16 * MT;
+    const int qr = a.ms >> 2;
 #pragma unroll
-    for (int j = 0; j < ChunkLoad<MT>::NQ; ++j) {
-        const int q = threadIdx.x + 256 * j;
-        const int row = q / MP, col = q - row * MP;
-        if (row < KC && col < m)
-            sh.rows[rbuf][row][col] = L.v[j];
+    for (int j = 0; j < MT; ++j) {
+        const int q = (int)threadIdx.x + 256 * j;
+        const int row = q / qr;
+        const int qd = q - row * qr;
+        if (row < KC)
+            *reinterpret_cast<float4*>(&sh.rows[rbuf][row][4 * qd]) = L.v[j];
     }
     if (threadIdx.x < KC)
         sh.sl[rbuf][threadIdx.x] = L.s * sh.lc[mbuf][threadIdx.x];
-}
-
-// Position of element (i, j) of an m x m matrix (m <= 16 B) in the Gram
-// scratch: the factor kernel's thread (i / B, j / B) of its 16 x 16 thread
-// grid owns the B x B block starting at (B (i / B), B (j / B)), stored as one
-// contiguous run of B * B doubles per thread.
-template <int B>
-__host__ __device__ constexpr size_t gram_index(int i, int j)
-{
-    return ((size_t)((i / B) * 16 + (j / B)) * B + (i % B)) * B + (j % B);
 }
 
 // acc[T] += frag[I] frag[J]^T for the tiles (I, J) of wave W, with the tile
@@ -230,7 +179,6 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
     const int lane = tid & 63;
     const int kq = lane >> 4;   // which of the 4 obs of a k-step
     const int col = lane & 15;
-    const int m = a.m;
 
     dbl4 acc[NTW > 0 ? NTW : 1];
 #pragma unroll
@@ -244,31 +192,35 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
     // zero both row buffers once: padding columns stay zero
     for (int q = tid; q < 2 * KC * GramLds<MT>::RS; q += 256)
         (&sh.rows[0][0][0])[q] = 0.0f;
-    ChunkLoad<MT> L;
+    GramRegs<MT> L;
     const int nchunk = (cnt + KC - 1) / KC;
     int mid;
     double mc;
     // prologue: list entries of chunks 0 and 1, rows of chunk 0
-    chunk_meta_load(a, off, cnt, 0, mid, mc);
+    gram_meta_load(a, off, cnt, 0, mid, mc);
     chunk_meta_store<MT>(sh, 0, mid, mc);
-    chunk_meta_load(a, off, cnt, KC, mid, mc);
+    gram_meta_load(a, off, cnt, KC, mid, mc);
     chunk_meta_store<MT>(sh, 1, mid, mc);
+    chunk_meta_store<MT>(sh, 2, -1, 0.0);   // every id buffer holds valid rows or -1
     __syncthreads();
-    chunk_vals<MT>(a, sh, 0, L);
-    chunk_store<MT>(sh, 0, 0, m, L);
+    gram_gather<MT>(a, sh.id[0], L);
+    chunk_store<MT>(a, sh, 0, 0, L);
     __syncthreads();
     // iteration c: gathers of chunk c+1 and list entries of chunk c+2 are in
     // flight while chunk c feeds the MFMAs; one barrier per chunk
     for (int c = 0; c < nchunk; ++c) {
         const bool more = c + 1 < nchunk;
         const bool more2 = c + 2 < nchunk;
-        if (more)
-            chunk_vals<MT>(a, sh, (c + 1) % 3, L);
-        if (more2)
-            chunk_meta_load(a, off, cnt, (c + 2) * KC, mid, mc);
+        // the loads are unconditional: past the end they read the padding
+        // of the lists and the zero row
+        gram_gather<MT>(a, sh.id[(c + 1) % 3], L);
+        gram_meta_load(a, off, cnt, (c + 2) * KC, mid, mc);
+        // keep the loads ahead of the MFMAs (the scheduler would sink them
+        // to their use, exposing their latency every chunk)
+        __builtin_amdgcn_sched_barrier(0);
         gram_wave_chunk<MT, W>(sh, c & 1, c % 3, acc, sts);
         if (more)
-            chunk_store<MT>(sh, (c + 1) & 1, (c + 1) % 3, m, L);
+            chunk_store<MT>(a, sh, (c + 1) & 1, (c + 1) % 3, L);
         if (more2)
             chunk_meta_store<MT>(sh, (c + 2) % 3, mid, mc);
         __syncthreads();
@@ -847,6 +799,39 @@ hipError_t launch_transpose_f32(hipStream_t s, const float* in, float* out, long
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, s, in, out, rows, cols);
     return hipGetLastError();
+}
+
+template <class T, bool SCATTER>
+hipError_t launch_permute(hipStream_t s, const T* src, const int* perm, T* dst, long long nrow, int width, int stride)
+{
+    const long long n = (SCATTER ? nrow : nrow + 1) * stride;
+    if (n == 0)
+        return hipSuccess;
+    const long long nb = std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL((permute_rows_kernel<T, SCATTER>), dim3((unsigned)nb), dim3(256), 0, s, src, perm, dst, nrow,
+                       width, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(hipStream_t s, const float* src, const int* perm, float* dst, long long nrow, int width,
+                              int stride)
+{
+    return launch_permute<float, false>(s, src, perm, dst, nrow, width, stride);
+}
+hipError_t launch_gather_rows(hipStream_t s, const double* src, const int* perm, double* dst, long long nrow,
+                              int width, int stride)
+{
+    return launch_permute<double, false>(s, src, perm, dst, nrow, width, stride);
+}
+hipError_t launch_scatter_rows(hipStream_t s, const float* src, const int* perm, float* dst, long long nrow, int width,
+                               int stride)
+{
+    return launch_permute<float, true>(s, src, perm, dst, nrow, width, stride);
+}
+hipError_t launch_scatter_rows(hipStream_t s, const double* src, const int* perm, double* dst, long long nrow,
+                               int width, int stride)
+{
+    return launch_permute<double, true>(s, src, perm, dst, nrow, width, stride);
 }
 
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a)

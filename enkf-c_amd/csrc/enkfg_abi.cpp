@@ -80,7 +80,7 @@ struct enkfg_ctx {
     std::string err;
     hipStream_t stream = nullptr;
     hipEvent_t ev[10] = {};
-    double timings[4] = {0, 0, 0, 0};
+    double timings[5] = {0, 0, 0, 0, 0};
 
     // obstypes and obs
     std::vector<enkfg_obstype> types;
@@ -88,11 +88,16 @@ struct enkfg_ctx {
     std::vector<short> otype, ostatus;
     std::vector<float> olon, olat, oestd;
     bool obs_set = false;
-    DevBuf d_nodes, d_types, d_obsids, d_type, d_status, d_estd, d_sqrtrf;
+    DevBuf d_nodes, d_types, d_obsids, d_type, d_status, d_estd, d_sqrtrf, d_perm;
     std::vector<TypeInfo> tinfo;
+    // Device rows of S, s and status are ordered by tree preorder (type by
+    // type), so the obs a node's range search returns -- a few subtrees --
+    // are a few runs of nearby rows; perm[row] is the observation index.
+    // kd nodes and the local lists hold rows; the API speaks obs indices.
+    std::vector<int> perm;
 
     // S and s
-    DevBuf d_S, d_s, d_tmp;
+    DevBuf d_S, d_s, d_tmp, d_tmp2;
     bool S_set = false;
 
     // localisation
@@ -212,6 +217,17 @@ int check_ctx(enkfg_ctx* c)
     return c == nullptr ? ENKFG_ERR_ARG : ENKFG_OK;
 }
 
+// search-time status (observations.c:1906) by device row
+int upload_status_rows(enkfg_ctx* c)
+{
+    std::vector<short> st((size_t)std::max(c->nobs, 1));
+    for (int r = 0; r < c->nobs; ++r)
+        st[r] = c->ostatus[c->perm[r]];
+    HIPCHECK(c, c->d_status.reserve(st.size() * sizeof(short)));
+    HIPCHECK(c, hipMemcpy(c->d_status.p, st.data(), st.size() * sizeof(short), hipMemcpyHostToDevice));
+    return ENKFG_OK;
+}
+
 int upload_types(enkfg_ctx* c)
 {
     HIPCHECK(c, c->d_types.reserve(sizeof(TypeInfo) * std::max<size_t>(c->tinfo.size(), 1)));
@@ -282,7 +298,7 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
     HIPCHECK(c, c->d_offsets.reserve((ncnt + 1) * sizeof(long long)));
     const unsigned char* vis = nodes->visible ? c->d_vis.as<unsigned char>() : nullptr;
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHECK(c, launch_loc_count(c->stream, c->nnode, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
+    HIPCHECK(c, launch_loc_count(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
                                  c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(),
                                  c->d_counts.as<int>()));
     size_t tmpb = 0;
@@ -295,9 +311,10 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
     HIPCHECK(c, hipMemcpyAsync(&total, c->d_offsets.as<long long>() + ncnt, sizeof total, hipMemcpyDeviceToHost,
                                c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    HIPCHECK(c, c->d_ids.reserve(std::max<long long>(total, 1) * sizeof(int)));
-    HIPCHECK(c, c->d_lc.reserve(std::max<long long>(total, 1) * sizeof(double)));
-    HIPCHECK(c, launch_loc_fill(c->stream, c->nnode, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
+    // the Gram kernels read up to kListPad entries past the last list
+    HIPCHECK(c, c->d_ids.reserve((total + kListPad) * sizeof(int)));
+    HIPCHECK(c, c->d_lc.reserve((total + kListPad) * sizeof(double)));
+    HIPCHECK(c, launch_loc_fill(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
                                 c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(),
                                 c->d_obsids.as<int>(), c->d_offsets.as<long long>(), c->d_ids.as<int>(),
                                 c->d_lc.as<double>()));
@@ -323,8 +340,8 @@ int enkfg_create(enkfg_ctx** out, const enkfg_config* cfg)
         return ENKFG_ERR_ARG;
     if (cfg->scheme != ENKFG_SCHEME_DENKF && cfg->scheme != ENKFG_SCHEME_ETKF)
         return ENKFG_ERR_ARG;
-    if (cfg->precision != ENKFG_PREC_FP64)
-        return ENKFG_ERR_UNSUPPORTED;
+    if (cfg->precision != ENKFG_PREC_FP64 && cfg->precision != ENKFG_PREC_MIXED)
+        return ENKFG_ERR_ARG;
     if (cfg->mode != ENKFG_MODE_HYBRID && cfg->nmem_dynamic != cfg->nmem)
         return ENKFG_ERR_ARG;
     int ndev = 0;
@@ -361,7 +378,7 @@ void enkfg_destroy(enkfg_ctx* c)
     if (c->stream != nullptr)
         (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->d_nodes, &c->d_types, &c->d_obsids, &c->d_type, &c->d_status, &c->d_estd, &c->d_sqrtrf,
-                      &c->d_S, &c->d_s, &c->d_tmp, &c->d_qxyz, &c->d_glob, &c->d_vis, &c->d_counts, &c->d_offsets,
+                      &c->d_perm, &c->d_S, &c->d_s, &c->d_tmp, &c->d_tmp2, &c->d_qxyz, &c->d_glob, &c->d_vis, &c->d_counts, &c->d_offsets,
                       &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
                       &c->d_pdfs, &c->d_psrf, &c->d_gram, &c->d_sts, &c->d_err, &c->d_E, &c->d_infl, &c->d_nlev,
                       &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio};
@@ -466,6 +483,29 @@ int enkfg_set_obs(enkfg_ctx* c, int nobs, const short* type, const short* status
         ti.nids = (int)ids.size();
         ti.searchable = ids.empty() ? 0 : 1;
     }
+    // row order: each type's tree in preorder, then every obs not in a tree
+    {
+        std::vector<int> rowof((size_t)nobs, -1);
+        c->perm.clear();
+        c->perm.reserve(nobs);
+        for (const KdNode& nd : nodes)
+            if (rowof[nd.obsid] < 0) {
+                rowof[nd.obsid] = (int)c->perm.size();
+                c->perm.push_back(nd.obsid);
+            }
+        for (int i = 0; i < nobs; ++i)
+            if (rowof[i] < 0) {
+                rowof[i] = (int)c->perm.size();
+                c->perm.push_back(i);
+            }
+        for (KdNode& nd : nodes)
+            nd.obsid = rowof[nd.obsid];
+        for (int& id : obsids)
+            id = rowof[id];
+    }
+    HIPCHECK(c, c->d_perm.reserve(std::max(nobs, 1) * sizeof(int)));
+    if (nobs > 0)
+        HIPCHECK(c, hipMemcpy(c->d_perm.p, c->perm.data(), nobs * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(c, c->d_nodes.reserve(std::max<size_t>(nodes.size(), 1) * sizeof(KdNode)));
     if (!nodes.empty())
         HIPCHECK(c, hipMemcpy(c->d_nodes.p, nodes.data(), nodes.size() * sizeof(KdNode), hipMemcpyHostToDevice));
@@ -480,8 +520,12 @@ int enkfg_set_obs(enkfg_ctx* c, int nobs, const short* type, const short* status
     HIPCHECK(c, c->d_estd.reserve(std::max(nobs, 1) * sizeof(float)));
     if (nobs > 0) {
         HIPCHECK(c, hipMemcpy(c->d_type.p, type, nobs * sizeof(short), hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_status.p, status, nobs * sizeof(short), hipMemcpyHostToDevice));
         HIPCHECK(c, hipMemcpy(c->d_estd.p, estd, nobs * sizeof(float), hipMemcpyHostToDevice));
+    }
+    {
+        int r = upload_status_rows(c);
+        if (r != ENKFG_OK)
+            return r;
     }
     std::vector<double> srf(std::max(nt, 1));
     for (int t = 0; t < nt; ++t)
@@ -501,8 +545,9 @@ int enkfg_set_obs_status(enkfg_ctx* c, const short* status)
     if (!c->obs_set || status == nullptr)
         return fail(c, ENKFG_ERR_STATE, "observations not set");
     c->ostatus.assign(status, status + c->nobs);
-    if (c->nobs > 0)
-        HIPCHECK(c, hipMemcpy(c->d_status.p, status, c->nobs * sizeof(short), hipMemcpyHostToDevice));
+    int r = upload_status_rows(c);
+    if (r != ENKFG_OK)
+        return r;
     c->lists_valid = false;
     return ENKFG_OK;
 }
@@ -514,15 +559,18 @@ int enkfg_set_S(enkfg_ctx* c, const float* S, const double* s, int standardised)
     if (!c->obs_set)
         return fail(c, ENKFG_ERR_STATE, "observations not set");
     const int m = c->cfg.nmem;
+    const int ms = (m + 3) & ~3;
     const size_t n = (size_t)m * c->nobs;
+    const size_t nrow = (size_t)ms * (c->nobs + 1);
     if (c->nobs > 0 && (S == nullptr || s == nullptr))
         return fail(c, ENKFG_ERR_ARG, "S or s is NULL");
     for (int i = 0; i < c->nobs; ++i)
         if (!std::isfinite(s[i]))
             return fail(c, ENKFG_ERR_NONFINITE, "obs %d: non-finite innovation", i);
-    HIPCHECK(c, c->d_S.reserve(std::max<size_t>(n, 1) * sizeof(float)));
-    HIPCHECK(c, c->d_tmp.reserve(std::max<size_t>(n, 1) * sizeof(float)));
-    HIPCHECK(c, c->d_s.reserve(std::max(c->nobs, 1) * sizeof(double)));
+    HIPCHECK(c, c->d_S.reserve(std::max(n, nrow) * sizeof(float)));
+    HIPCHECK(c, c->d_tmp.reserve(std::max(n, nrow) * sizeof(float)));
+    HIPCHECK(c, c->d_s.reserve((c->nobs + 1) * sizeof(double)));
+    HIPCHECK(c, c->d_tmp2.reserve((c->nobs + 1) * sizeof(double)));
     if (c->nobs > 0) {
         HIPCHECK(c, hipMemcpyAsync(c->d_tmp.p, S, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
         HIPCHECK(c, hipMemcpyAsync(c->d_s.p, s, c->nobs * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -531,6 +579,15 @@ int enkfg_set_S(enkfg_ctx* c, const float* S, const double* s, int standardised)
         if (!standardised)
             HIPCHECK(c, launch_standardise(c->stream, m, c->nobs, c->d_type.as<short>(), c->d_estd.as<float>(),
                                            c->d_sqrtrf.as<double>(), c->d_S.as<float>(), c->d_s.as<double>()));
+        // into device row order, rows padded to ms floats, plus the zero row
+        HIPCHECK(c, launch_gather_rows(c->stream, c->d_S.as<float>(), c->d_perm.as<int>(), c->d_tmp.as<float>(),
+                                       c->nobs, m, ms));
+        std::swap(c->d_S.p, c->d_tmp.p);
+        std::swap(c->d_S.bytes, c->d_tmp.bytes);
+        HIPCHECK(c, launch_gather_rows(c->stream, c->d_s.as<double>(), c->d_perm.as<int>(), c->d_tmp2.as<double>(),
+                                       c->nobs, 1, 1));
+        HIPCHECK(c, hipMemcpyAsync(c->d_s.p, c->d_tmp2.p, (c->nobs + 1) * sizeof(double), hipMemcpyDeviceToDevice,
+                                   c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));
     }
     c->S_set = true;
@@ -547,12 +604,21 @@ int enkfg_get_S(enkfg_ctx* c, float* S, double* s)
     const size_t n = (size_t)m * c->nobs;
     if (c->nobs == 0)
         return ENKFG_OK;
+    const int ms = (m + 3) & ~3;
+    HIPCHECK(c, c->d_tmp.reserve(n * sizeof(float)));
+    HIPCHECK(c, c->d_tmp2.reserve(n * sizeof(float)));
     if (S != nullptr) {
-        HIPCHECK(c, launch_cells_to_members(c->stream, c->d_S.as<float>(), c->d_tmp.as<float>(), m, c->nobs));
-        HIPCHECK(c, hipMemcpyAsync(S, c->d_tmp.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, launch_scatter_rows(c->stream, c->d_S.as<float>(), c->d_perm.as<int>(), c->d_tmp.as<float>(),
+                                        c->nobs, m, ms));
+        HIPCHECK(c, launch_cells_to_members(c->stream, c->d_tmp.as<float>(), c->d_tmp2.as<float>(), m, c->nobs));
+        HIPCHECK(c, hipMemcpyAsync(S, c->d_tmp2.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     }
-    if (s != nullptr)
-        HIPCHECK(c, hipMemcpyAsync(s, c->d_s.p, c->nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (s != nullptr) {
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        HIPCHECK(c, launch_scatter_rows(c->stream, c->d_s.as<double>(), c->d_perm.as<int>(), c->d_tmp.as<double>(),
+                                        c->nobs, 1, 1));
+        HIPCHECK(c, hipMemcpyAsync(s, c->d_tmp.p, c->nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return ENKFG_OK;
 }
@@ -598,8 +664,11 @@ int enkfg_get_localobs(enkfg_ctx* c, int node, int cap, int* ids, double* lc)
     if (p > cap)
         return fail(c, ENKFG_ERR_ARG, "cap %d < %lld local obs", cap, p);
     if (p > 0) {
-        if (ids != nullptr)
+        if (ids != nullptr) {
             HIPCHECK(c, hipMemcpy(ids, c->d_ids.as<int>() + o[0], p * sizeof(int), hipMemcpyDeviceToHost));
+            for (long long k = 0; k < p; ++k)
+                ids[k] = c->perm[ids[k]];   // device row -> obs index
+        }
         if (lc != nullptr)
             HIPCHECK(c, hipMemcpy(lc, c->d_lc.as<double>() + o[0], p * sizeof(double), hipMemcpyDeviceToHost));
     }
@@ -660,6 +729,8 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     a.lc = c->d_lc.as<double>();
     a.S = c->d_S.as<float>();
     a.sv = c->d_s.as<double>();
+    a.ms = (m + 3) & ~3;
+    a.nobs = c->nobs;
     a.gram = c->d_gram.as<double>();
     a.sts = c->d_sts.as<double>();
     a.T = dotT ? c->d_T.as<float>() : nullptr;
@@ -676,7 +747,10 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
         a.node0 = n0;
         a.nbatch = std::min(nbatch, nn - n0);
         HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
-        HIPCHECK(c, launch_gram(c->stream, a));
+        if (c->cfg.precision == ENKFG_PREC_MIXED)
+            HIPCHECK(c, launch_gram_mixed(c->stream, a));
+        else
+            HIPCHECK(c, launch_gram(c->stream, a));
         HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
         HIPCHECK(c, launch_factor(c->stream, a));
         HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
@@ -686,6 +760,7 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     }
     c->timings[1] = tg;
     c->timings[2] = tf;
+    c->timings[4] = (double)((nn + nbatch - 1) / nbatch);
     int err = 0;
     HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
     if (err != 0)
@@ -771,7 +846,7 @@ int enkfg_get_timings(enkfg_ctx* c, double* ms, int n)
 {
     if (check_ctx(c) || ms == nullptr)
         return ENKFG_ERR_ARG;
-    for (int k = 0; k < n && k < 4; ++k)
+    for (int k = 0; k < n && k < 5; ++k)
         ms[k] = c->timings[k];
     return ENKFG_OK;
 }

@@ -17,7 +17,7 @@ struct alignas(16) KdNode {
     int left, right;  // child node indices within the type's tree, -1 = none
     int parent;       // -1 for the root
     int dir;          // split dimension
-    int obsid;        // global observation index (obsids[data])
+    int obsid;        // device row of the observation (rows are in tree preorder)
     int pad;
 };
 static_assert(sizeof(KdNode) == 48, "KdNode layout");
@@ -40,10 +40,10 @@ struct TypeInfo {
 // ---- launchers (defined in the .hip files); all return hipError_t
 
 // localisation (loc_kernels.hip)
-hipError_t launch_loc_count(hipStream_t s, int nnode, const double* qxyz, const unsigned char* isglobal,
+hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                             const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                             const short* status, int* counts);
-hipError_t launch_loc_fill(hipStream_t s, int nnode, const double* qxyz, const unsigned char* isglobal,
+hipError_t launch_loc_fill(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                            const short* status, const int* obsids, const long long* offsets, int* ids,
                            double* lc);
@@ -54,8 +54,21 @@ hipError_t launch_scan_counts(hipStream_t s, const int* counts, long long* offse
 hipError_t launch_standardise(hipStream_t s, int nmem, int nobs, const short* type, const float* estd,
                               const double* sqrt_rfactor, float* S_obsmajor, double* sv);
 hipError_t launch_transpose_f32(hipStream_t s, const float* in, float* out, long long rows, long long cols);
+// Row permutations between the caller's order (rows of `width` elements)
+// and device row order (rows of `stride` >= width elements, zero padded):
+// gather writes dst[r] = src[perm[r]] for r < nrow plus one zero row at
+// r = nrow; scatter writes dst[perm[r]] = src[r].
+hipError_t launch_gather_rows(hipStream_t s, const float* src, const int* perm, float* dst, long long nrow, int width,
+                              int stride);
+hipError_t launch_gather_rows(hipStream_t s, const double* src, const int* perm, double* dst, long long nrow,
+                              int width, int stride);
+hipError_t launch_scatter_rows(hipStream_t s, const float* src, const int* perm, float* dst, long long nrow, int width,
+                               int stride);
+hipError_t launch_scatter_rows(hipStream_t s, const double* src, const int* perm, double* dst, long long nrow,
+                               int width, int stride);
 
 // per-column transforms (calc_kernels.hip)
+constexpr int kListPad = 256;      // readable entries past the end of the local lists
 struct CalcArgs {
     int m, md, mode, scheme, ntypes, pertype;
     double alpha;
@@ -65,8 +78,11 @@ struct CalcArgs {
     const long long* offsets;      // [nnode*ntypes + 1]
     const int* ids;
     const double* lc;
-    const float* S;                // [nobs][m]
-    const double* sv;              // [nobs]
+    const float* S;                // [nobs + 1][ms], device row order; row nobs is zero
+    const double* sv;              // [nobs + 1], device row order; sv[nobs] = 0
+    int ms;                        // row stride of S: m rounded up to a multiple of 4
+    int nobs;                      // index of the zero row
+    // ids and lc are readable for kListPad entries past the last list
     double* gram;                  // [nbatch][nseg][mp][mp]
     double* sts;                   // [nbatch][nseg][mp]
     float* T;                      // [nnode][md][m]
@@ -80,6 +96,7 @@ struct CalcArgs {
     int* err;                      // device error flag
 };
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a);
+hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a);   // gram_mixed.hip
 hipError_t launch_factor(hipStream_t s, const CalcArgs& a);
 
 // update (update_kernels.hip)

@@ -1,0 +1,102 @@
+// gram_layout.h -- tile enumeration of the Gram kernels and the layout of the
+// Gram scratch that the factor kernel reads.
+#pragma once
+
+#include <stddef.h>
+
+#include "enkfg_internal.h"
+
+namespace enkfg {
+
+// Upper-triangular 16x16 tiles (I <= J) of an MT x MT tile grid, enumerated
+// row by row; wave w of the 4 waves of a Gram workgroup owns tiles w, w + 4,
+// w + 8, ...
+__host__ __device__ constexpr int tri_count(int MT) { return MT * (MT + 1) / 2; }
+__host__ __device__ constexpr int tri_I(int MT, int f)
+{
+    int I = 0;
+    while (f >= MT - I) {
+        f -= MT - I;
+        ++I;
+    }
+    return I;
+}
+__host__ __device__ constexpr int tri_J(int MT, int f)
+{
+    int I = 0;
+    while (f >= MT - I) {
+        f -= MT - I;
+        ++I;
+    }
+    return I + f;
+}
+
+// Position of element (i, j) of an m x m matrix (m <= 16 B) in the Gram
+// scratch: the factor kernel's thread (i / B, j / B) of its 16 x 16 thread
+// grid owns the B x B block starting at (B (i / B), B (j / B)), stored as one
+// contiguous run of B * B doubles per thread.
+template <int B>
+__host__ __device__ constexpr size_t gram_index(int i, int j)
+{
+    return ((size_t)((i / B) * 16 + (j / B)) * B + (i % B)) * B + (j % B);
+}
+
+// ---- chunk staging shared by the Gram kernels
+//
+// Local obs are streamed in chunks of kGramKC.  All loads are unconditional
+// -- list entries past a segment's end are read from the padding of the
+// list arrays (kListPad entries) and masked to id -1 / taper 0, and the S
+// row of id -1 is the zero row `nobs` -- so no load writes a register on
+// only some paths, and the loads of chunk c+1 stay in flight while chunk c
+// feeds the MFMAs (a conditionally loaded register forces the compiler to
+// wait for every outstanding load before it can be redefined).
+constexpr int kGramKC = 64;
+// a segment's last chunk c reads list entries up to (c + 3) KC - 1, at most
+// 3 KC - 1 past the segment's end
+static_assert(kListPad >= 3 * kGramKC, "list padding too small for the Gram chunk pipeline");
+
+// list entries k0 + (tid % KC) of a segment (every thread loads; threads
+// below KC store)
+__device__ inline void gram_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c)
+{
+    const int k = k0 + (int)(threadIdx.x & (kGramKC - 1));
+    const int i = a.ids[off + k];
+    const double l = a.lc[off + k];
+    const bool in = k < cnt;
+    id = in ? i : -1;
+    c = in ? l : 0.0;
+}
+
+// One chunk in registers: quad q = tid + 256 j of the KC x (ms / 4) chunk is
+// row q / (ms / 4), members 4 (q % (ms / 4)) ..; MT quads per thread cover
+// the chunk for any m <= 16 MT.
+template <int MT>
+struct GramRegs {
+    float4 v[MT];
+    double s;                 // s of row tid % KC
+};
+
+__device__ inline int gram_row_id(const int* ids_lds, int row)
+{
+    const int id = ids_lds[row < kGramKC ? row : kGramKC - 1];
+    return row < kGramKC ? id : -1;
+}
+
+template <int MT>
+__device__ inline void gram_gather(const CalcArgs& a, const int* ids_lds, GramRegs<MT>& L)
+{
+    const int qr = a.ms >> 2;
+#pragma unroll
+    for (int j = 0; j < MT; ++j) {
+        const int q = (int)threadIdx.x + 256 * j;
+        const int row = q / qr;
+        const int qd = q - row * qr;
+        const int id = gram_row_id(ids_lds, row);
+        const long long r = id >= 0 ? id : a.nobs;
+        L.v[j] = *reinterpret_cast<const float4*>(a.S + r * a.ms + 4 * qd);
+    }
+    const int id = ids_lds[threadIdx.x & (kGramKC - 1)];
+    L.s = a.sv[id >= 0 ? id : a.nobs];
+}
+
+}  // namespace enkfg

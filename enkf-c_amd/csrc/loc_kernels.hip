@@ -14,7 +14,9 @@
 // reference's operation order in IEEE double with no contraction (the
 // library is compiled with -ffp-contract=off), so the taper bits match too.
 //
-// Two passes: count (per node and type), exclusive scan, fill.
+// Two passes: count (per node and type), exclusive scan, fill.  The obs
+// indices in the trees and in the lists are device rows (rows of S in tree
+// preorder); the host maps them back to observation indices.
 #include "enkfg_internal.h"
 
 namespace enkfg {
@@ -105,20 +107,43 @@ __device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, con
     }
 }
 
-__global__ void __launch_bounds__(128) loc_count_kernel(int nnode, const double* __restrict__ qxyz,
+// Thread mapping: one workgroup of 64 threads per 8 x 8 tile of neighbouring
+// nodes and one obstype, so the 64 searches of a wave descend the same
+// subtrees and share their cache lines.  Workgroups are dispatched to the 8
+// XCDs round-robin; work item w = (blockIdx % 8) * per + blockIdx / 8 gives
+// each XCD (and its L2) a contiguous band of tiles.
+struct NodeGrid {
+    int ni, nj;      // stride-grid nodes per row, rows
+    int ntx, nty;    // 8 x 8 tiles per row, rows of tiles
+    int per;         // work items per XCD
+};
+
+__device__ inline bool tile_node(const NodeGrid& g, int ntypes, int& t, int& node)
+{
+    const int w = (int)(blockIdx.x % 8) * g.per + (int)(blockIdx.x / 8);
+    const int ntile = g.ntx * g.nty;
+    if (w >= ntile * ntypes)
+        return false;
+    t = w / ntile;
+    const int tile = w - t * ntile;
+    const int i = (tile % g.ntx) * 8 + (int)(threadIdx.x & 7);
+    const int j = (tile / g.ntx) * 8 + (int)(threadIdx.x >> 3);
+    if (i >= g.ni || j >= g.nj)
+        return false;
+    node = j * g.ni + i;
+    return true;
+}
+
+__global__ void __launch_bounds__(64) loc_count_kernel(NodeGrid g, const double* __restrict__ qxyz,
                                                         const unsigned char* __restrict__ isglobal,
                                                         const unsigned char* __restrict__ visible,
                                                         const KdNode* __restrict__ nodes,
                                                         const TypeInfo* __restrict__ types, int ntypes,
                                                         const short* __restrict__ status, int* __restrict__ counts)
 {
-    // one thread per (node, type); consecutive threads take neighbouring nodes
-    // of the same type so that a wave walks similar paths of one tree
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)nnode * ntypes)
+    int t, node;
+    if (!tile_node(g, ntypes, t, node))
         return;
-    const int t = (int)(gid / nnode);
-    const int node = (int)(gid % nnode);
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
     const TypeInfo& ti = types[t];
@@ -136,7 +161,7 @@ __global__ void __launch_bounds__(128) loc_count_kernel(int nnode, const double*
     counts[(size_t)node * ntypes + t] = n;
 }
 
-__global__ void __launch_bounds__(128) loc_fill_kernel(int nnode, const double* __restrict__ qxyz,
+__global__ void __launch_bounds__(64) loc_fill_kernel(NodeGrid g, const double* __restrict__ qxyz,
                                                        const unsigned char* __restrict__ isglobal,
                                                        const unsigned char* __restrict__ visible,
                                                        const KdNode* __restrict__ nodes,
@@ -146,11 +171,9 @@ __global__ void __launch_bounds__(128) loc_fill_kernel(int nnode, const double* 
                                                        const long long* __restrict__ offsets, int* __restrict__ ids,
                                                        double* __restrict__ lc)
 {
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)nnode * ntypes)
+    int t, node;
+    if (!tile_node(g, ntypes, t, node))
         return;
-    const int t = (int)(gid / nnode);
-    const int node = (int)(gid % nnode);
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
     const TypeInfo& ti = types[t];
@@ -256,29 +279,46 @@ __global__ void __launch_bounds__(kScanThreads) scan_final_kernel(const int* __r
 
 }  // namespace
 
-hipError_t launch_loc_count(hipStream_t s, int nnode, const double* qxyz, const unsigned char* isglobal,
+namespace {
+
+NodeGrid node_grid(int ni, int nj, int ntypes, unsigned& nblock)
+{
+    NodeGrid g;
+    g.ni = ni;
+    g.nj = nj;
+    g.ntx = (ni + 7) / 8;
+    g.nty = (nj + 7) / 8;
+    const long long nwork = (long long)g.ntx * g.nty * ntypes;
+    g.per = (int)((nwork + 7) / 8);
+    nblock = (unsigned)(8 * (long long)g.per);
+    return g;
+}
+
+}  // namespace
+
+hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                             const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                             const short* status, int* counts)
 {
-    const long long nt = (long long)nnode * ntypes;
-    if (nt <= 0)
+    if ((long long)ni * nj * ntypes <= 0)
         return hipSuccess;
-    dim3 grid((unsigned)((nt + 63) / 64));
-    hipLaunchKernelGGL(loc_count_kernel, grid, dim3(64), 0, s, nnode, qxyz, isglobal, visible, nodes, types,
-                       ntypes, status, counts);
+    unsigned nb;
+    const NodeGrid g = node_grid(ni, nj, ntypes, nb);
+    hipLaunchKernelGGL(loc_count_kernel, dim3(nb), dim3(64), 0, s, g, qxyz, isglobal, visible, nodes, types, ntypes,
+                       status, counts);
     return hipGetLastError();
 }
 
-hipError_t launch_loc_fill(hipStream_t s, int nnode, const double* qxyz, const unsigned char* isglobal,
+hipError_t launch_loc_fill(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                            const short* status, const int* obsids, const long long* offsets, int* ids, double* lc)
 {
-    const long long nt = (long long)nnode * ntypes;
-    if (nt <= 0)
+    if ((long long)ni * nj * ntypes <= 0)
         return hipSuccess;
-    dim3 grid((unsigned)((nt + 63) / 64));
-    hipLaunchKernelGGL(loc_fill_kernel, grid, dim3(64), 0, s, nnode, qxyz, isglobal, visible, nodes, types,
-                       ntypes, status, obsids, offsets, ids, lc);
+    unsigned nb;
+    const NodeGrid g = node_grid(ni, nj, ntypes, nb);
+    hipLaunchKernelGGL(loc_fill_kernel, dim3(nb), dim3(64), 0, s, g, qxyz, isglobal, visible, nodes, types, ntypes,
+                       status, obsids, offsets, ids, lc);
     return hipGetLastError();
 }
 

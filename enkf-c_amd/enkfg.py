@@ -268,6 +268,12 @@ class Engine:
         self._check(lib().enkfg_get_timings(self.h, t, 4))
         return list(t)
 
+    def launches(self):
+        """Gram (and factor) launches of the last calc_transforms."""
+        t = (C.c_double * 5)()
+        self._check(lib().enkfg_get_timings(self.h, t, 5))
+        return int(t[4])
+
     # ---- update
     def set_transforms(self, T, w, stride):
         w = np.ascontiguousarray(w, np.float32)

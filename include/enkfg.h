@@ -56,7 +56,8 @@ extern "C" {
 
 /* ---- arithmetic of the calc and update kernels */
 #define ENKFG_PREC_FP64 0        /* FP64 algebra on FP32 S, FP32/FP64 update */
-#define ENKFG_PREC_MIXED 1       /* reserved: BF16/FP16 MFMA products */
+#define ENKFG_PREC_MIXED 1       /* S^T S as BF16 hi/lo MFMA products with FP32
+                                    accumulation; factorisation in FP64 */
 
 /* ---- config flags */
 #define ENKFG_FLAG_PERTYPE_DIAG 1  /* per-obstype dfs/srf as the default
@@ -210,7 +211,8 @@ int enkfg_ens_attach(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const e
 int enkfg_ens_update(enkfg_ctx* ctx, int flags, float* dev_infl);
 
 /* hipEvent-measured duration (ms) of the last calc / update kernels, per
- * phase: [0] localisation, [1] Gram, [2] factorisation, [3] update. */
+ * phase: [0] localisation, [1] Gram, [2] factorisation, [3] update; [4] is
+ * the number of Gram (and factor) launches of the last calc. */
 int enkfg_get_timings(enkfg_ctx* ctx, double* ms, int n);
 
 #ifdef __cplusplus

@@ -89,7 +89,7 @@ def test_argument_errors_before_device_calls():
     h = C.c_void_p()
     assert L.enkfg_create(None, None) == enkfg.ERR_ARG
     bad = [dict(nmem=1), dict(nmem=200), dict(nmem_dynamic=0), dict(mode=7), dict(scheme=9),
-           dict(mode=enkfg.MODE_ENKF, nmem_dynamic=10)]
+           dict(mode=enkfg.MODE_ENKF, nmem_dynamic=10), dict(precision=5)]
     for b in bad:
         cfg = dict(mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF, precision=0, nmem=24, nmem_dynamic=24,
                    alpha=1.0, gamma=0.0, device=0, flags=0)
