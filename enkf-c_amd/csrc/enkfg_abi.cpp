@@ -990,7 +990,7 @@ int enkfg_update_fields(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enk
     if ((flags & ENKFG_UPD_HYBRID_CONDITION) && c->cfg.mode == ENKFG_MODE_HYBRID)
         HIPCHECK(c, launch_hybrid_condition(c->stream, a, c->cfg.gamma));
     HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
-    HIPCHECK(c, launch_update(c->stream, a));
+    HIPCHECK(c, launch_update_mfma(c->stream, a, c->cfg.precision == ENKFG_PREC_MIXED));
     HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
     HIPCHECK(c, hipEventSynchronize(c->ev[6]));
     c->timings[3] = elapsed(c->ev[5], c->ev[6]);
@@ -1074,7 +1074,7 @@ int enkfg_ens_update(enkfg_ctx* c, int flags, float* dev_infl)
     if ((flags & ENKFG_UPD_HYBRID_CONDITION) && c->cfg.mode == ENKFG_MODE_HYBRID)
         HIPCHECK(c, launch_hybrid_condition(c->stream, a, c->cfg.gamma));
     HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
-    HIPCHECK(c, launch_update(c->stream, a));
+    HIPCHECK(c, launch_update_mfma(c->stream, a, c->cfg.precision == ENKFG_PREC_MIXED));
     HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
     HIPCHECK(c, hipEventSynchronize(c->ev[6]));
     c->timings[3] = elapsed(c->ev[5], c->ev[6]);

@@ -119,7 +119,7 @@ struct UpdArgs {
     long long ncell;               // cells per field in E
     float* infl;                   // optional [nfield][ncell]
 };
-hipError_t launch_update(hipStream_t s, const UpdArgs& a);
+hipError_t launch_update_mfma(hipStream_t s, const UpdArgs& a, bool mixed);   // update_mfma.hip
 hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a);
 hipError_t launch_hybrid_condition(hipStream_t s, const UpdArgs& a, double gamma);
 // [nmem][rows][cols] <-> [rows*cols][nmem]

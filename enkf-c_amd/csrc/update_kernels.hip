@@ -1,16 +1,12 @@
-// update_kernels.hip -- das_updatefields (update/update.c:92-564),
-// das_updatebg (update.c:568-814) and das_sethybridensemble
-// (common/dasystem.c:775-819) on a device-resident ensemble.
+// update_kernels.hip -- das_updatebg (update/update.c:568-814),
+// das_sethybridensemble (common/dasystem.c:775-819) and the layout
+// transposes of the update, on a device-resident ensemble (das_updatefields
+// itself is in update_mfma.hip).
 //
 // Layout: E[field][cell][member], cell = j * mni + i (members contiguous, so
-// one cell of one field is a 4*m-byte contiguous record).  One block of
-// 64*ceil(m/64) threads handles one grid cell for all fields: thread e owns
-// member e.  The transform for the cell is interpolated from the stride grid
-// exactly as update.c:195-272 does (FP32, j first then i, no contraction),
-// once per cell, and kept in registers (thread e1 holds row e1 of T); it is
-// then applied to every field of the column.  The per-cell arithmetic follows
-// update.c:282-527: FP64 ensemble mean and increment sums, FP32 anomalies and
-// transform product, capped inflation, masks and missing values.
+// one cell of one field is a 4*m-byte contiguous record).  w is interpolated
+// from the stride grid exactly as update.c:195-272 does (FP32, j first then
+// i, no contraction).
 #include "enkfg_internal.h"
 
 namespace enkfg {
@@ -18,7 +14,6 @@ namespace enkfg {
 namespace {
 
 constexpr float kMaxObsVal = 999999.0f;   // MAXOBSVAL, common/global.h:114
-constexpr float kEpsF = 1.0e-6f;          // EPSF, update/update.c:41
 
 // cell index for block b: each of the 8 XCDs (blocks are dispatched to XCDs
 // round-robin) gets a contiguous range of cells, so neighbouring cells, which
@@ -113,256 +108,6 @@ __device__ inline bool masked(int nl, int level, int issurf, int surfk, int surf
     if (surfk == 0)
         return nl <= level;
     return nl <= (issurf ? surf_thresh : surfk - level);
-}
-
-// ---------------------------------------------------------------- das_updatefields
-// One block of 256 threads per grid cell (and per group of up to FG fields).
-// The cell's transform is interpolated once into LDS and applied to every
-// field of the group as one small GEMM, V^a (fields x m_d) = V^f (fields x m)
-// T^T, register-blocked FPT fields x MT members per thread.  The per-field
-// sums (means, A^f w, spread sums) are sequential FP64 loops over the members
-// in the reference's order, one thread per field, so they are bit-exact;
-// only the FP32 transform product is summed in a different order than sgemv.
-template <int MT>
-struct UpdCfg {
-    static constexpr int MP = 16 * MT;
-    static constexpr int FG = (MT <= 9) ? 112 : 64;   // fields per group
-    static constexpr int FPT = FG / 16;               // fields per thread
-};
-
-template <int MT>
-struct UpdShared {
-    static constexpr int MP = UpdCfg<MT>::MP;
-    static constexpr int FG = UpdCfg<MT>::FG;
-    float at[MP][FG + 1];     // at[e][f]: member e of field f (values, anomalies, then v_a)
-    float tt[MP][MP + 1];     // tt[e2][e1] = T_cell[e1][e2]
-    float wc[MP];             // w_cell
-    double v1f[FG], v1a[FG], v2f[FG];
-    float infl[FG];
-    int zero[FG];             // 0 update, 1 all members zero, 2 static members zero
-};
-
-template <int MT>
-__global__ void __launch_bounds__(256) update_cell_kernel(UpdArgs a)
-{
-    constexpr int MP = UpdCfg<MT>::MP;
-    constexpr int FG = UpdCfg<MT>::FG;
-    constexpr int FPT = UpdCfg<MT>::FPT;
-    __shared__ UpdShared<MT> sh;
-
-    const long long cell = xcd_cell(a.ncell);
-    if (cell >= a.ncell)
-        return;
-    const int tid = threadIdx.x;
-    const int jl = (int)(cell / a.mni);
-    const int i = (int)(cell % a.mni);
-    const int j = a.j1 + jl;
-    const int m = a.m, md = a.md;
-    const bool hybrid = (a.mode == ENKFG_MODE_HYBRID);
-    const double k_d = (md > 1) ? sqrt((double)(m - 1) / (double)(md - 1)) : 1.0;
-    const Interp ip = make_interp(j, i, a.stride, a.nj_s, a.ni_s, a.periodic_i);
-    const int nl = a.nlevels[(long long)j * a.mni + i];
-
-    // ---- transform and w of the cell (interpolated once per cell); the node
-    // values of UB elements per thread are loaded before any is used, so that
-    // the loads are in flight together
-    {
-        constexpr int UB = 12;
-        const long long rec = (long long)md * m;
-        const float* p00 = a.T + ((long long)ip.jj * a.ni_s + ip.ii) * rec;
-        const float* p10 = a.T + ((long long)ip.jj2 * a.ni_s + ip.ii) * rec;
-        const float* p01 = a.T + ((long long)ip.jj * a.ni_s + ip.ii2) * rec;
-        const float* p11 = a.T + ((long long)ip.jj2 * a.ni_s + ip.ii2) * rec;
-        const int n = md * m;
-        for (int q0 = tid; q0 < n; q0 += 256 * UB) {
-            float v00[UB], v10[UB], v01[UB], v11[UB];
-#pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                const int q = q0 + 256 * u;
-                const bool ok = q < n;
-                v00[u] = ok ? p00[q] : 0.0f;
-                v10[u] = (ok && !ip.jcopy) ? p10[q] : 0.0f;
-                v01[u] = (ok && !ip.icopy) ? p01[q] : 0.0f;
-                v11[u] = (ok && !ip.jcopy && !ip.icopy) ? p11[q] : 0.0f;
-            }
-#pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                const int q = q0 + 256 * u;
-                if (q < n) {
-                    // update.c:224-271: j first, then i, FP32, no contraction
-                    const float ta = ip.jcopy ? v00[u] : v00[u] * ip.wj1 + v10[u] * ip.wj2;
-                    const float tb = ip.jcopy ? v01[u] : v01[u] * ip.wj1 + v11[u] * ip.wj2;
-                    const int e1 = q / m, e2 = q - e1 * m;
-                    sh.tt[e2][e1] = ip.icopy ? ta : ta * ip.wi1 + tb * ip.wi2;
-                }
-            }
-        }
-        for (int q = tid; q < MP * (MP + 1); q += 256) {
-            const int e2 = q / (MP + 1), e1 = q - e2 * (MP + 1);
-            if (e1 >= md || e2 >= m)
-                sh.tt[e2][e1] = 0.0f;
-        }
-        if (tid < MP)
-            sh.wc[tid] = (tid < m) ? interp_at(a.w, m, a.ni_s, ip, tid) : 0.0f;
-    }
-
-    for (int f0 = 0; f0 < a.nfield; f0 += FG) {
-        const int nf = min(FG, a.nfield - f0);
-        __syncthreads();   // previous group's reads of sh are done
-        // ---- load the group's fields of this cell: at[e][f] (all loads of a
-        // thread issued before the stores)
-        {
-            constexpr int NL = (FG * MP + 255) / 256;
-            float v[NL];
-#pragma unroll
-            for (int u = 0; u < NL; ++u) {
-                const int q = tid + 256 * u;
-                const int f = q / MP, e = q - f * MP;
-                v[u] = 0.0f;
-                if (f < nf && e < m && !masked(nl, a.flevel[f0 + f], a.fsurf[f0 + f], a.surfk, 0))
-                    v[u] = a.E[((long long)(f0 + f) * a.ncell + cell) * m + e];
-            }
-#pragma unroll
-            for (int u = 0; u < NL; ++u) {
-                const int q = tid + 256 * u;
-                const int f = q / MP, e = q - f * MP;
-                if (f < FG)
-                    sh.at[e][f] = v[u];
-            }
-        }
-        __syncthreads();
-        // ---- per-field stats (update.c:298-410), one thread per field
-        if (tid < nf) {
-            const int f = tid;
-            int z = 0;
-            if (masked(nl, a.flevel[f0 + f], a.fsurf[f0 + f], a.surfk, 0)) {
-                z = 1;
-            } else {
-                int eb = m;
-                for (int e = 0; e < m; ++e) {
-                    const float v = sh.at[e][f];
-                    if (!isfinite(v) || fabsf(v) > kMaxObsVal) {
-                        eb = e;
-                        break;
-                    }
-                }
-                if (eb < md) {
-                    z = 1;
-                } else if (eb < m) {
-                    z = 2;
-                    for (int e = md; e < m; ++e)
-                        sh.at[e][f] = 0.0f;
-                }
-            }
-            sh.zero[f] = z;
-            if (z != 1) {
-                double v1_f = 0.0;
-                for (int e = 0; e < md; ++e)
-                    v1_f += sh.at[e][f];
-                v1_f /= (double)md;
-                double v1_a = 0.0, v2_f = 0.0;
-                for (int e = 0; e < m; ++e) {
-                    const float vf = (float)((double)sh.at[e][f] - v1_f);
-                    sh.at[e][f] = vf;
-                    v1_a += vf * sh.wc[e];
-                    if (e < md) {
-                        const float vh = hybrid ? (float)((double)vf / k_d) : vf;
-                        v2_f += vh * vh;
-                    }
-                }
-                sh.v1f[f] = v1_f;
-                sh.v1a[f] = v1_a + v1_f;
-                sh.v2f[f] = v2_f;
-            }
-        }
-        __syncthreads();
-        // ---- V^a = V^f T^T (sgemv('T', m, md, 1, T, m, v_f, ...), update.c:399)
-        const int tf = tid >> 4, te = tid & 15;
-        float acc[FPT][MT];
-#pragma unroll
-        for (int x = 0; x < FPT; ++x)
-#pragma unroll
-            for (int y = 0; y < MT; ++y)
-                acc[x][y] = 0.0f;
-        for (int k = 0; k < m; ++k) {
-            float av[FPT], tv[MT];
-#pragma unroll
-            for (int x = 0; x < FPT; ++x)
-                av[x] = sh.at[k][tf * FPT + x];
-#pragma unroll
-            for (int y = 0; y < MT; ++y)
-                tv[y] = sh.tt[k][te * MT + y];
-#pragma unroll
-            for (int x = 0; x < FPT; ++x)
-#pragma unroll
-                for (int y = 0; y < MT; ++y)
-                    acc[x][y] = fmaf(av[x], tv[y], acc[x][y]);
-        }
-        __syncthreads();   // all reads of the anomalies are done
-#pragma unroll
-        for (int x = 0; x < FPT; ++x)
-#pragma unroll
-            for (int y = 0; y < MT; ++y)
-                sh.at[te * MT + y][tf * FPT + x] = acc[x][y];
-        __syncthreads();
-        // ---- capped inflation (update.c:432-458), one thread per field
-        if (tid < nf) {
-            const int f = tid;
-            const float inflation0 = a.finfl[f0 + f];
-            const double inf_ratio = a.finfratio[f0 + f];
-            float inflation = inflation0;
-            if (sh.zero[f] != 1 && !isnan(inf_ratio)) {
-                double v2_a = 0.0;
-                for (int e = 0; e < md; ++e) {
-                    const float va = hybrid ? (float)((double)sh.at[e][f] / k_d) : sh.at[e][f];
-                    v2_a += va * va;
-                }
-                if (v2_a > 0.0) {
-                    inflation = (float)(sqrt(sh.v2f[f] / v2_a) * inf_ratio + 1.0 - inf_ratio);
-                    if (inflation >= inflation0)
-                        inflation = inflation0;
-                }
-            }
-            if (!(inflation - 1.0f > kEpsF))
-                inflation = 1.0f;
-            sh.infl[f] = inflation;
-        }
-        __syncthreads();
-        // ---- outputs (update.c:459-526)
-        for (int q = tid; q < nf * m; q += 256) {
-            const int f = q / m, e = q - f * m;
-            float* rec = a.E + ((long long)(f0 + f) * a.ncell + cell) * m;
-            const int z = sh.zero[f];
-            if (z == 1) {
-                rec[e] = 0.0f;
-                continue;
-            }
-            if (e >= md) {
-                if (z == 2)
-                    rec[e] = 0.0f;
-                continue;
-            }
-            float va = sh.at[e][f];
-            if (hybrid)
-                va = (float)((double)va / k_d);
-            va = va * sh.infl[f];   // inflation is 1 when not applied
-            va = (float)((double)va + sh.v1a[f]);
-            float out = va;
-            if (a.outputinc) {
-                float vf = (float)((double)rec[e] - sh.v1f[f]);
-                if (hybrid)
-                    vf = (float)((double)vf / k_d);
-                vf = (float)((double)vf + sh.v1f[f]);
-                if (!a.fapplylog[f0 + f])
-                    out = va - vf;
-                else
-                    out = isnormal(vf) ? (float)(exp10((double)va) - exp10((double)vf)) : 0.0f;
-            }
-            rec[e] = out;
-        }
-        if (a.infl != nullptr && tid < nf)
-            a.infl[(long long)(f0 + tid) * a.ncell + cell] = (sh.zero[tid] == 1) ? 0.0f : sh.infl[tid];
-    }
 }
 
 // das_updatebg: E is [field][cell][m + 1], member m the background
@@ -488,29 +233,6 @@ unsigned cell_blocks(long long ncell)
 }
 
 }  // namespace
-
-hipError_t launch_update(hipStream_t s, const UpdArgs& a)
-{
-    if (a.ncell == 0 || a.nfield == 0)
-        return hipSuccess;
-    const int MT = (a.m + 15) / 16;
-    const dim3 grid(cell_blocks(a.ncell));
-    const dim3 block(256);
-    switch (MT) {
-    case 1: hipLaunchKernelGGL(update_cell_kernel<1>, grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(update_cell_kernel<2>, grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(update_cell_kernel<3>, grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL(update_cell_kernel<4>, grid, block, 0, s, a); break;
-    case 5: hipLaunchKernelGGL(update_cell_kernel<5>, grid, block, 0, s, a); break;
-    case 6: hipLaunchKernelGGL(update_cell_kernel<6>, grid, block, 0, s, a); break;
-    case 7: hipLaunchKernelGGL(update_cell_kernel<7>, grid, block, 0, s, a); break;
-    case 8: hipLaunchKernelGGL(update_cell_kernel<8>, grid, block, 0, s, a); break;
-    case 9: hipLaunchKernelGGL(update_cell_kernel<9>, grid, block, 0, s, a); break;
-    case 10: hipLaunchKernelGGL(update_cell_kernel<10>, grid, block, 0, s, a); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
 
 hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a)
 {

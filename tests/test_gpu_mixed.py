@@ -78,3 +78,47 @@ def test_mixed_transforms_and_analysis_error(mode, scheme, m, md, profiles):
     print("  analysis: mean %.3g  spread %.3g" % (emean, espread), flush=True)
     assert emean <= 1e-5
     assert espread <= 1e-4
+
+
+@pytest.mark.parametrize("mode,m,md,stride", [(enkfg.MODE_ENKF, 96, 96, 3), (enkfg.MODE_HYBRID, 48, 32, 2),
+                                              (enkfg.MODE_ENKF, 24, 24, 1)])
+def test_mixed_update_error(mode, m, md, stride):
+    """The FP16 hi/lo MFMA apply against the oracle's FP32 update with the
+    same T and w: every output within 1e-5 of the field's magnitude, masks
+    and missing values exact."""
+    print("case mixed update mode=%d m=%d md=%d stride=%d" % (mode, m, md, stride), flush=True)
+    P = synth.make_problem(seed=200 + m, nmem=m, nobs=1500, locrad=70.0, nj=25, ni=31, stride=stride,
+                           land_frac=0.2, nmem_static=m - md)
+    ob = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd)
+    S, s = O.standardise(m, ob, P.HA, P.innov)
+    r = O.calc_transforms(mode, enkfg.SCHEME_DENKF, m, md, 1.0, ob, S, s, P.node_lon.ravel(), P.node_lat.ravel())
+    T = r["T"].reshape(P.nj_s, P.ni_s, md, m)
+    w = r["w"].reshape(P.nj_s, P.ni_s, m)
+    bufs, fl = [], []
+    for k in range(P.nk):
+        bufs.append(np.ascontiguousarray(P.ens["temp"][:m, k]))
+        fl.append(dict(level=k, inflation=1.1, inf_ratio=1.0))
+        bufs.append(np.ascontiguousarray(P.ens["salt"][:m, k]))
+        fl.append(dict(level=k, inflation=1.0, inf_ratio=float("nan")))
+    bufs.append(np.ascontiguousarray(P.ens["eta"][:m, 0]))
+    fl.append(dict(level=0, issurfacevar=1, inflation=1.05, inf_ratio=0.5))
+    bufs[0][min(3, md - 1), 5, 7] = np.nan
+    ref = [b.copy() for b in bufs]
+    gpu = [b.copy() for b in bufs]
+    O.update_fields(mode, m, md, stride, 0, 0, P.nlevels, T, w, fl, ref, 0, P.nj - 1)
+    E = enkfg.Engine(m, nmem_dynamic=md, mode=mode, precision=enkfg.PREC_MIXED)
+    E.set_transforms(T, w, stride)
+    E.update_fields(P.nlevels, stride, 0, 0, fl, gpu, 0, P.nj - 1)
+    E.close()
+    worst = 0.0
+    for a, b in zip(gpu, ref):
+        assert np.array_equal(a == 0, b == 0)
+        worst = max(worst, float(np.abs(a.astype(np.float64) - b).max() / max(np.abs(b).max(), 1.0)))
+    A = np.stack(gpu)[:, :md].astype(np.float64)
+    B = np.stack(ref)[:, :md].astype(np.float64)
+    emean = float(np.abs(A.mean(1) - B.mean(1)).max() / np.abs(B.mean(1)).max())
+    espread = float(np.abs(A.std(1) - B.std(1)).max() / np.abs(B.std(1)).max())
+    print("  max error %.3g  mean %.3g  spread %.3g" % (worst, emean, espread), flush=True)
+    assert worst <= 1e-5
+    assert emean <= 1e-5
+    assert espread <= 1e-4
