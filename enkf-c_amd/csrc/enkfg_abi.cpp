@@ -142,16 +142,22 @@ int fail(enkfg_ctx* c, int code, const char* fmt, ...)
                         #expr, hipGetErrorString(e_), __FILE__, __LINE__);                                 \
     } while (0)
 
+// a kd-tree node while the tree is built on the host
+struct BuildNode {
+    double x, y, z;
+    int left, right, parent, dir, obsid;
+};
+
 // obs_createkdtrees for one obstype: sequential insertion of the type's OK obs
 // in ascending index order (common/kdtree.c:194-274, observations.c:1695-1720).
-void build_tree(const std::vector<double>& xyz, int ndim, const std::vector<int>& ids, std::vector<KdNode>& out)
+void build_tree(const std::vector<double>& xyz, int ndim, const std::vector<int>& ids, std::vector<BuildNode>& out)
 {
     const size_t base = out.size();
     for (size_t k = 0; k < ids.size(); ++k) {
         const double* c = &xyz[3 * k];
         if (!std::isfinite(c[0]))
             continue;   // kd_insertnode skips non-finite points
-        KdNode n{};
+        BuildNode n{};
         n.x = c[0];
         n.y = c[1];
         n.z = (ndim == 3) ? c[2] : 0.0;
@@ -166,7 +172,7 @@ void build_tree(const std::vector<double>& xyz, int ndim, const std::vector<int>
         }
         int cur = 0;
         for (;;) {
-            KdNode& p = out[base + cur];
+            BuildNode& p = out[base + cur];
             const double pc = (p.dir == 0) ? p.x : (p.dir == 1 ? p.y : p.z);
             int& slot = (c[p.dir] < pc) ? p.left : p.right;
             if (slot < 0) {
@@ -191,7 +197,7 @@ void build_tree(const std::vector<double>& xyz, int ndim, const std::vector<int>
             const int v = stack.back();
             stack.pop_back();
             order.push_back(v);
-            const KdNode& nd = out[base + v];
+            const BuildNode& nd = out[base + v];
             if (nd.right >= 0)
                 stack.push_back(nd.right);
             if (nd.left >= 0)
@@ -200,9 +206,9 @@ void build_tree(const std::vector<double>& xyz, int ndim, const std::vector<int>
         std::vector<int> pos(n);
         for (int k = 0; k < n; ++k)
             pos[order[k]] = k;
-        std::vector<KdNode> tmp(n);
+        std::vector<BuildNode> tmp(n);
         for (int k = 0; k < n; ++k) {
-            KdNode nd = out[base + order[k]];
+            BuildNode nd = out[base + order[k]];
             nd.left = (nd.left >= 0) ? pos[nd.left] : -1;
             nd.right = (nd.right >= 0) ? pos[nd.right] : -1;
             nd.parent = (nd.parent >= 0) ? pos[nd.parent] : -1;
@@ -439,7 +445,7 @@ int enkfg_set_obs(enkfg_ctx* c, int nobs, const short* type, const short* status
     c->oestd.assign(estd, estd + nobs);
 
     // trees (obs_createkdtrees)
-    std::vector<KdNode> nodes;
+    std::vector<BuildNode> nodes;
     std::vector<int> obsids;
     c->tinfo.assign(nt, TypeInfo{});
     for (int t = 0; t < nt; ++t) {
@@ -488,7 +494,7 @@ int enkfg_set_obs(enkfg_ctx* c, int nobs, const short* type, const short* status
         std::vector<int> rowof((size_t)nobs, -1);
         c->perm.clear();
         c->perm.reserve(nobs);
-        for (const KdNode& nd : nodes)
+        for (const BuildNode& nd : nodes)
             if (rowof[nd.obsid] < 0) {
                 rowof[nd.obsid] = (int)c->perm.size();
                 c->perm.push_back(nd.obsid);
@@ -498,17 +504,30 @@ int enkfg_set_obs(enkfg_ctx* c, int nobs, const short* type, const short* status
                 rowof[i] = (int)c->perm.size();
                 c->perm.push_back(i);
             }
-        for (KdNode& nd : nodes)
-            nd.obsid = rowof[nd.obsid];
         for (int& id : obsids)
             id = rowof[id];
     }
+    // device nodes: the row of node k is k (rows follow the node order); in
+    // preorder a left child directly follows its parent
+    std::vector<KdNode> dnodes(nodes.size());
+    for (int t = 0; t < nt; ++t)
+        for (int v = 0; v < c->tinfo[t].nnode; ++v) {
+            const BuildNode& b = nodes[(size_t)c->tinfo[t].node0 + v];
+            if (b.left >= 0 && b.left != v + 1)
+                return fail(c, ENKFG_ERR_ARG, "internal: kd-tree of obstype %d is not in preorder", t);
+            KdNode& d = dnodes[(size_t)c->tinfo[t].node0 + v];
+            d.x = b.x;
+            d.y = b.y;
+            d.z = b.z;
+            d.right = b.right;
+            d.link = (unsigned)(b.parent + 1) | (b.left >= 0 ? 0x80000000u : 0u);
+        }
     HIPCHECK(c, c->d_perm.reserve(std::max(nobs, 1) * sizeof(int)));
     if (nobs > 0)
         HIPCHECK(c, hipMemcpy(c->d_perm.p, c->perm.data(), nobs * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHECK(c, c->d_nodes.reserve(std::max<size_t>(nodes.size(), 1) * sizeof(KdNode)));
-    if (!nodes.empty())
-        HIPCHECK(c, hipMemcpy(c->d_nodes.p, nodes.data(), nodes.size() * sizeof(KdNode), hipMemcpyHostToDevice));
+    HIPCHECK(c, c->d_nodes.reserve(std::max<size_t>(dnodes.size(), 1) * sizeof(KdNode)));
+    if (!dnodes.empty())
+        HIPCHECK(c, hipMemcpy(c->d_nodes.p, dnodes.data(), dnodes.size() * sizeof(KdNode), hipMemcpyHostToDevice));
     HIPCHECK(c, c->d_obsids.reserve(std::max<size_t>(obsids.size(), 1) * sizeof(int)));
     if (!obsids.empty())
         HIPCHECK(c, hipMemcpy(c->d_obsids.p, obsids.data(), obsids.size() * sizeof(int), hipMemcpyHostToDevice));

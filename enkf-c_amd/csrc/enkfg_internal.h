@@ -9,18 +9,19 @@
 
 namespace enkfg {
 
-// One kd-tree node, array-of-structures so that a visit touches one 64-byte
-// line.  The tree is the one obs_createkdtrees builds (sequential insertion,
-// common/kdtree.c:194-274); parent links allow a stackless depth-first walk.
+// One kd-tree node on the device, 32 bytes (4 per 128-byte line).  The tree
+// is the one obs_createkdtrees builds (sequential insertion,
+// common/kdtree.c:194-274), stored in preorder: the left child of node v is
+// v + 1, the split dimension of a node is its depth modulo ndim (as
+// kd_insertnode assigns it), and a node's global index in the node array is
+// its observation's device row.  Parent links allow a stackless depth-first
+// walk.
 struct alignas(16) KdNode {
     double x, y, z;   // coordinates (z unused for 2D trees)
-    int left, right;  // child node indices within the type's tree, -1 = none
-    int parent;       // -1 for the root
-    int dir;          // split dimension
-    int obsid;        // device row of the observation (rows are in tree preorder)
-    int pad;
+    int right;        // right child within the type's tree, -1 = none
+    unsigned link;    // bit 31: has a left child; bits 0-30: parent + 1 (0 at the root)
 };
-static_assert(sizeof(KdNode) == 48, "KdNode layout");
+static_assert(sizeof(KdNode) == 32, "KdNode layout");
 
 // Per-obstype search parameters (obstype fields of common/obstypes.h)
 struct TypeInfo {

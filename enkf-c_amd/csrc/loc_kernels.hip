@@ -56,22 +56,27 @@ __device__ inline double coord(const KdNode& n, int d)
     return d == 0 ? n.x : (d == 1 ? n.y : n.z);
 }
 
-// Depth-first range search over one tree; calls emit(obsid, distsq) for every
-// hit in the reference's visiting order.
+// Depth-first range search over one tree; calls emit(node, distsq) for every
+// hit in the reference's visiting order.  The split dimension is the depth
+// modulo ndim, tracked on the way down and up.
 template <class Emit>
 __device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, const double* q, double range, Emit&& emit)
 {
     const double r2 = range * range;
     int node = 0;
     int from = -1;
+    int dir = 0;
     bool down = true;
 
     while (node >= 0) {
         const KdNode nd = tree[node];
-        const double c = coord(nd, nd.dir);
-        const double dx = q[nd.dir] - c;
-        const int near = (dx <= 0.0) ? nd.left : nd.right;
-        const int far = (dx <= 0.0) ? nd.right : nd.left;
+        const double dx = q[dir] - coord(nd, dir);
+        const int left = (nd.link & 0x80000000u) ? node + 1 : -1;
+        const int near = (dx <= 0.0) ? left : nd.right;
+        const int far = (dx <= 0.0) ? nd.right : left;
+        const int parent = (int)(nd.link & 0x7fffffffu) - 1;
+        const int below = (dir + 1 == ndim) ? 0 : dir + 1;
+        const int above = (dir == 0) ? ndim - 1 : dir - 1;
         if (down) {
             double distsq = 0.0;
             for (int i = 0; i < ndim; i++) {
@@ -79,62 +84,53 @@ __device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, con
                 distsq += d * d;
             }
             if (distsq <= r2)
-                emit(nd.obsid, distsq);
+                emit(node, distsq);
             if (near >= 0) {
                 from = node;
                 node = near;
+                dir = below;
                 continue;
             }
             if (fabs(dx) < range && far >= 0) {
                 from = node;
                 node = far;
+                dir = below;
                 continue;
             }
             from = node;
-            node = nd.parent;
+            node = parent;
+            dir = above;
             down = false;
         } else {
             // arrived from child `from`
             if (from == near && fabs(dx) < range && far >= 0) {
                 from = node;
                 node = far;
+                dir = below;
                 down = true;
                 continue;
             }
             from = node;
-            node = nd.parent;
+            node = parent;
+            dir = above;
         }
     }
 }
 
-// Thread mapping: one workgroup of 64 threads per 8 x 8 tile of neighbouring
-// nodes and one obstype, so the 64 searches of a wave descend the same
-// subtrees and share their cache lines.  Workgroups are dispatched to the 8
-// XCDs round-robin; work item w = (blockIdx % 8) * per + blockIdx / 8 gives
-// each XCD (and its L2) a contiguous band of tiles.
-struct NodeGrid {
-    int ni, nj;      // stride-grid nodes per row, rows
-    int ntx, nty;    // 8 x 8 tiles per row, rows of tiles
-    int per;         // work items per XCD
-};
-
-__device__ inline bool tile_node(const NodeGrid& g, int ntypes, int& t, int& node)
+// Thread mapping: one thread per (node, obstype); consecutive threads take
+// consecutive nodes of a row of the stride grid and the same obstype, so a
+// wave walks neighbouring paths of one tree.
+__device__ inline bool thread_node(int nnode, int ntypes, int& t, int& node)
 {
-    const int w = (int)(blockIdx.x % 8) * g.per + (int)(blockIdx.x / 8);
-    const int ntile = g.ntx * g.nty;
-    if (w >= ntile * ntypes)
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)nnode * ntypes)
         return false;
-    t = w / ntile;
-    const int tile = w - t * ntile;
-    const int i = (tile % g.ntx) * 8 + (int)(threadIdx.x & 7);
-    const int j = (tile / g.ntx) * 8 + (int)(threadIdx.x >> 3);
-    if (i >= g.ni || j >= g.nj)
-        return false;
-    node = j * g.ni + i;
+    t = (int)(gid / nnode);
+    node = (int)(gid - (long long)t * nnode);
     return true;
 }
 
-__global__ void __launch_bounds__(64) loc_count_kernel(NodeGrid g, const double* __restrict__ qxyz,
+__global__ void __launch_bounds__(64) loc_count_kernel(int nnode, const double* __restrict__ qxyz,
                                                         const unsigned char* __restrict__ isglobal,
                                                         const unsigned char* __restrict__ visible,
                                                         const KdNode* __restrict__ nodes,
@@ -142,7 +138,7 @@ __global__ void __launch_bounds__(64) loc_count_kernel(NodeGrid g, const double*
                                                         const short* __restrict__ status, int* __restrict__ counts)
 {
     int t, node;
-    if (!tile_node(g, ntypes, t, node))
+    if (!thread_node(nnode, ntypes, t, node))
         return;
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
@@ -152,8 +148,8 @@ __global__ void __launch_bounds__(64) loc_count_kernel(NodeGrid g, const double*
         if (glob) {
             n = ti.nids;   // obs_getglobal: all obs of the type, no status check
         } else if (ti.nnode > 0) {
-            range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int obsid, double) {
-                if (status[obsid] == kStatusOK)
+            range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int v, double) {
+                if (status[ti.node0 + v] == kStatusOK)
                     n++;
             });
         }
@@ -161,7 +157,7 @@ __global__ void __launch_bounds__(64) loc_count_kernel(NodeGrid g, const double*
     counts[(size_t)node * ntypes + t] = n;
 }
 
-__global__ void __launch_bounds__(64) loc_fill_kernel(NodeGrid g, const double* __restrict__ qxyz,
+__global__ void __launch_bounds__(64) loc_fill_kernel(int nnode, const double* __restrict__ qxyz,
                                                        const unsigned char* __restrict__ isglobal,
                                                        const unsigned char* __restrict__ visible,
                                                        const KdNode* __restrict__ nodes,
@@ -172,7 +168,7 @@ __global__ void __launch_bounds__(64) loc_fill_kernel(NodeGrid g, const double* 
                                                        double* __restrict__ lc)
 {
     int t, node;
-    if (!tile_node(g, ntypes, t, node))
+    if (!thread_node(nnode, ntypes, t, node))
         return;
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
@@ -186,9 +182,10 @@ __global__ void __launch_bounds__(64) loc_fill_kernel(NodeGrid g, const double* 
             lc[o + i] = 1.0;
         }
     } else if (ti.nnode > 0) {
-        range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int obsid, double distsq) {
-            if (status[obsid] == kStatusOK) {
-                ids[o] = obsid;
+        range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int v, double distsq) {
+            const int row = ti.node0 + v;   // device row of the node's observation
+            if (status[row] == kStatusOK) {
+                ids[o] = row;
                 lc[o] = calclcoeff(ti, sqrt(distsq));
                 o++;
             }
@@ -279,33 +276,15 @@ __global__ void __launch_bounds__(kScanThreads) scan_final_kernel(const int* __r
 
 }  // namespace
 
-namespace {
-
-NodeGrid node_grid(int ni, int nj, int ntypes, unsigned& nblock)
-{
-    NodeGrid g;
-    g.ni = ni;
-    g.nj = nj;
-    g.ntx = (ni + 7) / 8;
-    g.nty = (nj + 7) / 8;
-    const long long nwork = (long long)g.ntx * g.nty * ntypes;
-    g.per = (int)((nwork + 7) / 8);
-    nblock = (unsigned)(8 * (long long)g.per);
-    return g;
-}
-
-}  // namespace
-
 hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                             const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                             const short* status, int* counts)
 {
-    if ((long long)ni * nj * ntypes <= 0)
+    const long long nt = (long long)ni * nj * ntypes;
+    if (nt <= 0)
         return hipSuccess;
-    unsigned nb;
-    const NodeGrid g = node_grid(ni, nj, ntypes, nb);
-    hipLaunchKernelGGL(loc_count_kernel, dim3(nb), dim3(64), 0, s, g, qxyz, isglobal, visible, nodes, types, ntypes,
-                       status, counts);
+    hipLaunchKernelGGL(loc_count_kernel, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, s, ni * nj, qxyz, isglobal,
+                       visible, nodes, types, ntypes, status, counts);
     return hipGetLastError();
 }
 
@@ -313,12 +292,11 @@ hipError_t launch_loc_fill(hipStream_t s, int ni, int nj, const double* qxyz, co
                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                            const short* status, const int* obsids, const long long* offsets, int* ids, double* lc)
 {
-    if ((long long)ni * nj * ntypes <= 0)
+    const long long nt = (long long)ni * nj * ntypes;
+    if (nt <= 0)
         return hipSuccess;
-    unsigned nb;
-    const NodeGrid g = node_grid(ni, nj, ntypes, nb);
-    hipLaunchKernelGGL(loc_fill_kernel, dim3(nb), dim3(64), 0, s, g, qxyz, isglobal, visible, nodes, types, ntypes,
-                       status, obsids, offsets, ids, lc);
+    hipLaunchKernelGGL(loc_fill_kernel, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, s, ni * nj, qxyz, isglobal,
+                       visible, nodes, types, ntypes, status, obsids, offsets, ids, lc);
     return hipGetLastError();
 }
 
