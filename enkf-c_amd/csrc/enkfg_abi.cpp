@@ -79,8 +79,10 @@ struct enkfg_ctx {
     enkfg_config cfg{};
     std::string err;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;   // factor kernels, overlapping the next batch's Gram
     hipEvent_t ev[10] = {};
-    double timings[5] = {0, 0, 0, 0, 0};
+    std::vector<hipEvent_t> bev;     // 4 per calc batch
+    double timings[6] = {0, 0, 0, 0, 0, 0};
 
     // obstypes and obs
     std::vector<enkfg_obstype> types;
@@ -363,6 +365,11 @@ int enkfg_create(enkfg_ctx** out, const enkfg_config* cfg)
         delete c;
         return ENKFG_ERR_HIP;
     }
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return ENKFG_ERR_HIP;
+    }
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) {
             delete c;
@@ -383,6 +390,8 @@ void enkfg_destroy(enkfg_ctx* c)
     (void)hipSetDevice(c->cfg.device);
     if (c->stream != nullptr)
         (void)hipStreamSynchronize(c->stream);
+    if (c->stream2 != nullptr)
+        (void)hipStreamSynchronize(c->stream2);
     DevBuf* bufs[] = {&c->d_nodes, &c->d_types, &c->d_obsids, &c->d_type, &c->d_status, &c->d_estd, &c->d_sqrtrf,
                       &c->d_perm, &c->d_S, &c->d_s, &c->d_tmp, &c->d_tmp2, &c->d_qxyz, &c->d_glob, &c->d_vis, &c->d_counts, &c->d_offsets,
                       &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
@@ -393,8 +402,12 @@ void enkfg_destroy(enkfg_ctx* c)
     for (auto& e : c->ev)
         if (e != nullptr)
             (void)hipEventDestroy(e);
+    for (auto& e : c->bev)
+        (void)hipEventDestroy(e);
     if (c->stream != nullptr)
         (void)hipStreamDestroy(c->stream);
+    if (c->stream2 != nullptr)
+        (void)hipStreamDestroy(c->stream2);
     delete c;
 }
 
@@ -728,8 +741,12 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     int nbatch = (int)std::max<size_t>(1, std::min<size_t>(budget / per_node, (size_t)nn));
     if (nbatch > 65536)
         nbatch = 65536;
-    HIPCHECK(c, c->d_gram.reserve((size_t)nbatch * per_node));
-    HIPCHECK(c, c->d_sts.reserve((size_t)nbatch * nseg * mp * sizeof(double)));
+    // two scratch buffers: the Gram kernel of batch b + 1 runs while the
+    // factor kernel of batch b reads the other one
+    const size_t gram_elems = (size_t)nbatch * per_node / sizeof(double);
+    const size_t sts_elems = (size_t)nbatch * nseg * mp;
+    HIPCHECK(c, c->d_gram.reserve(2 * gram_elems * sizeof(double)));
+    HIPCHECK(c, c->d_sts.reserve(2 * sts_elems * sizeof(double)));
     int zero = 0;
     HIPCHECK(c, hipMemcpy(c->d_err.p, &zero, sizeof zero, hipMemcpyHostToDevice));
 
@@ -761,25 +778,45 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     a.psrf = c->d_psrf.as<float>();
     a.nnode = nn;
     a.err = c->d_err.as<int>();
-    double tg = 0.0, tf = 0.0;
-    for (int n0 = 0; n0 < nn; n0 += nbatch) {
-        a.node0 = n0;
-        a.nbatch = std::min(nbatch, nn - n0);
-        HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+    // batch b: Gram on `stream` into buffer b % 2, factor on `stream2`; the
+    // Gram of batch b + 2 waits for the factor of batch b (same buffer)
+    const int nb = (nn + nbatch - 1) / nbatch;
+    while ((int)c->bev.size() < 4 * nb) {
+        hipEvent_t e;
+        HIPCHECK(c, hipEventCreate(&e));
+        c->bev.push_back(e);
+    }
+    for (int b = 0; b < nb; ++b) {
+        const int buf = b & 1;
+        hipEvent_t* e = &c->bev[4 * (size_t)b];
+        a.node0 = b * nbatch;
+        a.nbatch = std::min(nbatch, nn - a.node0);
+        a.gram = c->d_gram.as<double>() + buf * gram_elems;
+        a.sts = c->d_sts.as<double>() + buf * sts_elems;
+        if (b >= 2)
+            HIPCHECK(c, hipStreamWaitEvent(c->stream, c->bev[4 * (size_t)(b - 2) + 3], 0));
+        HIPCHECK(c, hipEventRecord(e[0], c->stream));
         if (c->cfg.precision == ENKFG_PREC_MIXED)
             HIPCHECK(c, launch_gram_mixed(c->stream, a));
         else
             HIPCHECK(c, launch_gram(c->stream, a));
-        HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
-        HIPCHECK(c, launch_factor(c->stream, a));
-        HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
-        HIPCHECK(c, hipEventSynchronize(c->ev[4]));
-        tg += elapsed(c->ev[2], c->ev[3]);
-        tf += elapsed(c->ev[3], c->ev[4]);
+        HIPCHECK(c, hipEventRecord(e[1], c->stream));
+        HIPCHECK(c, hipStreamWaitEvent(c->stream2, e[1], 0));
+        HIPCHECK(c, hipEventRecord(e[2], c->stream2));
+        HIPCHECK(c, launch_factor(c->stream2, a));
+        HIPCHECK(c, hipEventRecord(e[3], c->stream2));
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream2));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    double tg = 0.0, tf = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        tg += elapsed(c->bev[4 * (size_t)b], c->bev[4 * (size_t)b + 1]);
+        tf += elapsed(c->bev[4 * (size_t)b + 2], c->bev[4 * (size_t)b + 3]);
     }
     c->timings[1] = tg;
     c->timings[2] = tf;
-    c->timings[4] = (double)((nn + nbatch - 1) / nbatch);
+    c->timings[4] = (double)nb;
+    c->timings[5] = elapsed(c->ev[0], c->bev[4 * (size_t)(nb - 1) + 3]);
     int err = 0;
     HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
     if (err != 0)
@@ -865,7 +902,7 @@ int enkfg_get_timings(enkfg_ctx* c, double* ms, int n)
 {
     if (check_ctx(c) || ms == nullptr)
         return ENKFG_ERR_ARG;
-    for (int k = 0; k < n && k < 5; ++k)
+    for (int k = 0; k < n && k < 6; ++k)
         ms[k] = c->timings[k];
     return ENKFG_OK;
 }

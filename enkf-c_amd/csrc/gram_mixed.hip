@@ -16,7 +16,8 @@
 // c feeds the MFMAs).  A chunk is stored in LDS member-major, 64 obs of one
 // member contiguous as bf16, so that an operand of
 // v_mfma_f32_16x16x32_bf16 -- 8 consecutive obs of one member -- is one
-// 16-byte LDS read.
+// 16-byte LDS read; the gathers are arranged so that each lane holds 4
+// consecutive obs of one member and stores them as one 8-byte write.
 #include "enkfg_internal.h"
 #include "gram_layout.h"
 
@@ -49,30 +50,60 @@ __device__ inline void meta_store(MixLds<MT>& sh, int buf, int id, double c)
     }
 }
 
-__device__ inline void split_store(__bf16* hi, __bf16* lo, float v, double c)
-{
-    const float x = (float)((double)v * c);
-    const __bf16 h = (__bf16)x;
-    *hi = h;
-    *lo = (__bf16)(x - (float)h);
-}
+typedef __bf16 bf16x4 __attribute__((__vector_size__(4 * sizeof(__bf16))));
 
-// taper, split, transpose into the member-major bf16 rows of buffer rbuf
+// One chunk in registers, member-contiguous: item u = tid + 256 j holds
+// member u % MP of rows 4 (u / MP) .. 4 (u / MP) + 3, so a wave's loads of
+// one row are coalesced and its LDS stores are 4 consecutive obs (8 bytes)
+// of one member.  Loads are unconditional (zero row for ids -1, the last
+// member for padding members, which are not stored).
 template <int MT>
-__device__ inline void vals_store(const CalcArgs& a, MixLds<MT>& sh, int rbuf, int mbuf, const GramRegs<MT>& L)
+struct MixRegs {
+    float v[MT][4];
+    double s;                  // s of row tid % KC
+};
+
+template <int MT>
+__device__ inline void mix_gather(const CalcArgs& a, const int* ids_lds, MixRegs<MT>& L)
 {
-    const int qr = a.ms >> 2;
+    constexpr int MP = 16 * MT;
 #pragma unroll
     for (int j = 0; j < MT; ++j) {
-        const int q = (int)threadIdx.x + 256 * j;
-        const int row = q / qr;
-        const int e = 4 * (q - row * qr);
-        if (row < KC) {
-            const double c = sh.lc[mbuf][row];
-            split_store(&sh.hi[rbuf][e][row], &sh.lo[rbuf][e][row], L.v[j].x, c);
-            split_store(&sh.hi[rbuf][e + 1][row], &sh.lo[rbuf][e + 1][row], L.v[j].y, c);
-            split_store(&sh.hi[rbuf][e + 2][row], &sh.lo[rbuf][e + 2][row], L.v[j].z, c);
-            split_store(&sh.hi[rbuf][e + 3][row], &sh.lo[rbuf][e + 3][row], L.v[j].w, c);
+        const int u = (int)threadIdx.x + 256 * j;
+        const int rg = u / MP;               // < KC / 4 since 256 MT = 16 MP
+        const int e = min(u - rg * MP, a.m - 1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int id = ids_lds[4 * rg + k];
+            const long long r = id >= 0 ? id : a.nobs;
+            L.v[j][k] = a.S[r * a.ms + e];
+        }
+    }
+    const int id = ids_lds[threadIdx.x & (KC - 1)];
+    L.s = a.sv[id >= 0 ? id : a.nobs];
+}
+
+// taper, split, store 4 obs of one member into the member-major rows of
+// buffer rbuf
+template <int MT>
+__device__ inline void vals_store(const CalcArgs& a, MixLds<MT>& sh, int rbuf, int mbuf, const MixRegs<MT>& L)
+{
+    constexpr int MP = 16 * MT;
+#pragma unroll
+    for (int j = 0; j < MT; ++j) {
+        const int u = (int)threadIdx.x + 256 * j;
+        const int rg = u / MP;
+        const int e = u - rg * MP;
+        if (e < a.m) {
+            bf16x4 h, l;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float x = (float)((double)L.v[j][k] * sh.lc[mbuf][4 * rg + k]);
+                h[k] = (__bf16)x;
+                l[k] = (__bf16)(x - (float)h[k]);
+            }
+            *reinterpret_cast<bf16x4*>(&sh.hi[rbuf][e][4 * rg]) = h;
+            *reinterpret_cast<bf16x4*>(&sh.lo[rbuf][e][4 * rg]) = l;
         }
     }
     if (threadIdx.x < KC)
@@ -158,7 +189,7 @@ __device__ inline void mix_wave(const CalcArgs& a, long long off, int cnt, doubl
         for (int q = tid; q < nw; q += 256)
             z[q] = 0u;
     }
-    GramRegs<MT> L;
+    MixRegs<MT> L;
     const int nchunk = (cnt + KC - 1) / KC;
     int mid;
     double mc;
@@ -168,14 +199,14 @@ __device__ inline void mix_wave(const CalcArgs& a, long long off, int cnt, doubl
     meta_store<MT>(sh, 1, mid, mc);
     meta_store<MT>(sh, 2, -1, 0.0);   // every id buffer holds valid rows or -1
     __syncthreads();
-    gram_gather<MT>(a, sh.id[0], L);
+    mix_gather<MT>(a, sh.id[0], L);
     vals_store<MT>(a, sh, 0, 0, L);
     __syncthreads();
     for (int c = 0; c < nchunk; ++c) {
         const bool more = c + 1 < nchunk;
         const bool more2 = c + 2 < nchunk;
         // unconditional loads (list padding, zero row), see gram_layout.h
-        gram_gather<MT>(a, sh.id[(c + 1) % 3], L);
+        mix_gather<MT>(a, sh.id[(c + 1) % 3], L);
         gram_meta_load(a, off, cnt, (c + 2) * KC, mid, mc);
         __builtin_amdgcn_sched_barrier(0);   // loads stay ahead of the MFMAs
         mix_wave_chunk<MT, W>(sh, c & 1, acc, sts);

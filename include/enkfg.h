@@ -212,7 +212,10 @@ int enkfg_ens_update(enkfg_ctx* ctx, int flags, float* dev_infl);
 
 /* hipEvent-measured duration (ms) of the last calc / update kernels, per
  * phase: [0] localisation, [1] Gram, [2] factorisation, [3] update; [4] is
- * the number of Gram (and factor) launches of the last calc. */
+ * the number of Gram (and factor) launches of the last calc and [5] the
+ * device time of the whole calc (the factor kernels of one batch run
+ * concurrently with the Gram kernel of the next, so [1] + [2] may exceed
+ * the time they take together). */
 int enkfg_get_timings(enkfg_ctx* ctx, double* ms, int n);
 
 #ifdef __cplusplus
