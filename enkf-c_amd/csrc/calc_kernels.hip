@@ -868,6 +868,8 @@ void launch_factor_b(hipStream_t s, const CalcArgs& a, bool etkf)
 
 hipError_t launch_factor(hipStream_t s, const CalcArgs& a)
 {
+    if (factor_wave_supported(a))
+        return launch_factor_wave(s, a);   // factor_wave.hip: one wavefront per matrix
     const int B = (a.m + 15) / 16;
     const bool etkf = a.scheme == ENKFG_SCHEME_ETKF && a.mode != ENKFG_MODE_ENOI;
     if (a.nbatch == 0)

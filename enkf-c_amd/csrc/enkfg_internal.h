@@ -99,6 +99,9 @@ struct CalcArgs {
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a);
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a);   // gram_mixed.hip
 hipError_t launch_factor(hipStream_t s, const CalcArgs& a);
+// factor_wave.hip: DEnKF / EnOI with m <= 96, one wavefront per matrix
+bool factor_wave_supported(const CalcArgs& a);
+hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a);
 
 // update (update_kernels.hip)
 struct UpdArgs {

@@ -1,0 +1,317 @@
+// factor_wave.hip -- the per-column factorisation of the DEnKF and EnOI
+// analyses by unblocked Gauss-Jordan elimination.
+//
+// A work item is one symmetric positive definite M = I + G0 of a stride-grid
+// node: slot 0 the node's full Gram matrix (transforms, w, total
+// diagnostics), slot t + 1 the Gram matrix of obstype t alone (per-type
+// diagnostics, calc/transforms.c:843-879).  The 256 lanes of the workgroup
+// form a 16 x 16 grid; lane (lr, lc) holds the MT x MT block of rows lr MT..,
+// columns lc MT.. of the padded 16 MT x 16 MT matrix in registers (padding
+// kept as identity, which every step leaves intact): 36 doubles per lane for
+// m = 96, so that four matrices fit on a CU.
+//
+// M^-1 comes from unblocked Gauss-Jordan elimination with diagonal pivots
+// (stable for SPD matrices, whose pivots are positive Schur complements):
+// the owners of pivot k's row and column publish them in LDS (double
+// buffered, one barrier per pivot) and every lane applies the rank-1 update
+// to its block.  The serial chain of the elimination (m pivots) is hidden by
+// the other workgroups of the CU.  Then
+//
+//   DEnKF  T = (I + M^-1) / 2 (first m_d rows), relaxed to the prior and cast
+//          to FP32 (transforms.c:814-842);
+//   w      = M^-1 S^T s;
+//   dfs    = tr(M^-1 G0) = m - tr(M^-1)   (as M^-1 G0 = I - M^-1),
+//   srf    = sqrt(tr(G0) / (float) dfs) - 1, per type with the DFS_MIN guard.
+#include "enkfg_internal.h"
+#include "gram_layout.h"
+
+namespace enkfg {
+
+namespace {
+
+constexpr double kDfsMin = 1.0e-6;   // DFS_MIN, calc/transforms.c:32
+
+// sum over the 256 lanes: wave reductions, then the four partials in order
+// through `tmp` (which must not be in use)
+__device__ inline double group_sum(double v, double* tmp, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1)
+        v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((lane & 63) == 0)
+        tmp[lane >> 6] = v;
+    __syncthreads();
+    const double s = ((tmp[0] + tmp[1]) + tmp[2]) + tmp[3];
+    __syncthreads();
+    return s;
+}
+
+template <int MT>
+struct PivotLds {
+    double prow[2][16 * MT];
+    double pcol[2][16 * MT];
+};
+
+template <int MT>
+__global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
+{
+    constexpr int MP = 16 * MT;
+    constexpr int RR = MT;       // rows per lane (16 lane rows)
+    constexpr int RC = MT;       // columns per lane (16 lane columns)
+    __shared__ PivotLds<MT> sh;
+    const int lane = threadIdx.x;
+    const int lr = lane >> 4, lc = lane & 15;
+    const int nt = a.ntypes;
+    const int nslot = a.pertype ? nt + 1 : 1;
+    const int bnode = (int)(blockIdx.x / nslot);
+    const int slot = (int)(blockIdx.x - (unsigned)bnode * nslot);
+    if (bnode >= a.nbatch)
+        return;
+    const int node = a.node0 + bnode;
+    const int m = a.m, md = a.md;
+    const bool dotT = (a.mode == ENKFG_MODE_ENKF || a.mode == ENKFG_MODE_HYBRID);
+    const long long off = a.offsets[(size_t)node * nt];
+    const int p = (int)(a.offsets[(size_t)(node + 1) * nt] - off);
+
+    if (slot == 0 && !a.pertype) {
+        if (lane < nt) {
+            a.pnlobs[(size_t)lane * a.nnode + node] = a.counts[(size_t)node * nt + lane];
+            a.pdfs[(size_t)lane * a.nnode + node] = 0.0f;
+            a.psrf[(size_t)lane * a.nnode + node] = 0.0f;
+        }
+    }
+    int pslot = p;
+    if (slot > 0) {
+        pslot = a.counts[(size_t)node * nt + (slot - 1)];
+        if (lane == 0)
+            a.pnlobs[(size_t)(slot - 1) * a.nnode + node] = pslot;
+    }
+    if (pslot == 0) {
+        const int d = (slot == 0) ? nt : slot - 1;
+        if (lane == 0) {
+            a.pnlobs[(size_t)d * a.nnode + node] = 0;
+            a.pdfs[(size_t)d * a.nnode + node] = 0.0f;
+            a.psrf[(size_t)d * a.nnode + node] = 0.0f;
+        }
+        if (slot == 0) {
+            // empty column: T = I, w = 0 (transforms.c:766-780)
+            if (dotT)
+                for (int k = lane; k < md * m; k += 256) {
+                    const int e1 = k / m, e2 = k - e1 * m;
+                    a.T[(size_t)node * md * m + k] = (e1 == e2) ? 1.0f : 0.0f;
+                    if (a.T64 != nullptr)
+                        a.T64[(size_t)node * md * m + k] = (e1 == e2) ? 1.0 : 0.0;
+                }
+            for (int e = lane; e < m; e += 256) {
+                a.w[(size_t)node * m + e] = 0.0f;
+                if (a.w64 != nullptr)
+                    a.w64[(size_t)node * m + e] = 0.0;
+            }
+        }
+        return;
+    }
+
+    // ---- M = I + G0 (G0 summed over the slot's segments), tr(G0)
+    double x[RR][RC];
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+#pragma unroll
+        for (int c = 0; c < RC; ++c)
+            x[r][c] = (lr * RR + r == lc * RC + c) ? 1.0 : 0.0;
+    double trg = 0.0;
+    for (int s = 0; s < a.nseg; ++s) {
+        if (slot > 0 && s != slot - 1)
+            continue;
+        if (a.pertype && a.counts[(size_t)node * nt + s] == 0)
+            continue;
+        // this lane's rows and columns are block (lr, lc) of the gram_index
+        // layout: one run of MT^2 doubles
+        const double* g = a.gram + ((size_t)bnode * a.nseg + s) * MP * MP;
+        const double* g0 = g + gram_index<MT>(lr * RR, lc * RC);
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int i = lr * RR + r;
+#pragma unroll
+            for (int c = 0; c < RC; ++c) {
+                const int j = lc * RC + c;
+                const double v = g0[r * MT + c];
+                if (i < m && j < m) {
+                    x[r][c] += v;
+                    if (i == j)
+                        trg += v;
+                }
+            }
+        }
+    }
+    trg = group_sum(trg, sh.prow[0], lane);
+
+    // ---- Gauss-Jordan: x <- M^-1.  Pivot k = kb MT + rk lies in lane row
+    // and lane column kb, local row and column rk.
+    bool bad = false;
+    int buf = 0;
+    for (int kb = 0; kb < 16; ++kb) {
+#pragma unroll
+        for (int rk = 0; rk < RR; ++rk) {
+            const int k = kb * RR + rk;
+            if (k >= m)
+                break;   // the remaining pivots are identity padding
+            const int kc = k / RC;   // lane column owning column k
+            if (lr == kb) {
+#pragma unroll
+                for (int c = 0; c < RC; ++c)
+                    sh.prow[buf][lc * RC + c] = x[rk][c];
+            }
+            if (lc == kc) {
+#pragma unroll
+                for (int r = 0; r < RR; ++r)
+                    sh.pcol[buf][lr * RR + r] = x[r][rk % RC];
+            }
+            __syncthreads();
+            const double d = sh.prow[buf][k];
+            bad |= !(d > 0.0);
+            const double pinv = 1.0 / d;
+            double prow[RC], pcol[RR];
+#pragma unroll
+            for (int c = 0; c < RC; ++c)
+                prow[c] = sh.prow[buf][lc * RC + c] * pinv;
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+                pcol[r] = sh.pcol[buf][lr * RR + r];
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+#pragma unroll
+                for (int c = 0; c < RC; ++c)
+                    x[r][c] = fma(-pcol[r], prow[c], x[r][c]);
+            if (lr == kb) {
+#pragma unroll
+                for (int c = 0; c < RC; ++c)
+                    x[rk][c] = prow[c];                 // A_kc / d
+            }
+            if (lc == kc) {
+#pragma unroll
+                for (int r = 0; r < RR; ++r)
+                    x[r][rk % RC] = -pcol[r] * pinv;    // -A_rk / d
+            }
+            if (lr == kb && lc == kc)
+                x[rk][rk % RC] = pinv;
+            buf ^= 1;
+        }
+    }
+    if (bad)
+        atomicExch(a.err, ENKFG_ERR_LINALG);
+
+    // ---- dfs = m - tr(M^-1)
+    double trx = 0.0;
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            const int i = lr * RR + r;
+            if (i < m && i == lc * RC + c)
+                trx += x[r][c];
+        }
+    trx = group_sum(trx, sh.prow[0], lane);
+    const double dfs = (double)m - trx;
+
+    if (slot > 0) {
+        if (lane == 0) {
+            const float fd = (float)dfs;
+            a.pdfs[(size_t)(slot - 1) * a.nnode + node] = fd;
+            a.psrf[(size_t)(slot - 1) * a.nnode + node] =
+                ((double)fd > kDfsMin) ? (float)(sqrt(trg / (double)fd) - 1.0) : 0.0f;
+        }
+        return;
+    }
+    if (lane == 0) {
+        const float fd = (float)dfs;
+        a.pnlobs[(size_t)nt * a.nnode + node] = p;
+        a.pdfs[(size_t)nt * a.nnode + node] = fd;
+        a.psrf[(size_t)nt * a.nnode + node] = (float)(sqrt(trg / (double)fd) - 1.0);
+    }
+
+    // ---- w = M^-1 S^T s (row partials reduced over the 16 lanes of a row)
+    double v[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c)
+        v[c] = 0.0;
+    for (int s = 0; s < a.nseg; ++s) {
+        if (a.pertype && a.counts[(size_t)node * nt + s] == 0)
+            continue;
+        const double* st = a.sts + ((size_t)bnode * a.nseg + s) * MP;
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            const int j = lc * RC + c;
+            if (j < m)
+                v[c] += st[j];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        double y = 0.0;
+#pragma unroll
+        for (int c = 0; c < RC; ++c)
+            y = fma(x[r][c], v[c], y);
+        y += __shfl_xor(y, 1);
+        y += __shfl_xor(y, 2);
+        y += __shfl_xor(y, 4);
+        y += __shfl_xor(y, 8);
+        const int e = lr * RR + r;
+        if (lc == 0 && e < m) {
+            a.w[(size_t)node * m + e] = (float)y;
+            if (a.w64 != nullptr)
+                a.w64[(size_t)node * m + e] = y;
+        }
+    }
+
+    // ---- DEnKF T = (I + M^-1) / 2, relaxation T <- (T - I) alpha + I, FP32
+    if (dotT) {
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int e1 = lr * RR + r;
+            if (e1 >= md)
+                continue;
+#pragma unroll
+            for (int c = 0; c < RC; ++c) {
+                const int e2 = lc * RC + c;
+                if (e2 < m) {
+                    const double dlt = (e1 == e2) ? 1.0 : 0.0;
+                    double t = 0.5 * x[r][c] + 0.5 * dlt;
+                    t = (t - dlt) * a.alpha + dlt;
+                    const size_t o = ((size_t)node * md + e1) * m + e2;
+                    a.T[o] = (float)t;
+                    if (a.T64 != nullptr)
+                        a.T64[o] = t;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+bool factor_wave_supported(const CalcArgs& a)
+{
+    const bool etkf = a.scheme == ENKFG_SCHEME_ETKF && a.mode != ENKFG_MODE_ENOI;
+    return !etkf && a.m <= 96;
+}
+
+hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a)
+{
+    if (a.nbatch == 0)
+        return hipSuccess;
+    const int MT = (a.m + 15) / 16;
+    const int nslot = a.pertype ? a.ntypes + 1 : 1;
+    const dim3 grid((unsigned)((long long)a.nbatch * nslot)), block(256);
+    switch (MT) {
+    case 1: hipLaunchKernelGGL(factor_wave_kernel<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(factor_wave_kernel<2>, grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(factor_wave_kernel<3>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(factor_wave_kernel<4>, grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL(factor_wave_kernel<5>, grid, block, 0, s, a); break;
+    case 6: hipLaunchKernelGGL(factor_wave_kernel<6>, grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace enkfg
