@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
     const int b = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (b >= nwork)
         return;
-    const int node = a.node0 + b / a.nseg;
+    const int node = a.node_list ? a.node_list[a.node0 + b / a.nseg] : a.node0 + b / a.nseg;
     const int seg = b % a.nseg;
     long long off;
     int cnt;

@@ -115,6 +115,8 @@ struct enkfg_ctx {
 
     // update
     DevBuf d_E, d_infl, d_nlev, d_flevel, d_fsurf, d_flog, d_finfl, d_fratio;
+    // adjoint
+    DevBuf d_adjg, d_adjflag, d_adjlist, d_adjz, d_adjw, d_gy, d_gR;
     float* attached_E = nullptr;
     enkfg_grid att_grid{};
     int att_nfield = 0;
@@ -396,7 +398,8 @@ void enkfg_destroy(enkfg_ctx* c)
                       &c->d_perm, &c->d_S, &c->d_s, &c->d_tmp, &c->d_tmp2, &c->d_qxyz, &c->d_glob, &c->d_vis, &c->d_counts, &c->d_offsets,
                       &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
                       &c->d_pdfs, &c->d_psrf, &c->d_gram, &c->d_sts, &c->d_err, &c->d_E, &c->d_infl, &c->d_nlev,
-                      &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio};
+                      &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio, &c->d_adjg, &c->d_adjflag,
+                      &c->d_adjlist, &c->d_adjz, &c->d_adjw, &c->d_gy, &c->d_gR};
     for (DevBuf* b : bufs)
         b->release();
     for (auto& e : c->ev)
@@ -1134,6 +1137,117 @@ int enkfg_ens_update(enkfg_ctx* c, int flags, float* dev_infl)
     HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
     HIPCHECK(c, hipEventSynchronize(c->ev[6]));
     c->timings[3] = elapsed(c->ev[5], c->ev[6]);
+    return ENKFG_OK;
+}
+
+int enkfg_adjoint(enkfg_ctx* c, const float* dev_dJ_dmean, double* dJ_dy, double* dJ_dR, double* dJ_dinfl)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (dev_dJ_dmean == nullptr)
+        return fail(c, ENKFG_ERR_ARG, "dJ/dmean is NULL");
+    if (!c->S_set || !c->lists_valid || !c->transforms_valid)
+        return fail(c, ENKFG_ERR_STATE, "the adjoint needs S, and enkfg_calc_transforms on the same grid");
+    if (c->attached_E == nullptr)
+        return fail(c, ENKFG_ERR_STATE, "the adjoint needs the forecast ensemble attached (enkfg_ens_attach)");
+    const int m = c->cfg.nmem;
+    if (m > 96)
+        return fail(c, ENKFG_ERR_UNSUPPORTED, "the adjoint supports nmem <= 96 (got %d)", m);
+    const enkfg_grid& g = c->att_grid;
+    if (g.stride != c->stride || (g.mni - 1) / g.stride + 1 != c->ni_s || (g.mnj - 1) / g.stride + 1 != c->nj_s)
+        return fail(c, ENKFG_ERR_ARG, "the attached grid does not match the node grid of the last calc");
+    const int nn = c->nnode;
+    const int nt = (int)c->types.size();
+    // g_n = sum over cells of beta(c, n) sum_f dJ/dmean A^f
+    HIPCHECK(c, c->d_adjg.reserve((size_t)nn * m * sizeof(double)));
+    HIPCHECK(c, hipMemsetAsync(c->d_adjg.p, 0, (size_t)nn * m * sizeof(double), c->stream));
+    UpdArgs ua = upd_args(c, &g, c->att_nfield, 0, g.mnj, 0);
+    ua.E = c->attached_E;
+    HIPCHECK(c, launch_adj_cells(c->stream, ua, dev_dJ_dmean, c->d_adjg.as<double>()));
+    // nodes with a non-zero g
+    HIPCHECK(c, c->d_adjflag.reserve((size_t)nn * sizeof(int)));
+    HIPCHECK(c, launch_nonzero_rows(c->stream, c->d_adjg.as<double>(), nn, m, c->d_adjflag.as<int>()));
+    std::vector<int> flag(nn);
+    HIPCHECK(c, hipMemcpyAsync(flag.data(), c->d_adjflag.p, (size_t)nn * sizeof(int), hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<int> list;
+    for (int k = 0; k < nn; ++k)
+        if (flag[k])
+            list.push_back(k);
+    const int nlist = (int)list.size();
+    HIPCHECK(c, c->d_gy.reserve((size_t)std::max(c->nobs, 1) * sizeof(double)));
+    HIPCHECK(c, c->d_gR.reserve((size_t)std::max(c->nobs, 1) * sizeof(double)));
+    HIPCHECK(c, hipMemsetAsync(c->d_gy.p, 0, (size_t)std::max(c->nobs, 1) * sizeof(double), c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_gR.p, 0, (size_t)std::max(c->nobs, 1) * sizeof(double), c->stream));
+    if (nlist > 0) {
+        HIPCHECK(c, c->d_adjlist.reserve((size_t)nlist * sizeof(int)));
+        HIPCHECK(c, hipMemcpy(c->d_adjlist.p, list.data(), (size_t)nlist * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHECK(c, c->d_adjz.reserve((size_t)nlist * m * sizeof(double)));
+        HIPCHECK(c, c->d_adjw.reserve((size_t)nlist * m * sizeof(double)));
+        const int mp = 16 * ((m + 15) / 16);
+        const size_t per_node = (size_t)mp * mp * sizeof(double);
+        int nbatch = (int)std::max<size_t>(1, std::min<size_t>(((size_t)2 << 30) / per_node, (size_t)nlist));
+        nbatch = std::min(nbatch, 65536);
+        HIPCHECK(c, c->d_gram.reserve((size_t)nbatch * per_node));
+        HIPCHECK(c, c->d_sts.reserve((size_t)nbatch * mp * sizeof(double)));
+        int zero = 0;
+        HIPCHECK(c, hipMemcpy(c->d_err.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+        CalcArgs a{};
+        a.m = m;
+        a.md = c->cfg.nmem_dynamic;
+        a.mode = c->cfg.mode;
+        a.scheme = c->cfg.scheme;
+        a.ntypes = nt;
+        a.pertype = 0;
+        a.nseg = 1;
+        a.alpha = c->cfg.alpha;
+        a.counts = c->d_counts.as<int>();
+        a.offsets = c->d_offsets.as<long long>();
+        a.ids = c->d_ids.as<int>();
+        a.lc = c->d_lc.as<double>();
+        a.S = c->d_S.as<float>();
+        a.sv = c->d_s.as<double>();
+        a.ms = (m + 3) & ~3;
+        a.nobs = c->nobs;
+        a.gram = c->d_gram.as<double>();
+        a.sts = c->d_sts.as<double>();
+        a.nnode = nn;
+        a.err = c->d_err.as<int>();
+        a.node_list = c->d_adjlist.as<int>();
+        a.adj_g = c->d_adjg.as<double>();
+        a.adj_z = c->d_adjz.as<double>();
+        a.adj_w = c->d_adjw.as<double>();
+        // M recomputed in FP64 from the local lists, z = M^-1 g, w = M^-1 S^T s
+        for (int k0 = 0; k0 < nlist; k0 += nbatch) {
+            a.node0 = k0;
+            a.nbatch = std::min(nbatch, nlist - k0);
+            HIPCHECK(c, launch_gram(c->stream, a));
+            HIPCHECK(c, launch_factor_wave(c->stream, a));
+        }
+        HIPCHECK(c, launch_adj_obs(c->stream, a, nlist, c->d_perm.as<int>(), c->d_type.as<short>(),
+                                   c->d_estd.as<float>(), c->d_sqrtrf.as<double>(), c->d_gy.as<double>(),
+                                   c->d_gR.as<double>()));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        int err = 0;
+        HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
+        if (err != 0)
+            return fail(c, ENKFG_ERR_LINALG, "non-positive pivot in the factorisation of I + S^T S");
+    }
+    std::vector<double> gy((size_t)c->nobs), gR((size_t)c->nobs);
+    if (c->nobs > 0) {
+        HIPCHECK(c, hipMemcpy(gy.data(), c->d_gy.p, (size_t)c->nobs * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHECK(c, hipMemcpy(gR.data(), c->d_gR.p, (size_t)c->nobs * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int r = 0; r < c->nobs; ++r) {
+        const int o = c->perm[r];
+        if (dJ_dy != nullptr)
+            dJ_dy[o] = gy[r];
+        if (dJ_dR != nullptr)
+            dJ_dR[o] = gR[r];
+    }
+    if (dJ_dinfl != nullptr)
+        *dJ_dinfl = 0.0;   // the analysis mean does not depend on the inflation
     return ENKFG_OK;
 }
 

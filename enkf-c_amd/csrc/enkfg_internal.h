@@ -95,6 +95,13 @@ struct CalcArgs {
     float* psrf;
     int nnode;
     int* err;                      // device error flag
+    // adjoint of the analysis mean: nodes taken from node_list[node0 ..]
+    // (when set), and the factor kernel returns z = M^-1 g (adj_g: [node][m])
+    // and w = M^-1 S^T s per listed node (adj_z, adj_w: [list index][m])
+    const int* node_list;
+    const double* adj_g;
+    double* adj_z;
+    double* adj_w;
 };
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a);
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a);   // gram_mixed.hip
@@ -124,6 +131,15 @@ struct UpdArgs {
     float* infl;                   // optional [nfield][ncell]
 };
 hipError_t launch_update_mfma(hipStream_t s, const UpdArgs& a, bool mixed);   // update_mfma.hip
+
+// adjoint of the analysis mean (adjoint.hip)
+// g_nodes[node][m] += sum over cells of beta(cell, node) sum_f dJ/dmean[f][cell] A^f[f][cell][:]
+hipError_t launch_adj_cells(hipStream_t s, const UpdArgs& a, const float* dJ_dmean, double* g_nodes);
+// flags[r] = 1 if row r of the nrow x width matrix has a non-zero entry
+hipError_t launch_nonzero_rows(hipStream_t s, const double* g, int nrow, int width, int* flags);
+// per listed node k (nodes[k]): dJ/dy and dJ/dR of its local obs, by device row
+hipError_t launch_adj_obs(hipStream_t s, const CalcArgs& a, int nlist, const int* perm, const short* type,
+                          const float* estd, const double* sqrt_rfactor, double* dJ_dy, double* dJ_dR);
 hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a);
 hipError_t launch_hybrid_condition(hipStream_t s, const UpdArgs& a, double gamma);
 // [nmem][rows][cols] <-> [rows*cols][nmem]

@@ -67,13 +67,21 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
     const int slot = (int)(blockIdx.x - (unsigned)bnode * nslot);
     if (bnode >= a.nbatch)
         return;
-    const int node = a.node0 + bnode;
+    const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
+    const bool adjoint = a.adj_g != nullptr;   // z = M^-1 g and w only
     const int m = a.m, md = a.md;
     const bool dotT = (a.mode == ENKFG_MODE_ENKF || a.mode == ENKFG_MODE_HYBRID);
     const long long off = a.offsets[(size_t)node * nt];
     const int p = (int)(a.offsets[(size_t)(node + 1) * nt] - off);
 
-    if (slot == 0 && !a.pertype) {
+    if (adjoint && p == 0) {
+        for (int e = lane; e < m; e += 256) {
+            a.adj_z[(size_t)(a.node0 + bnode) * m + e] = 0.0;
+            a.adj_w[(size_t)(a.node0 + bnode) * m + e] = 0.0;
+        }
+        return;
+    }
+    if (slot == 0 && !a.pertype && !adjoint) {
         if (lane < nt) {
             a.pnlobs[(size_t)lane * a.nnode + node] = a.counts[(size_t)node * nt + lane];
             a.pdfs[(size_t)lane * a.nnode + node] = 0.0f;
@@ -199,6 +207,37 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
     }
     if (bad)
         atomicExch(a.err, ENKFG_ERR_LINALG);
+
+    if (adjoint) {
+        // z = M^-1 g and w = M^-1 S^T s for the listed node
+        const size_t k = (size_t)(a.node0 + bnode);
+        double gv[RC], sv[RC];
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            const int j = lc * RC + c;
+            gv[c] = (j < m) ? a.adj_g[(size_t)node * m + j] : 0.0;
+            sv[c] = (j < m) ? a.sts[(size_t)bnode * MP + j] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            double yz = 0.0, yw = 0.0;
+#pragma unroll
+            for (int c = 0; c < RC; ++c) {
+                yz = fma(x[r][c], gv[c], yz);
+                yw = fma(x[r][c], sv[c], yw);
+            }
+            for (int o = 1; o < 16; o <<= 1) {
+                yz += __shfl_xor(yz, o);
+                yw += __shfl_xor(yw, o);
+            }
+            const int e = lr * RR + r;
+            if (lc == 0 && e < m) {
+                a.adj_z[k * m + e] = yz;
+                a.adj_w[k * m + e] = yw;
+            }
+        }
+        return;
+    }
 
     // ---- dfs = m - tr(M^-1)
     double trx = 0.0;

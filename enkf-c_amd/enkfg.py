@@ -32,7 +32,7 @@ EXPORTED = [
     "enkfg_set_obs_status", "enkfg_set_S", "enkfg_get_S", "enkfg_calc_transforms", "enkfg_get_transforms",
     "enkfg_get_diag", "enkfg_get_stats", "enkfg_localise", "enkfg_get_local_counts", "enkfg_get_localobs",
     "enkfg_set_transforms", "enkfg_update_fields", "enkfg_update_bg", "enkfg_ens_attach", "enkfg_ens_update",
-    "enkfg_get_timings",
+    "enkfg_get_timings", "enkfg_adjoint",
 ]
 
 
@@ -108,6 +108,7 @@ def lib():
         L.enkfg_ens_attach.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Field), vp]
         L.enkfg_ens_update.argtypes = [vp, C.c_int, vp]
         L.enkfg_get_timings.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
+        L.enkfg_adjoint.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -267,6 +268,16 @@ class Engine:
         t = (C.c_double * 4)()
         self._check(lib().enkfg_get_timings(self.h, t, 4))
         return list(t)
+
+    def adjoint(self, dev_dJ_dmean, nobs):
+        """Gradients of J with respect to the observation values and the
+        observation-error variances, given dJ/d(analysis mean) as a device
+        pointer to float [nfield][ncell]; returns (dJ/dy, dJ/dR, dJ/dinfl)."""
+        gy = np.zeros(nobs, np.float64)
+        gR = np.zeros(nobs, np.float64)
+        gi = C.c_double()
+        self._check(lib().enkfg_adjoint(self.h, C.c_void_p(dev_dJ_dmean), _p(gy), _p(gR), C.byref(gi)))
+        return gy, gR, gi.value
 
     def launches(self):
         """Gram (and factor) launches of the last calc_transforms."""

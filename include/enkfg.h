@@ -210,6 +210,26 @@ int enkfg_update_bg(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const en
 int enkfg_ens_attach(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const enkfg_field* fields, float* dev_E);
 int enkfg_ens_update(enkfg_ctx* ctx, int flags, float* dev_infl);
 
+/* ---------------------------------------------------------------- adjoint */
+/* Adjoint of the analysis mean.  No counterpart in the reference; it
+ * differentiates the reference's own forward path: w = M^-1 S^T s per node
+ * (calc/calcs.c:189-263 on the standardised S, s of calc/ensobs.c:849-898),
+ * the stride-grid interpolation of w (update/update.c:195-272) and
+ * x^a = x^f + A^f w (update.c:381-410).
+ *   dev_dJ_dmean  device float [nfield][mnj * mni]: dJ/dx^a of a scalar J for
+ *                 the fields of the attached ensemble (zero where J does not
+ *                 depend on the cell)
+ *   dJ_dy, dJ_dR  host double [nobs]: gradients with respect to every
+ *                 observation value and observation-error variance
+ *                 (R = estd^2), in observation order; either may be NULL
+ *   dJ_dinfl      host double: gradient with respect to the inflation, which
+ *                 scales the analysis anomalies only and so is 0 for J of
+ *                 the mean
+ * Needs enkfg_calc_transforms for the grid and the forecast ensemble attached
+ * (enkfg_ens_attach, before enkfg_ens_update); recomputes I + S^T S and its
+ * inverse in FP64 for the nodes that J depends on (none is stored); nmem <= 96. */
+int enkfg_adjoint(enkfg_ctx* ctx, const float* dev_dJ_dmean, double* dJ_dy, double* dJ_dR, double* dJ_dinfl);
+
 /* hipEvent-measured duration (ms) of the last calc / update kernels, per
  * phase: [0] localisation, [1] Gram, [2] factorisation, [3] update; [4] is
  * the number of Gram (and factor) launches of the last calc and [5] the

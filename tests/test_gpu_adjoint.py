@@ -1,0 +1,184 @@
+"""The adjoint of the analysis mean (enkfg_adjoint) against an FP64 reference
+gradient and central finite differences.
+
+J = sum over a box of cells and the fields of the ensemble of
+(x^a - x^t)^2, with x^a = x^f + A^f w the analysis mean of update.c:381-410
+and w interpolated from the stride-grid transforms.  The GPU forward runs
+through the C ABI (FP64 calc path); its backward is enkfg_adjoint, wrapped in
+a torch.autograd.Function here only.  The reference is the same forward
+written in float64 torch with no FP32 rounding of S, differentiated by torch
+autograd, and central differences of that forward (h = 1e-6, relative for
+the variances).  Bounds: max |gpu - reference| <= 1e-4 max |reference| for
+dJ/dy and dJ/dR; dJ/dinflation is exactly 0 (inflation scales anomalies
+only).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import enkfg
+import oracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+class Case:
+    def __init__(self, seed=31):
+        P = synth.make_problem(seed=seed, nj=24, ni=30, nk=2, nmem=24, nobs=700, locrad=80.0, stride=3, profiles=10)
+        self.P = P
+        m = P.nmem
+        self.m = m
+        self.nobs = P.nobs
+        self.rf = np.array([P.obstypes[t]["rfactor"] for t in P.obs_type], np.float64)
+        ob = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd)
+        self.lists = [ob.findlocal(lo, la) for lo, la in zip(P.node_lon.ravel(), P.node_lat.ravel())]
+        F = [P.ens["temp"][:, k] for k in range(P.nk)] + [P.ens["salt"][:, k] for k in range(P.nk)]
+        self.fields = [dict(level=k) for k in range(P.nk)] * 2
+        Ef = np.stack(F).astype(np.float32)                        # [nfield][m][nj][ni]
+        self.Ecell = np.ascontiguousarray(Ef.reshape(len(F), m, -1).transpose(0, 2, 1))   # [f][cell][m]
+        X = self.Ecell.astype(np.float64)
+        self.xf = X.mean(2)
+        self.Af = torch.tensor(X - self.xf[:, :, None])
+        self.xf = torch.tensor(self.xf)
+        xt = [P.truth["temp"][k] for k in range(P.nk)] + [P.truth["salt"][k] for k in range(P.nk)]
+        self.xt = torch.tensor(np.stack(xt).reshape(len(xt), -1).astype(np.float64))
+        box = np.zeros((P.nj, P.ni))
+        box[6:17, 9:22] = 1.0
+        self.box = torch.tensor(np.broadcast_to(box.ravel(), self.xt.shape).copy())
+        self.B = torch.tensor(self.interp_matrix())
+        self.HA = torch.tensor(P.HA.astype(np.float64))           # [m][nobs]
+
+    def interp_matrix(self):
+        """cells x nodes: the bilinear weights of update.c:195-272 (FP32
+        weights, as real numbers)"""
+        P = self.P
+        s = P.stride
+        B = np.zeros((P.nj * P.ni, P.nj_s * P.ni_s))
+        for j in range(P.nj):
+            for i in range(P.ni):
+                jj, sj, ii, si = j // s, j % s, i // s, i % s
+                jj2, ii2 = min(jj + 1, P.nj_s - 1), min(ii + 1, P.ni_s - 1)
+                wj2 = np.float32(sj) / np.float32(s)
+                wi2 = np.float32(si) / np.float32(s)
+                bj = (1.0, 0.0) if sj == 0 else (float(np.float32(1) - wj2), float(wj2))
+                bi = (1.0, 0.0) if si == 0 else (float(np.float32(1) - wi2), float(wi2))
+                c = j * P.ni + i
+                for (nj_, bjv) in ((jj, bj[0]), (jj2, bj[1])):
+                    for (ni_, biv) in ((ii, bi[0]), (ii2, bi[1])):
+                        B[c, nj_ * P.ni_s + ni_] += bjv * biv
+        return B
+
+    def mean_from_w(self, W):
+        """x^a = x^f + A^f w_cell, W: [node][m] float64 tensor"""
+        Wc = self.B @ W                                          # [cell][m]
+        return self.xf + (self.Af * Wc[None]).sum(2)
+
+    def J(self, xa):
+        return (((xa - self.xt) ** 2) * self.box).sum()
+
+    def forward64(self, innov, R):
+        """the calc in float64 torch, S not rounded to FP32"""
+        m = self.m
+        rf = torch.tensor(self.rf)
+        W = []
+        for ids, lc in self.lists:
+            if len(ids) == 0:
+                W.append(torch.zeros(m, dtype=torch.float64))
+                continue
+            ids_t = torch.tensor(ids, dtype=torch.long)
+            lc_t = torch.tensor(lc)
+            d = torch.sqrt(R[ids_t] * rf[ids_t]) * math.sqrt(m - 1)
+            S = self.HA[:, ids_t].T / d[:, None] * lc_t[:, None]
+            s = innov[ids_t] / d * lc_t
+            M = torch.eye(m, dtype=torch.float64) + S.T @ S
+            W.append(torch.linalg.solve(M, S.T @ s))
+        return self.mean_from_w(torch.stack(W))
+
+
+class GpuMean(torch.autograd.Function):
+    """x^a from the GPU calc (forward) and enkfg_adjoint (backward)"""
+
+    @staticmethod
+    def forward(ctx, innov, R, case, eng, dev_E):
+        P = case.P
+        estd = np.sqrt(R.detach().numpy()).astype(np.float32)
+        eng.set_obs(P.obs_type, P.obs_status, P.obs_lon, P.obs_lat, estd)
+        eng.set_S(P.HA, innov.detach().numpy(), standardised=False)
+        eng.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
+        _, _, _, w64 = eng.transforms(fp64=True)
+        eng.ens_attach(P.nlevels, P.stride, 0, 0, case.fields, dev_E.data_ptr())
+        ctx.case, ctx.eng = case, eng
+        return case.mean_from_w(torch.tensor(w64.reshape(-1, case.m)))
+
+    @staticmethod
+    def backward(ctx, grad):
+        dJ = grad.to(torch.float32).contiguous().cuda()
+        gy, gR, gi = ctx.eng.adjoint(dJ.data_ptr(), ctx.case.nobs)
+        torch.cuda.synchronize()
+        ctx.ginfl = gi
+        assert gi == 0.0
+        return torch.tensor(gy), torch.tensor(gR), None, None, None
+
+
+@pytest.fixture(scope="module")
+def case():
+    return Case()
+
+
+def test_adjoint_against_autograd_and_finite_differences(case):
+    print("case adjoint m=%d nobs=%d" % (case.m, case.nobs), flush=True)
+    P = case.P
+    R0 = torch.tensor(P.obs_estd.astype(np.float64) ** 2)
+    innov0 = torch.tensor(P.innov)
+    # reference gradient: float64 forward, torch autograd
+    iv = innov0.clone().requires_grad_(True)
+    Rv = R0.clone().requires_grad_(True)
+    Jr = case.J(case.forward64(iv, Rv))
+    Jr.backward()
+    gy_ref, gR_ref = iv.grad.clone(), Rv.grad.clone()
+    # GPU forward + enkfg_adjoint
+    eng = enkfg.Engine(case.m, mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF, keep_fp64=True)
+    eng.set_obstypes(P.obstypes)
+    dev_E = torch.tensor(case.Ecell, device="cuda")
+    ig = innov0.clone().requires_grad_(True)
+    Rg = R0.clone().requires_grad_(True)
+    Jg = case.J(GpuMean.apply(ig, Rg, case, eng, dev_E))
+    Jg.backward()
+    eng.close()
+    gy, gR = ig.grad, Rg.grad
+    assert abs(Jg.item() - Jr.item()) <= 1e-5 * abs(Jr.item())
+    ey = float((gy - gy_ref).abs().max() / gy_ref.abs().max())
+    eR = float((gR - gR_ref).abs().max() / gR_ref.abs().max())
+    print("  J %.6g  |dJ/dy| %.3g  |dJ/dR| %.3g  err vs autograd: y %.2g  R %.2g"
+          % (Jr.item(), gy_ref.abs().max(), gR_ref.abs().max(), ey, eR), flush=True)
+    assert (gy_ref != 0).sum() > 50, "the box must be sensitive to many observations"
+    assert ey <= 1e-4 and eR <= 1e-4
+    # central differences of the float64 forward on the most sensitive obs
+    with torch.no_grad():
+        for name, base, g in (("y", innov0, gy), ("R", R0, gR)):
+            worst = 0.0
+            for o in torch.argsort(g.abs(), descending=True)[:6].tolist():
+                h = 1e-6 if name == "y" else 1e-6 * float(base[o])
+                xp, xm = base.clone(), base.clone()
+                xp[o] += h
+                xm[o] -= h
+                if name == "y":
+                    fd = (case.J(case.forward64(xp, R0)) - case.J(case.forward64(xm, R0))) / (2 * h)
+                else:
+                    fd = (case.J(case.forward64(innov0, xp)) - case.J(case.forward64(innov0, xm))) / (2 * h)
+                worst = max(worst, abs(float(fd) - float(g[o])) / float(g.abs().max()))
+            print("  finite differences %s: max error %.2g" % (name, worst), flush=True)
+            assert worst <= 1e-4
+
+
+def test_adjoint_requires_the_calc_and_the_ensemble(case):
+    eng = enkfg.Engine(case.m, mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF)
+    eng.set_obstypes(case.P.obstypes)
+    dJ = torch.zeros(4, 10, device="cuda")
+    with pytest.raises(enkfg.EnkfgError) as ei:
+        eng.adjoint(dJ.data_ptr(), case.nobs)
+    assert ei.value.code == enkfg.ERR_STATE
+    eng.close()
