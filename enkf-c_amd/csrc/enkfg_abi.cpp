@@ -105,6 +105,7 @@ struct enkfg_ctx {
     // localisation
     int nnode = 0;
     DevBuf d_qxyz, d_glob, d_vis, d_counts, d_offsets, d_scantmp, d_ids, d_lc;
+    DevBuf d_raw, d_rawoff, d_rrow, d_rd2, d_srow, d_sd2;   // NLOBSMAX candidates
     bool lists_valid = false;
 
     // transforms and diagnostics
@@ -296,9 +297,10 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
 {
     if (!c->obs_set)
         return fail(c, ENKFG_ERR_STATE, "observations not set");
+    bool nlobsmax = false;
     for (const auto& t : c->types)
         if (t.nlobsmax != ENKFG_NLOBSMAX_NONE)
-            return fail(c, ENKFG_ERR_UNSUPPORTED, "NLOBSMAX is not supported by the GPU search yet");
+            nlobsmax = true;
     int r = node_inputs(c, nodes);
     if (r != ENKFG_OK)
         return r;
@@ -307,14 +309,44 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
     HIPCHECK(c, c->d_counts.reserve(std::max<long long>(ncnt, 1) * sizeof(int)));
     HIPCHECK(c, c->d_offsets.reserve((ncnt + 1) * sizeof(long long)));
     const unsigned char* vis = nodes->visible ? c->d_vis.as<unsigned char>() : nullptr;
+    if (nlobsmax)
+        HIPCHECK(c, c->d_raw.reserve(std::max<long long>(ncnt, 1) * sizeof(int)));
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHECK(c, launch_loc_count(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
                                  c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(),
-                                 c->d_counts.as<int>()));
+                                 c->d_counts.as<int>(), nlobsmax ? c->d_raw.as<int>() : nullptr));
     size_t tmpb = 0;
     HIPCHECK(c, launch_scan_counts(c->stream, nullptr, nullptr, ncnt, nullptr, &tmpb));
     HIPCHECK(c, c->d_scantmp.reserve(tmpb));
     tmpb = c->d_scantmp.bytes;
+    if (nlobsmax) {
+        // NLOBSMAX types: candidates in visiting order, sorted by distance,
+        // the first NLOBSMAX with status OK kept (observations.c:1899-1908)
+        HIPCHECK(c, c->d_rawoff.reserve((ncnt + 1) * sizeof(long long)));
+        HIPCHECK(c, launch_scan_counts(c->stream, c->d_raw.as<int>(), c->d_rawoff.as<long long>(), ncnt,
+                                       c->d_scantmp.p, &tmpb));
+        long long nraw = 0;
+        HIPCHECK(c, hipMemcpyAsync(&nraw, c->d_rawoff.as<long long>() + ncnt, sizeof nraw, hipMemcpyDeviceToHost,
+                                   c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        HIPCHECK(c, c->d_rrow.reserve(std::max<long long>(nraw, 1) * sizeof(int)));
+        HIPCHECK(c, c->d_srow.reserve(std::max<long long>(nraw, 1) * sizeof(int)));
+        HIPCHECK(c, c->d_rd2.reserve(std::max<long long>(nraw, 1) * sizeof(double)));
+        HIPCHECK(c, c->d_sd2.reserve(std::max<long long>(nraw, 1) * sizeof(double)));
+        int zero = 0;
+        HIPCHECK(c, hipMemcpy(c->d_err.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+        HIPCHECK(c, launch_loc_nlobsmax(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(),
+                                        c->d_glob.as<unsigned char>(), vis, c->d_nodes.as<KdNode>(),
+                                        c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(), c->d_raw.as<int>(),
+                                        c->d_rawoff.as<long long>(), c->d_rrow.as<int>(), c->d_rd2.as<double>(),
+                                        c->d_srow.as<int>(), c->d_sd2.as<double>(), c->d_counts.as<int>(),
+                                        c->d_err.as<int>()));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        int err = 0;
+        HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
+        if (err != 0)
+            return fail(c, ENKFG_ERR_UNSUPPORTED, "NLOBSMAX: more than 8192 candidate obs of one type at a node");
+    }
     HIPCHECK(c, launch_scan_counts(c->stream, c->d_counts.as<int>(), c->d_offsets.as<long long>(), ncnt,
                                    c->d_scantmp.p, &tmpb));
     long long total = 0;
@@ -327,7 +359,9 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
     HIPCHECK(c, launch_loc_fill(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
                                 c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(),
                                 c->d_obsids.as<int>(), c->d_offsets.as<long long>(), c->d_ids.as<int>(),
-                                c->d_lc.as<double>()));
+                                c->d_lc.as<double>(), nlobsmax ? c->d_rawoff.as<long long>() : nullptr,
+                                nlobsmax ? c->d_raw.as<int>() : nullptr, nlobsmax ? c->d_srow.as<int>() : nullptr,
+                                nlobsmax ? c->d_sd2.as<double>() : nullptr));
     HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->timings[0] = elapsed(c->ev[0], c->ev[1]);
@@ -396,7 +430,8 @@ void enkfg_destroy(enkfg_ctx* c)
         (void)hipStreamSynchronize(c->stream2);
     DevBuf* bufs[] = {&c->d_nodes, &c->d_types, &c->d_obsids, &c->d_type, &c->d_status, &c->d_estd, &c->d_sqrtrf,
                       &c->d_perm, &c->d_S, &c->d_s, &c->d_tmp, &c->d_tmp2, &c->d_qxyz, &c->d_glob, &c->d_vis, &c->d_counts, &c->d_offsets,
-                      &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
+                      &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_raw, &c->d_rawoff, &c->d_rrow, &c->d_rd2,
+                      &c->d_srow, &c->d_sd2, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
                       &c->d_pdfs, &c->d_psrf, &c->d_gram, &c->d_sts, &c->d_err, &c->d_E, &c->d_infl, &c->d_nlev,
                       &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio, &c->d_adjg, &c->d_adjflag,
                       &c->d_adjlist, &c->d_adjz, &c->d_adjw, &c->d_gy, &c->d_gR};

@@ -43,11 +43,18 @@ struct TypeInfo {
 // localisation (loc_kernels.hip)
 hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                             const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
-                            const short* status, int* counts);
+                            const short* status, int* counts, int* raw);
+// NLOBSMAX types: candidates (raw counts scanned into rawoff) gathered, sorted
+// by distance and selected; writes the list lengths into counts; *err = 1
+// when a (node, type) has more candidates than the sort handles (8192)
+hipError_t launch_loc_nlobsmax(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
+                               const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
+                               const short* status, const int* raw, const long long* rawoff, int* rrow, double* rd2,
+                               int* srow, double* sd2, int* counts, int* err);
 hipError_t launch_loc_fill(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                            const short* status, const int* obsids, const long long* offsets, int* ids,
-                           double* lc);
+                           double* lc, const long long* rawoff, const int* raw, const int* srow, const double* sd2);
 hipError_t launch_scan_counts(hipStream_t s, const int* counts, long long* offsets, long long n, void* tmp,
                               size_t* tmp_bytes);
 

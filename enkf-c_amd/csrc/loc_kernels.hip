@@ -135,7 +135,8 @@ __global__ void __launch_bounds__(64) loc_count_kernel(int nnode, const double* 
                                                         const unsigned char* __restrict__ visible,
                                                         const KdNode* __restrict__ nodes,
                                                         const TypeInfo* __restrict__ types, int ntypes,
-                                                        const short* __restrict__ status, int* __restrict__ counts)
+                                                        const short* __restrict__ status, int* __restrict__ counts,
+                                                        int* __restrict__ raw)
 {
     int t, node;
     if (!thread_node(nnode, ntypes, t, node))
@@ -143,10 +144,13 @@ __global__ void __launch_bounds__(64) loc_count_kernel(int nnode, const double* 
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
     const TypeInfo& ti = types[t];
-    int n = 0;
+    int n = 0, nraw = 0;
     if (ti.searchable && (visible == nullptr || visible[t])) {
         if (glob) {
             n = ti.nids;   // obs_getglobal: all obs of the type, no status check
+        } else if (ti.nnode > 0 && ti.nlobsmax != ENKFG_NLOBSMAX_NONE) {
+            // NLOBSMAX: every hit is a candidate; the sort kernel selects
+            range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int, double) { nraw++; });
         } else if (ti.nnode > 0) {
             range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int v, double) {
                 if (status[ti.node0 + v] == kStatusOK)
@@ -155,6 +159,110 @@ __global__ void __launch_bounds__(64) loc_count_kernel(int nnode, const double* 
         }
     }
     counts[(size_t)node * ntypes + t] = n;
+    if (raw != nullptr)
+        raw[(size_t)node * ntypes + t] = nraw;
+}
+
+// NLOBSMAX candidates of a (node, type): rows and squared distances in the
+// reference's visiting order
+__global__ void __launch_bounds__(64) loc_rawfill_kernel(int nnode, const double* __restrict__ qxyz,
+                                                         const unsigned char* __restrict__ isglobal,
+                                                         const unsigned char* __restrict__ visible,
+                                                         const KdNode* __restrict__ nodes,
+                                                         const TypeInfo* __restrict__ types, int ntypes,
+                                                         const long long* __restrict__ rawoff, int* __restrict__ rrow,
+                                                         double* __restrict__ rd2)
+{
+    int t, node;
+    if (!thread_node(nnode, ntypes, t, node))
+        return;
+    const TypeInfo& ti = types[t];
+    if (isglobal[node] || !ti.searchable || !(visible == nullptr || visible[t]) || ti.nnode == 0 ||
+        ti.nlobsmax == ENKFG_NLOBSMAX_NONE)
+        return;
+    double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
+    long long o = rawoff[(size_t)node * ntypes + t];
+    range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int v, double distsq) {
+        rrow[o] = ti.node0 + v;
+        rd2[o] = distsq;
+        o++;
+    });
+}
+
+// The candidates of one (node, type) sorted by distance, ties in visiting
+// order (the stable sort of kd_findnodeswithinrange's qsort on this
+// platform, common/kdtree.c:569-570), by a bitonic sort of (distsq, index)
+// keys in LDS; then obs_findlocal's selection (observations.c:1899-1908):
+// the first min(n, NLOBSMAX) candidates, skipping those whose status is not
+// OK.  Writes the sorted candidates and the list length.
+constexpr int kSortMax = 8192;
+
+__global__ void __launch_bounds__(256) loc_sort_kernel(int nnode, const TypeInfo* __restrict__ types, int ntypes,
+                                                       const short* __restrict__ status,
+                                                       const int* __restrict__ raw,
+                                                       const long long* __restrict__ rawoff,
+                                                       const int* __restrict__ rrow, const double* __restrict__ rd2,
+                                                       int* __restrict__ srow, double* __restrict__ sd2,
+                                                       int* __restrict__ counts, int* __restrict__ err)
+{
+    __shared__ double kd[kSortMax];
+    __shared__ int ki[kSortMax];
+    __shared__ int cnt;
+    const long long item = blockIdx.x;
+    if (item >= (long long)nnode * ntypes)
+        return;
+    const int n = raw[item];
+    if (n == 0)
+        return;
+    if (n > kSortMax) {
+        if (threadIdx.x == 0)
+            atomicExch(err, 1);
+        return;
+    }
+    const TypeInfo& ti = types[item % ntypes];
+    const long long off = rawoff[item];
+    int P = 1;
+    while (P < n)
+        P <<= 1;
+    for (int k = threadIdx.x; k < P; k += 256) {
+        kd[k] = (k < n) ? rd2[off + k] : INFINITY;
+        ki[k] = k;
+    }
+    if (threadIdx.x == 0)
+        cnt = 0;
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < P / 2; k += 256) {
+                const int lo = 2 * stride * (k / stride) + (k % stride);
+                const int hi = lo + stride;
+                const bool up = ((lo & size) == 0);
+                const double a = kd[lo], b = kd[hi];
+                const int ia = ki[lo], ib = ki[hi];
+                const bool agt = (a > b) || (a == b && ia > ib);
+                if (agt == up) {
+                    kd[lo] = b;
+                    kd[hi] = a;
+                    ki[lo] = ib;
+                    ki[hi] = ia;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int nsel = min(n, ti.nlobsmax);
+    int mine = 0;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const int row = rrow[off + ki[k]];
+        srow[off + k] = row;
+        sd2[off + k] = kd[k];
+        if (k < nsel && status[row] == kStatusOK)
+            mine++;
+    }
+    atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        counts[item] = cnt;
 }
 
 __global__ void __launch_bounds__(64) loc_fill_kernel(int nnode, const double* __restrict__ qxyz,
@@ -165,7 +273,9 @@ __global__ void __launch_bounds__(64) loc_fill_kernel(int nnode, const double* _
                                                        const short* __restrict__ status,
                                                        const int* __restrict__ obsids,
                                                        const long long* __restrict__ offsets, int* __restrict__ ids,
-                                                       double* __restrict__ lc)
+                                                       double* __restrict__ lc, const long long* __restrict__ rawoff,
+                                                       const int* __restrict__ raw, const int* __restrict__ srow,
+                                                       const double* __restrict__ sd2)
 {
     int t, node;
     if (!thread_node(nnode, ntypes, t, node))
@@ -180,6 +290,18 @@ __global__ void __launch_bounds__(64) loc_fill_kernel(int nnode, const double* _
         for (int i = 0; i < ti.nids; ++i) {
             ids[o + i] = obsids[ti.obsid0 + i];
             lc[o + i] = 1.0;
+        }
+    } else if (ti.nnode > 0 && ti.nlobsmax != ENKFG_NLOBSMAX_NONE) {
+        const size_t item = (size_t)node * ntypes + t;
+        const long long r0 = rawoff[item];
+        const int nsel = min(raw[item], ti.nlobsmax);
+        for (int k = 0; k < nsel; ++k) {
+            const int row = srow[r0 + k];
+            if (status[row] == kStatusOK) {
+                ids[o] = row;
+                lc[o] = calclcoeff(ti, sqrt(sd2[r0 + k]));
+                o++;
+            }
         }
     } else if (ti.nnode > 0) {
         range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int v, double distsq) {
@@ -278,25 +400,43 @@ __global__ void __launch_bounds__(kScanThreads) scan_final_kernel(const int* __r
 
 hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                             const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
-                            const short* status, int* counts)
+                            const short* status, int* counts, int* raw)
 {
     const long long nt = (long long)ni * nj * ntypes;
     if (nt <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(loc_count_kernel, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, s, ni * nj, qxyz, isglobal,
-                       visible, nodes, types, ntypes, status, counts);
+                       visible, nodes, types, ntypes, status, counts, raw);
+    return hipGetLastError();
+}
+
+hipError_t launch_loc_nlobsmax(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
+                               const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
+                               const short* status, const int* raw, const long long* rawoff, int* rrow, double* rd2,
+                               int* srow, double* sd2, int* counts, int* err)
+{
+    const long long nt = (long long)ni * nj * ntypes;
+    if (nt <= 0)
+        return hipSuccess;
+    if (nt > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(loc_rawfill_kernel, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, s, ni * nj, qxyz, isglobal,
+                       visible, nodes, types, ntypes, rawoff, rrow, rd2);
+    hipLaunchKernelGGL(loc_sort_kernel, dim3((unsigned)nt), dim3(256), 0, s, ni * nj, types, ntypes, status, raw,
+                       rawoff, rrow, rd2, srow, sd2, counts, err);
     return hipGetLastError();
 }
 
 hipError_t launch_loc_fill(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
-                           const short* status, const int* obsids, const long long* offsets, int* ids, double* lc)
+                           const short* status, const int* obsids, const long long* offsets, int* ids, double* lc,
+                           const long long* rawoff, const int* raw, const int* srow, const double* sd2)
 {
     const long long nt = (long long)ni * nj * ntypes;
     if (nt <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(loc_fill_kernel, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, s, ni * nj, qxyz, isglobal,
-                       visible, nodes, types, ntypes, status, obsids, offsets, ids, lc);
+                       visible, nodes, types, ntypes, status, obsids, offsets, ids, lc, rawoff, raw, srow, sd2);
     return hipGetLastError();
 }
 

@@ -75,6 +75,45 @@ def test_localisation_status_visibility_global():
     E.close()
 
 
+@pytest.mark.parametrize("nmax", [1, 17, 60])
+def test_localisation_nlobsmax(nmax):
+    """NLOBSMAX (observations.c:1899-1908): candidates sorted by distance,
+    ties (profile levels at one position) in visiting order, the first
+    NLOBSMAX kept, then the search-time status filter."""
+    print("case localisation nlobsmax=%d" % nmax, flush=True)
+    P = problem(seed=5, profiles=40, nobs=2500, locrad=90.0)
+    P.obstypes[0]["nlobsmax"] = nmax
+    P.obstypes[1]["nlobsmax"] = nmax // 2 + 1
+    rng = np.random.default_rng(9)
+    status = P.obs_status.copy()
+    status[rng.random(P.nobs) < 0.1] = 7          # bad after the trees were built
+    E = engine_for(P, enkfg.MODE_ENKF, enkfg.SCHEME_DENKF)
+    E.set_obs_status(status)
+    E.localise(P.node_lon, P.node_lat, stride=P.stride)
+    counts = E.local_counts()
+    ob = oracle_obs(P, status=status, status_build=P.obs_status)
+    ties = 0
+    for k in range(P.node_lon.size):
+        ids_o, lc_o = ob.findlocal(P.node_lon.flat[k], P.node_lat.flat[k])
+        assert counts[k] == len(ids_o), "node %d count" % k
+        ids_g, lc_g = E.localobs(k, counts[k])
+        assert np.array_equal(ids_g[:counts[k]], ids_o), "node %d ids" % k
+        assert np.array_equal(lc_g[:counts[k]].view(np.uint64), lc_o.view(np.uint64))
+        ties += len(lc_o) - len(np.unique(lc_o))
+    if nmax > 1:
+        assert ties > 0, "the case must contain equal distances"
+    # and the transforms with NLOBSMAX lists
+    S, s = O.standardise(P.nmem, ob, P.HA, P.innov)
+    ref = O.calc_transforms(enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, P.nmem, P.nmem, 1.0, ob, S, s,
+                            P.node_lon.ravel(), P.node_lat.ravel())
+    E.set_S(P.HA, P.innov, standardised=False)
+    E.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
+    T, w, T64, w64 = E.transforms(fp64=True)
+    assert np.array_equal(E.diag()[0], ref["pnlobs"])
+    assert relerr(w64, ref["w64"]) < 1e-10 and relerr(T64, ref["T64"]) < 1e-10
+    E.close()
+
+
 def relerr(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
