@@ -131,7 +131,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
 {
     using G = UpdGeom<MT, MIX>;
     constexpr int MP = G::MP, KP = G::KP;
-    constexpr int QM = MP / 4;            // members per stats lane
     __shared__ UpdLds<MT, MIX> sh;
 
     const long long cell = cell_of_block(a.ncell);
@@ -152,24 +151,57 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
     {
         const long long rec = (long long)md * m;
         const Bilin b = make_bilin(a, j, i, rec);
-        constexpr int UB = 8;
-        const int n = md * m;
-        for (int q0 = tid; q0 < n; q0 += 256 * UB) {
-            float v00[UB], v10[UB], v01[UB], v11[UB];
+        if ((m & 3) == 0) {
+            // quads of 4 consecutive e2: row e1 = q / (m / 4) by a multiply-high
+            // (exact for these small q)
+            constexpr int UB = 4;
+            const int d = m >> 2;
+            const unsigned magic = 0xffffffffu / (unsigned)d + 1u;
+            const int n = md * d;
+            for (int q0 = tid; q0 < n; q0 += 256 * UB) {
+                float4 v00[UB], v10[UB], v01[UB], v11[UB];
 #pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                const int q = min(q0 + 256 * u, n - 1);   // in range; duplicates are not stored
-                v00[u] = a.T[b.o00 + q];
-                v10[u] = a.T[b.o10 + q];
-                v01[u] = a.T[b.o01 + q];
-                v11[u] = a.T[b.o11 + q];
+                for (int u = 0; u < UB; ++u) {
+                    const int q = 4 * min(q0 + 256 * u, n - 1);   // in range; duplicates are not stored
+                    v00[u] = *reinterpret_cast<const float4*>(a.T + b.o00 + q);
+                    v10[u] = *reinterpret_cast<const float4*>(a.T + b.o10 + q);
+                    v01[u] = *reinterpret_cast<const float4*>(a.T + b.o01 + q);
+                    v11[u] = *reinterpret_cast<const float4*>(a.T + b.o11 + q);
+                }
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    const int q = q0 + 256 * u;
+                    if (q < n) {
+                        const int e1 = (int)__umulhi((unsigned)q, magic);
+                        const int e2 = 4 * (q - e1 * d);
+                        store_t<MT, MIX>(sh, e1, e2, bilin(b, v00[u].x, v10[u].x, v01[u].x, v11[u].x));
+                        store_t<MT, MIX>(sh, e1, e2 + 1, bilin(b, v00[u].y, v10[u].y, v01[u].y, v11[u].y));
+                        store_t<MT, MIX>(sh, e1, e2 + 2, bilin(b, v00[u].z, v10[u].z, v01[u].z, v11[u].z));
+                        store_t<MT, MIX>(sh, e1, e2 + 3, bilin(b, v00[u].w, v10[u].w, v01[u].w, v11[u].w));
+                    }
+                }
             }
+        } else {
+            constexpr int UB = 8;
+            const unsigned magic = 0xffffffffu / (unsigned)m + 1u;
+            const int n = md * m;
+            for (int q0 = tid; q0 < n; q0 += 256 * UB) {
+                float v00[UB], v10[UB], v01[UB], v11[UB];
 #pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                const int q = q0 + 256 * u;
-                if (q < n) {
-                    const int e1 = q / m, e2 = q - e1 * m;
-                    store_t<MT, MIX>(sh, e1, e2, bilin(b, v00[u], v10[u], v01[u], v11[u]));
+                for (int u = 0; u < UB; ++u) {
+                    const int q = min(q0 + 256 * u, n - 1);   // in range; duplicates are not stored
+                    v00[u] = a.T[b.o00 + q];
+                    v10[u] = a.T[b.o10 + q];
+                    v01[u] = a.T[b.o01 + q];
+                    v11[u] = a.T[b.o11 + q];
+                }
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    const int q = q0 + 256 * u;
+                    if (q < n) {
+                        const int e1 = (int)__umulhi((unsigned)q, magic), e2 = q - e1 * m;
+                        store_t<MT, MIX>(sh, e1, e2, bilin(b, v00[u], v10[u], v01[u], v11[u]));
+                    }
                 }
             }
         }
@@ -213,19 +245,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
         }
         __syncthreads();
         // ---- 2b. per-field statistics: lane (f, g) covers members
-        // [g QM, (g + 1) QM); the 4 lanes of a field are lanes f, f + 16,
-        // f + 32, f + 48 of wave f / 16
+        // e = g (mod 4); the 4 lanes of a field are lanes f, f + 16, f + 32,
+        // f + 48 of wave f / 16 (and read 4 consecutive rows of at[][], which
+        // fall in disjoint banks)
         {
             const int fl = 16 * w + (lane & 15);
             const int g = lane >> 4;
             const int fgl = f0 + fl;
             const bool live = fgl < a.nfield;
-            const int e0 = g * QM;
-            const int e1 = min(e0 + QM, m);
             int eb = m;
-            for (int e = e1 - 1; e >= e0; --e) {
+            for (int e = g; e < m; e += 4) {
                 const float v = sh.at[e][fl];
-                if (!isfinite(v) || fabsf(v) > kMaxVal)
+                if (eb == m && (!isfinite(v) || fabsf(v) > kMaxVal))
                     eb = e;
             }
             eb = min(eb, __shfl_xor(eb, 16));
@@ -239,15 +270,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
                 z = 2;
             // FP64 mean of the dynamic members (partials combined in lane order)
             double p = 0.0;
-            for (int e = e0; e < e1; ++e)
-                if (e < md)
-                    p += (double)((z == 2 && e >= md) ? 0.0f : sh.at[e][fl]);
+            for (int e = g; e < md; e += 4)
+                p += (double)sh.at[e][fl];
             double s0 = __shfl(p, lane & 15), s1 = __shfl(p, (lane & 15) + 16);
             double s2 = __shfl(p, (lane & 15) + 32), s3 = __shfl(p, (lane & 15) + 48);
             const double v1_f = (((s0 + s1) + s2) + s3) / (double)md;
             double pa = 0.0, pf = 0.0;
             float amax = 0.0f;
-            for (int e = e0; e < e1; ++e) {
+            for (int e = g; e < m; e += 4) {
                 const float raw = (z == 2 && e >= md) ? 0.0f : sh.at[e][fl];
                 const float vf = (z == 1) ? 0.0f : (float)((double)raw - v1_f);
                 sh.at[e][fl] = vf;

@@ -132,6 +132,7 @@ CASES = [
     (enkfg.MODE_ENKF, enkfg.SCHEME_ETKF, 96, 96, 1.0, 30),
     (enkfg.MODE_ENOI, enkfg.SCHEME_DENKF, 40, 40, 1.0, 0),
     (enkfg.MODE_HYBRID, enkfg.SCHEME_DENKF, 48, 32, 1.0, 20),
+    (enkfg.MODE_HYBRID, enkfg.SCHEME_DENKF, 144, 96, 1.0, 20),    # 96 dynamic + 48 static members
 ]
 
 

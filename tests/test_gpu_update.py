@@ -86,9 +86,9 @@ def test_update_tile_rows():
     E.close()
 
 
-def test_update_hybrid_with_conditioning():
-    print("case update hybrid", flush=True)
-    m, md = 40, 28
+@pytest.mark.parametrize("m,md", [(40, 28), (144, 96)])
+def test_update_hybrid_with_conditioning(m, md):
+    print("case update hybrid m=%d md=%d" % (m, md), flush=True)
     P = synth.make_problem(seed=5, nmem=m, nobs=1200, locrad=60.0, nj=30, ni=36, stride=3)
     T, w = transforms(P, 3, 1, md)
     bufs, fl = fields_of(P, m)
