@@ -6,7 +6,7 @@
 // T and w.  Here the p x m matrices are never formed:
 //
 //   gram   (MFMA FP64)  G0 = S^T S and S^T s, streamed from the gathered
-//                       FP32 rows of S with the taper applied in FP64;
+//                       FP32 rows of S, tapered in FP64 on their way to LDS;
 //   factor (LDS/VGPR)   M = I + G0 inverted by in-register Gauss-Jordan, then
 //                         DEnKF  T = (I + M^-1)/2          (= I - G S / 2)
 //                         ETKF   T = M^-1/2                (= I - (M+M^1/2)^-1 S^T S)
@@ -88,18 +88,18 @@ __global__ void permute_rows_kernel(const T* __restrict__ src, const int* __rest
 
 // ------------------------------------------------------------------ Gram (MFMA)
 
-// Chunks of KC local obs are staged in LDS as the raw FP32 rows of S (double
-// buffered: the gathers of chunk c+1, and the list entries of chunk c+2, are
-// in flight while chunk c feeds the MFMAs); the taper is applied in FP64 when
-// a fragment is formed, (double) S * lc, as the reference forms Sloc.  Row
-// stride RS (floats) is congruent to 16 mod 32 and to 16 or 48 mod 64, so the
-// four rows a k-step reads fall in disjoint LDS banks.
-constexpr int KC = kGramKC;
+// Chunks of KC local obs are staged in LDS as tapered FP64 rows,
+// (double) S * lc as the reference forms Sloc (double buffered: the gathers
+// of chunk c+1, and the list entries of chunk c+2, are in flight while chunk
+// c feeds the MFMAs), so that an MFMA operand is one LDS read.  Row stride RS
+// (doubles) is congruent to 16 mod 32, so the four rows a k-step reads fall
+// in disjoint LDS banks.
+constexpr int KC = 32;
 template <int MT>
 struct GramLds {
     static constexpr int MP = 16 * MT;
     static constexpr int RS = MP + ((16 - MP % 32) + 32) % 32;
-    alignas(16) float rows[2][KC][RS];    // chunk c in rows[c % 2]
+    alignas(16) double rows[2][KC][RS];   // (double) S * lc of chunk c in rows[c % 2]
     double sl[2][KC];         // s_o * lc_o of the chunk's rows, chunk c in [c % 2]
     double lc[3][KC];         // taper coefficients, chunk c in [c % 3]
     int id[3][KC];            // obs ids (-1 past the end), chunk c in [c % 3]
@@ -114,19 +114,24 @@ __device__ inline void chunk_meta_store(GramLds<MT>& sh, int buf, int id, double
     }
 }
 
-// rows of a chunk into row buffer rbuf; s * lc with the tapers of meta
-// buffer mbuf
+// rows of a chunk into row buffer rbuf, tapered in FP64 as the reference
+// forms Sloc ((double) S * lc, calc/transforms.c:794-802); s * lc with the
+// tapers of meta buffer mbuf
 template <int MT>
-__device__ inline void chunk_store(const CalcArgs& a, GramLds<MT>& sh, int rbuf, int mbuf, const GramRegs<MT>& L)
+__device__ inline void chunk_store(const CalcArgs& a, GramLds<MT>& sh, int rbuf, int mbuf, const GramRegs<MT, KC>& L)
 {
     const int qr = a.ms >> 2;
 #pragma unroll
-    for (int j = 0; j < MT; ++j) {
+    for (int j = 0; j < GramRegs<MT, KC>::NQ; ++j) {
         const int q = (int)threadIdx.x + 256 * j;
         const int row = q / qr;
         const int qd = q - row * qr;
-        if (row < KC)
-            *reinterpret_cast<float4*>(&sh.rows[rbuf][row][4 * qd]) = L.v[j];
+        if (row < KC) {
+            const double c = sh.lc[mbuf][row];
+            double2* dst = reinterpret_cast<double2*>(&sh.rows[rbuf][row][4 * qd]);
+            dst[0] = double2{(double)L.v[j].x * c, (double)L.v[j].y * c};
+            dst[1] = double2{(double)L.v[j].z * c, (double)L.v[j].w * c};
+        }
     }
     if (threadIdx.x < KC)
         sh.sl[rbuf][threadIdx.x] = L.s * sh.lc[mbuf][threadIdx.x];
@@ -154,11 +159,10 @@ __device__ inline void gram_wave_chunk(const GramLds<MT>& sh, int buf, int mbuf,
 #pragma unroll 2
     for (int s = 0; s < KC / 4; ++s) {
         const int r = 4 * s + kq;
-        const double c = sh.lc[mbuf][r];
         double frag[MT];
 #pragma unroll
         for (int X = 0; X < MT; ++X)
-            frag[X] = (double)sh.rows[buf][r][16 * X + col] * c;
+            frag[X] = sh.rows[buf][r][16 * X + col];
         mfma_tiles<MT, W, 0>(frag, acc);
         if (W == 0) {
             const double sl = sh.sl[buf][r];
@@ -191,19 +195,19 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
 
     // zero both row buffers once: padding columns stay zero
     for (int q = tid; q < 2 * KC * GramLds<MT>::RS; q += 256)
-        (&sh.rows[0][0][0])[q] = 0.0f;
-    GramRegs<MT> L;
+        (&sh.rows[0][0][0])[q] = 0.0;
+    GramRegs<MT, KC> L;
     const int nchunk = (cnt + KC - 1) / KC;
     int mid;
     double mc;
     // prologue: list entries of chunks 0 and 1, rows of chunk 0
-    gram_meta_load(a, off, cnt, 0, mid, mc);
+    gram_meta_load<KC>(a, off, cnt, 0, mid, mc);
     chunk_meta_store<MT>(sh, 0, mid, mc);
-    gram_meta_load(a, off, cnt, KC, mid, mc);
+    gram_meta_load<KC>(a, off, cnt, KC, mid, mc);
     chunk_meta_store<MT>(sh, 1, mid, mc);
     chunk_meta_store<MT>(sh, 2, -1, 0.0);   // every id buffer holds valid rows or -1
     __syncthreads();
-    gram_gather<MT>(a, sh.id[0], L);
+    gram_gather<MT, KC>(a, sh.id[0], L);
     chunk_store<MT>(a, sh, 0, 0, L);
     __syncthreads();
     // iteration c: gathers of chunk c+1 and list entries of chunk c+2 are in
@@ -213,8 +217,8 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
         const bool more2 = c + 2 < nchunk;
         // the loads are unconditional: past the end they read the padding
         // of the lists and the zero row
-        gram_gather<MT>(a, sh.id[(c + 1) % 3], L);
-        gram_meta_load(a, off, cnt, (c + 2) * KC, mid, mc);
+        gram_gather<MT, KC>(a, sh.id[(c + 1) % 3], L);
+        gram_meta_load<KC>(a, off, cnt, (c + 2) * KC, mid, mc);
         // keep the loads ahead of the MFMAs (the scheduler would sink them
         // to their use, exposing their latency every chunk)
         __builtin_amdgcn_sched_barrier(0);

@@ -50,16 +50,17 @@ __host__ __device__ constexpr size_t gram_index(int i, int j)
 // only some paths, and the loads of chunk c+1 stay in flight while chunk c
 // feeds the MFMAs (a conditionally loaded register forces the compiler to
 // wait for every outstanding load before it can be redefined).
-constexpr int kGramKC = 64;
+constexpr int kGramKC = 64;   // the largest chunk of the Gram kernels
 // a segment's last chunk c reads list entries up to (c + 3) KC - 1, at most
 // 3 KC - 1 past the segment's end
 static_assert(kListPad >= 3 * kGramKC, "list padding too small for the Gram chunk pipeline");
 
 // list entries k0 + (tid % KC) of a segment (every thread loads; threads
 // below KC store)
+template <int KC>
 __device__ inline void gram_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c)
 {
-    const int k = k0 + (int)(threadIdx.x & (kGramKC - 1));
+    const int k = k0 + (int)(threadIdx.x & (KC - 1));
     const int i = a.ids[off + k];
     const double l = a.lc[off + k];
     const bool in = k < cnt;
@@ -68,34 +69,36 @@ __device__ inline void gram_meta_load(const CalcArgs& a, long long off, int cnt,
 }
 
 // One chunk in registers: quad q = tid + 256 j of the KC x (ms / 4) chunk is
-// row q / (ms / 4), members 4 (q % (ms / 4)) ..; MT quads per thread cover
+// row q / (ms / 4), members 4 (q % (ms / 4)) ..; NQ quads per thread cover
 // the chunk for any m <= 16 MT.
-template <int MT>
+template <int MT, int KC>
 struct GramRegs {
-    float4 v[MT];
+    static constexpr int NQ = (KC * 4 * MT + 255) / 256;
+    float4 v[NQ];
     double s;                 // s of row tid % KC
 };
 
+template <int KC>
 __device__ inline int gram_row_id(const int* ids_lds, int row)
 {
-    const int id = ids_lds[row < kGramKC ? row : kGramKC - 1];
-    return row < kGramKC ? id : -1;
+    const int id = ids_lds[row < KC ? row : KC - 1];
+    return row < KC ? id : -1;
 }
 
-template <int MT>
-__device__ inline void gram_gather(const CalcArgs& a, const int* ids_lds, GramRegs<MT>& L)
+template <int MT, int KC>
+__device__ inline void gram_gather(const CalcArgs& a, const int* ids_lds, GramRegs<MT, KC>& L)
 {
     const int qr = a.ms >> 2;
 #pragma unroll
-    for (int j = 0; j < MT; ++j) {
+    for (int j = 0; j < GramRegs<MT, KC>::NQ; ++j) {
         const int q = (int)threadIdx.x + 256 * j;
         const int row = q / qr;
         const int qd = q - row * qr;
-        const int id = gram_row_id(ids_lds, row);
+        const int id = gram_row_id<KC>(ids_lds, row);
         const long long r = id >= 0 ? id : a.nobs;
         L.v[j] = *reinterpret_cast<const float4*>(a.S + r * a.ms + 4 * qd);
     }
-    const int id = ids_lds[threadIdx.x & (kGramKC - 1)];
+    const int id = ids_lds[threadIdx.x & (KC - 1)];
     L.s = a.sv[id >= 0 ? id : a.nobs];
 }
 

@@ -193,9 +193,9 @@ __device__ inline void mix_wave(const CalcArgs& a, long long off, int cnt, doubl
     const int nchunk = (cnt + KC - 1) / KC;
     int mid;
     double mc;
-    gram_meta_load(a, off, cnt, 0, mid, mc);
+    gram_meta_load<KC>(a, off, cnt, 0, mid, mc);
     meta_store<MT>(sh, 0, mid, mc);
-    gram_meta_load(a, off, cnt, KC, mid, mc);
+    gram_meta_load<KC>(a, off, cnt, KC, mid, mc);
     meta_store<MT>(sh, 1, mid, mc);
     meta_store<MT>(sh, 2, -1, 0.0);   // every id buffer holds valid rows or -1
     __syncthreads();
@@ -207,7 +207,7 @@ __device__ inline void mix_wave(const CalcArgs& a, long long off, int cnt, doubl
         const bool more2 = c + 2 < nchunk;
         // unconditional loads (list padding, zero row), see gram_layout.h
         mix_gather<MT>(a, sh.id[(c + 1) % 3], L);
-        gram_meta_load(a, off, cnt, (c + 2) * KC, mid, mc);
+        gram_meta_load<KC>(a, off, cnt, (c + 2) * KC, mid, mc);
         __builtin_amdgcn_sched_barrier(0);   // loads stay ahead of the MFMAs
         mix_wave_chunk<MT, W>(sh, c & 1, acc, sts);
         if (more)

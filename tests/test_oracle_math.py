@@ -125,6 +125,41 @@ def test_findlocal_brute_force():
         assert np.all(np.diff(P.obs_type[ids]) >= 0)
 
 
+@pytest.mark.parametrize("nmax", [1, 9, 40])
+def test_findlocal_nlobsmax_brute_force(nmax):
+    """NLOBSMAX (observations.c:1899-1908, kdtree.c:569-570): of the
+    candidates in the type's tree within range, the nmax nearest by squared
+    distance, equal distances in visiting order (a stable sort), and only then
+    the search-time status filter."""
+    P = synth.make_problem(seed=4, nj=10, ni=12, nmem=12, nobs=1200, locrad=150.0, profiles=30)
+    status = P.obs_status.copy()
+    status[::11] = 3                  # bad at search time, in the trees
+    lim = [dict(o, nlobsmax=nmax) for o in P.obstypes]
+    ob = O.OracleObs(lim, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd, status=status,
+                     status_at_build=P.obs_status)
+    full = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd)   # visiting order, all candidates
+    xyz = np.array([O.ll2xyz(a, b) for a, b in zip(P.obs_lon.astype(np.float64), P.obs_lat.astype(np.float64))])
+    ties = 0
+    for lon, lat in zip(P.node_lon.ravel()[::5], P.node_lat.ravel()[::5]):
+        ids, lc = ob.findlocal(lon, lat)
+        cand, clc = full.findlocal(lon, lat)
+        q = np.array(O.ll2xyz(lon, lat))
+        want, want_lc = [], []
+        for t in range(len(P.obstypes)):
+            sel = P.obs_type[cand] == t
+            ct, lt = cand[sel], clc[sel]
+            d2 = ((xyz[ct] - q) ** 2).sum(1)
+            keep = np.argsort(d2, kind="stable")[:nmax]
+            ties += len(d2[keep]) - len(np.unique(d2[keep]))
+            good = status[ct[keep]] == 0
+            want += ct[keep][good].tolist()
+            want_lc += lt[keep][good].tolist()
+        assert np.array_equal(ids, np.array(want, np.int32))
+        assert np.array_equal(lc, np.array(want_lc))
+    if nmax > 1:
+        assert ties > 0, "the case must contain equal distances"
+
+
 # ---------------------------------------------------------------- per-column algebra
 def spd_inv_sqrt(M):
     lam, U = np.linalg.eigh(M)
