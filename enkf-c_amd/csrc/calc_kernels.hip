@@ -95,23 +95,27 @@ __global__ void permute_rows_kernel(const T* __restrict__ src, const int* __rest
 // (doubles) is congruent to 16 mod 32, so the four rows a k-step reads fall
 // in disjoint LDS banks.
 constexpr int KC = 32;
+// the last chunk c of a segment loads list entries up to (c + 4) KC - 1, at
+// most 4 KC - 1 past the segment's end
+static_assert(kListPad >= 4 * KC, "list padding too small for the FP64 Gram pipeline");
 template <int MT>
 struct GramLds {
     static constexpr int MP = 16 * MT;
     static constexpr int RS = MP + ((16 - MP % 32) + 32) % 32;
-    alignas(16) double rows[2][KC][RS];   // (double) S * lc of chunk c in rows[c % 2]
+    alignas(16) double rows[2][KC + 1][RS];   // (double) S * lc of chunk c in rows[c % 2]; row KC: discard
     double sl[2][KC];         // s_o * lc_o of the chunk's rows, chunk c in [c % 2]
     double lc[3][KC];         // taper coefficients, chunk c in [c % 3]
     int id[3][KC];            // obs ids (-1 past the end), chunk c in [c % 3]
 };
 
+// Threads t and t + KC hold the same entry, so every thread stores: the
+// stores are unconditional (a store under a branch leaves its load counted
+// as in flight past the branch, and the loads after it wait for it).
 template <int MT>
 __device__ inline void chunk_meta_store(GramLds<MT>& sh, int buf, int id, double c)
 {
-    if (threadIdx.x < KC) {
-        sh.id[buf][threadIdx.x] = id;
-        sh.lc[buf][threadIdx.x] = c;
-    }
+    sh.id[buf][threadIdx.x & (KC - 1)] = id;
+    sh.lc[buf][threadIdx.x & (KC - 1)] = c;
 }
 
 // rows of a chunk into row buffer rbuf, tapered in FP64 as the reference
@@ -126,15 +130,15 @@ __device__ inline void chunk_store(const CalcArgs& a, GramLds<MT>& sh, int rbuf,
         const int q = (int)threadIdx.x + 256 * j;
         const int row = q / qr;
         const int qd = q - row * qr;
-        if (row < KC) {
-            const double c = sh.lc[mbuf][row];
-            double2* dst = reinterpret_cast<double2*>(&sh.rows[rbuf][row][4 * qd]);
-            dst[0] = double2{(double)L.v[j].x * c, (double)L.v[j].y * c};
-            dst[1] = double2{(double)L.v[j].z * c, (double)L.v[j].w * c};
-        }
+        // quads past the chunk go to the discard row, unconditionally (as
+        // in chunk_meta_store)
+        const int rr = row < KC ? row : KC;
+        const double c = sh.lc[mbuf][rr < KC ? rr : 0];
+        double2* dst = reinterpret_cast<double2*>(&sh.rows[rbuf][rr][4 * qd]);
+        dst[0] = double2{(double)L.v[j].x * c, (double)L.v[j].y * c};
+        dst[1] = double2{(double)L.v[j].z * c, (double)L.v[j].w * c};
     }
-    if (threadIdx.x < KC)
-        sh.sl[rbuf][threadIdx.x] = L.s * sh.lc[mbuf][threadIdx.x];
+    sh.sl[rbuf][threadIdx.x & (KC - 1)] = L.s * sh.lc[mbuf][threadIdx.x & (KC - 1)];
 }
 
 // acc[T] += frag[I] frag[J]^T for the tiles (I, J) of wave W, with the tile
@@ -173,6 +177,36 @@ __device__ inline void gram_wave_chunk(const GramLds<MT>& sh, int buf, int mbuf,
     }
 }
 
+// One iteration of the chunk pipeline (chunk c): Lg receives the gathers of
+// chunk c+2, Ls holds chunk c+1.  The loads are unconditional: past the end
+// they read the padding of the lists and the zero row.
+template <int MT, int W>
+__device__ inline void gram_step(const CalcArgs& a, long long off, int cnt, int nchunk, int c, GramLds<MT>& sh,
+                                 GramRegs<MT, KC>& Lg, const GramRegs<MT, KC>& Ls,
+                                 dbl4 (&acc)[(tri_count(MT) - W + 3) / 4 > 0 ? (tri_count(MT) - W + 3) / 4 : 1],
+                                 double (&sts)[MT])
+{
+    int mid;
+    double mc;
+    // list entries first: the wait for them at the end of the step then
+    // leaves the younger gathers in flight
+    gram_meta_load<KC>(a, off, cnt, (c + 3) * KC, mid, mc);
+    gram_gather<MT, KC>(a, sh.id[(c + 2) % 3], Lg);
+    // keep the loads ahead of the MFMAs (the scheduler would sink them to
+    // their use, exposing their latency every chunk)
+    __builtin_amdgcn_sched_barrier(0);
+    gram_wave_chunk<MT, W>(sh, c & 1, c % 3, acc, sts);
+    // Both stores are unconditional, so that no load of this step or the
+    // previous one is in flight on any path into the next step (the wait
+    // counts are computed over all paths); past the segment's end they store
+    // zero rows and entries with id -1, taper 0, which nothing reads.  Meta
+    // buffer c % 3 (chunk c's) is free: chunk c's rows were stored in the
+    // previous step.
+    chunk_store<MT>(a, sh, (c + 1) & 1, (c + 1) % 3, Ls);
+    chunk_meta_store<MT>(sh, c % 3, mid, mc);
+    __syncthreads();
+}
+
 template <int MT, int W>
 __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, double* __restrict__ gout,
                                  double* __restrict__ stsout, GramLds<MT>& sh)
@@ -194,41 +228,41 @@ __device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, doub
         sts[X] = 0.0;
 
     // zero both row buffers once: padding columns stay zero
-    for (int q = tid; q < 2 * KC * GramLds<MT>::RS; q += 256)
+    for (int q = tid; q < 2 * (KC + 1) * GramLds<MT>::RS; q += 256)
         (&sh.rows[0][0][0])[q] = 0.0;
-    GramRegs<MT, KC> L;
     const int nchunk = (cnt + KC - 1) / KC;
     int mid;
     double mc;
-    // prologue: list entries of chunks 0 and 1, rows of chunk 0
+    // prologue: list entries of chunks 0, 1 and 2; rows of chunk 0 in LDS,
+    // gathers of chunk 1 in flight
     gram_meta_load<KC>(a, off, cnt, 0, mid, mc);
     chunk_meta_store<MT>(sh, 0, mid, mc);
     gram_meta_load<KC>(a, off, cnt, KC, mid, mc);
     chunk_meta_store<MT>(sh, 1, mid, mc);
-    chunk_meta_store<MT>(sh, 2, -1, 0.0);   // every id buffer holds valid rows or -1
+    gram_meta_load<KC>(a, off, cnt, 2 * KC, mid, mc);
+    chunk_meta_store<MT>(sh, 2, mid, mc);
     __syncthreads();
-    gram_gather<MT, KC>(a, sh.id[0], L);
-    chunk_store<MT>(a, sh, 0, 0, L);
+    GramRegs<MT, KC> L0, L1;
+    gram_gather<MT, KC>(a, sh.id[0], L0);
+    chunk_store<MT>(a, sh, 0, 0, L0);
+    gram_gather<MT, KC>(a, sh.id[1], L1);
     __syncthreads();
-    // iteration c: gathers of chunk c+1 and list entries of chunk c+2 are in
-    // flight while chunk c feeds the MFMAs; one barrier per chunk
-    for (int c = 0; c < nchunk; ++c) {
-        const bool more = c + 1 < nchunk;
-        const bool more2 = c + 2 < nchunk;
-        // the loads are unconditional: past the end they read the padding
-        // of the lists and the zero row
-        gram_gather<MT, KC>(a, sh.id[(c + 1) % 3], L);
-        gram_meta_load<KC>(a, off, cnt, (c + 2) * KC, mid, mc);
-        // keep the loads ahead of the MFMAs (the scheduler would sink them
-        // to their use, exposing their latency every chunk)
-        __builtin_amdgcn_sched_barrier(0);
-        gram_wave_chunk<MT, W>(sh, c & 1, c % 3, acc, sts);
-        if (more)
-            chunk_store<MT>(a, sh, (c + 1) & 1, (c + 1) % 3, L);
-        if (more2)
-            chunk_meta_store<MT>(sh, (c + 2) % 3, mid, mc);
-        __syncthreads();
+    // iteration c: the gathers of chunk c+2 (into one register set) and the
+    // list entries of chunk c+3 are issued, chunk c feeds the MFMAs, and
+    // chunk c+1 (gathered during iteration c-1, in the other register set)
+    // is stored; a gather thus has two chunks of MFMA work to land.  The
+    // loop is unrolled by two so that the register sets are never indexed
+    // at run time; one barrier per chunk.
+    // (the odd last chunk is peeled off, so that the loop's back edge is
+    // reached from the second step only and the wait counts at its top see
+    // no gathers into the first register set in flight)
+    int c = 0;
+    for (; c + 1 < nchunk; c += 2) {
+        gram_step<MT, W>(a, off, cnt, nchunk, c, sh, L0, L1, acc, sts);
+        gram_step<MT, W>(a, off, cnt, nchunk, c + 1, sh, L1, L0, acc, sts);
     }
+    if (c < nchunk)
+        gram_step<MT, W>(a, off, cnt, nchunk, c, sh, L0, L1, acc, sts);
     // Operand layouts of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l
     // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register
     // i of lane l holds D[4 i + l / 16][l % 16].

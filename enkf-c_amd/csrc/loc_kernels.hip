@@ -51,18 +51,19 @@ __device__ inline double calclcoeff(const TypeInfo& t, double dist)
     return sum;
 }
 
-__device__ inline double coord(const KdNode& n, int d)
-{
-    return d == 0 ? n.x : (d == 1 ? n.y : n.z);
-}
-
 // Depth-first range search over one tree; calls emit(node, distsq) for every
 // hit in the reference's visiting order.  The split dimension is the depth
-// modulo ndim, tracked on the way down and up.
-template <class Emit>
-__device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, const double* q, double range, Emit&& emit)
+// modulo ND, tracked on the way down and up.  Each step selects the next
+// node without branching (down to the near child if there is one, else to
+// the far child if the range crosses the split, else up; coming up from the
+// near child, to the far child under the same condition, else further up),
+// so that the lanes of a wave at different stages of their walks stay in
+// one instruction stream.
+template <int ND, class Emit>
+__device__ inline void range_walk_nd(const KdNode* __restrict__ tree, const double* q, double range, Emit&& emit)
 {
     const double r2 = range * range;
+    const double q0 = q[0], q1 = q[1], q2 = ND == 3 ? q[2] : 0.0;
     int node = 0;
     int from = -1;
     int dir = 0;
@@ -70,51 +71,46 @@ __device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, con
 
     while (node >= 0) {
         const KdNode nd = tree[node];
-        const double dx = q[dir] - coord(nd, dir);
-        const int left = (nd.link & 0x80000000u) ? node + 1 : -1;
-        const int near = (dx <= 0.0) ? left : nd.right;
-        const int far = (dx <= 0.0) ? nd.right : left;
-        const int parent = (int)(nd.link & 0x7fffffffu) - 1;
-        const int below = (dir + 1 == ndim) ? 0 : dir + 1;
-        const int above = (dir == 0) ? ndim - 1 : dir - 1;
+        const double cd = dir == 0 ? nd.x : ((ND == 2 || dir == 1) ? nd.y : nd.z);
+        const double qd = dir == 0 ? q0 : ((ND == 2 || dir == 1) ? q1 : q2);
+        const double dx = qd - cd;
         if (down) {
+            // kd_findnodeswithinrange's sum, in its order (kdtree.c:529-533)
             double distsq = 0.0;
-            for (int i = 0; i < ndim; i++) {
-                double d = coord(nd, i) - q[i];
+            double d = nd.x - q0;
+            distsq += d * d;
+            d = nd.y - q1;
+            distsq += d * d;
+            if (ND == 3) {
+                d = nd.z - q2;
                 distsq += d * d;
             }
             if (distsq <= r2)
                 emit(node, distsq);
-            if (near >= 0) {
-                from = node;
-                node = near;
-                dir = below;
-                continue;
-            }
-            if (fabs(dx) < range && far >= 0) {
-                from = node;
-                node = far;
-                dir = below;
-                continue;
-            }
-            from = node;
-            node = parent;
-            dir = above;
-            down = false;
-        } else {
-            // arrived from child `from`
-            if (from == near && fabs(dx) < range && far >= 0) {
-                from = node;
-                node = far;
-                dir = below;
-                down = true;
-                continue;
-            }
-            from = node;
-            node = parent;
-            dir = above;
         }
+        const int left = (nd.link & 0x80000000u) ? node + 1 : -1;
+        const bool lside = dx <= 0.0;
+        const int near = lside ? left : nd.right;
+        const int far = lside ? nd.right : left;
+        const int parent = (int)(nd.link & 0x7fffffffu) - 1;
+        const bool farok = fabs(dx) < range && far >= 0;
+        const bool tonear = down && near >= 0;
+        const bool tofar = !tonear && farok && (down || from == near);
+        const bool ndown = tonear || tofar;
+        from = node;
+        node = tonear ? near : (tofar ? far : parent);
+        dir = ndown ? (dir + 1 == ND ? 0 : dir + 1) : (dir == 0 ? ND - 1 : dir - 1);
+        down = ndown;
     }
+}
+
+template <class Emit>
+__device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, const double* q, double range, Emit&& emit)
+{
+    if (ndim == 3)
+        range_walk_nd<3>(tree, q, range, emit);
+    else
+        range_walk_nd<2>(tree, q, range, emit);
 }
 
 // Thread mapping: one thread per (node, obstype); consecutive threads take
