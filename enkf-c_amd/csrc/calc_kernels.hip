@@ -329,6 +329,259 @@ __global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
     }
 }
 
+// ------------------------------------------------------------------ Gram, independent wavefronts
+//
+// Each wavefront computes one group of upper-triangle 16 x 16 tiles (tiles
+// f = g, g + ING, ... of tri_count(MT); ING = 3 groups of 7 tiles for m =
+// 96) of one (node, obstype segment), with no workgroup barrier: the wave
+// stages its own chunks of IKC local obs through a private LDS area, so the
+// waves of one segment share nothing but L2 (the other waves' gathers of the
+// same rows hit there).  Per chunk: the gathers of chunk c+1 and the
+// list entries of chunk c+3 are in flight while chunk c feeds the MFMAs.
+// LDS writes of one lane and reads of another need no barrier: a wave
+// executes its LDS instructions in order, and the compiler keeps them in
+// order (the accesses may alias).  The taper is applied in FP64 as an
+// operand is formed, (double) S * lc, as the reference forms Sloc.
+constexpr int IKC = 8;         // obs per chunk: two k-steps
+constexpr int ING = 3;         // tile groups (wavefronts) per segment
+
+template <int MT>
+struct ItemLds {
+    static constexpr int MP = 16 * MT;
+    static constexpr int RS = MP + 16;   // floats per row: the 4 rows of a k-step fall in distinct banks
+    alignas(16) float rows[2][IKC + 1][RS];   // raw S rows of chunk c in rows[c % 2]; row IKC: discard
+    double lc[4][IKC];         // tapers, chunk c in [c % 4]
+    double sl[2][IKC];         // s * lc, chunk c in [c % 2]
+    int id[4][IKC];            // rows of S (-1 past the end), chunk c in [c % 4]
+};
+
+template <int MT>
+struct ItemRegs {
+    static constexpr int NQ = (IKC * 4 * MT + 63) / 64;
+    float v[NQ][4];            // plain floats: the registers stay promotable across iterations
+    double s;                  // s of row lane % IKC
+};
+
+// number of tiles of group G
+template <int MT, int G>
+constexpr int item_ntile()
+{
+    return (tri_count(MT) - G + ING - 1) / ING;
+}
+
+// the tiles f = G + ING t of group G: acc[t] += frag[I] frag[J]^T
+template <int MT, int G, int T>
+__device__ __attribute__((always_inline)) inline void item_tiles(const double (&frag)[MT], dbl4* acc)
+{
+    if constexpr (G + ING * T < tri_count(MT)) {
+        constexpr int f = G + ING * T;
+        constexpr int I = tri_I(MT, f), J = tri_J(MT, f);
+        acc[T] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[I], frag[J], acc[T], 0, 0, 0);
+        item_tiles<MT, G, T + 1>(frag, acc);
+    }
+}
+
+__device__ __attribute__((always_inline)) inline void item_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c)
+{
+    const int k = k0 + (int)(threadIdx.x & (IKC - 1));
+    const int i = a.ids[off + k];
+    const double l = a.lc[off + k];
+    const bool in = k < cnt;
+    id = in ? i : -1;
+    c = in ? l : 0.0;
+}
+
+template <int MT>
+__device__ __attribute__((always_inline)) inline void item_meta_store(ItemLds<MT>& sh, int slot, int id, double c)
+{
+    // lanes l and l + IKC hold the same entry: every lane stores (a store
+    // under a branch would leave its load counted as in flight past it)
+    sh.id[slot][threadIdx.x & (IKC - 1)] = id;
+    sh.lc[slot][threadIdx.x & (IKC - 1)] = c;
+}
+
+template <int MT>
+__device__ __attribute__((always_inline)) inline void item_gather(const CalcArgs& a, const ItemLds<MT>& sh, int slot, ItemRegs<MT>& L)
+{
+    constexpr int NQ = ItemRegs<MT>::NQ;
+    const int qr = a.ms >> 2;
+    // all ids first (one wait), then the loads
+    long long r[NQ];
+    int qd[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+        const int q = (int)threadIdx.x + 64 * j;
+        const int row = q / qr;
+        qd[j] = q - row * qr;
+        const int id = sh.id[slot][row < IKC ? row : IKC - 1];
+        r[j] = (row < IKC && id >= 0) ? id : a.nobs;   // the zero row
+    }
+    const int sid = sh.id[slot][threadIdx.x & (IKC - 1)];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+        const float4 q = *reinterpret_cast<const float4*>(a.S + r[j] * a.ms + 4 * qd[j]);
+        L.v[j][0] = q.x;
+        L.v[j][1] = q.y;
+        L.v[j][2] = q.z;
+        L.v[j][3] = q.w;
+    }
+    L.s = a.sv[sid >= 0 ? sid : a.nobs];
+}
+
+template <int MT>
+__device__ __attribute__((always_inline)) inline void item_stage(const CalcArgs& a, ItemLds<MT>& sh, int rbuf, int mslot, const ItemRegs<MT>& L)
+{
+    const int qr = a.ms >> 2;
+#pragma unroll
+    for (int j = 0; j < ItemRegs<MT>::NQ; ++j) {
+        const int q = (int)threadIdx.x + 64 * j;
+        const int row = q / qr;
+        const int qd = q - row * qr;
+        *reinterpret_cast<float4*>(&sh.rows[rbuf][row < IKC ? row : IKC][4 * qd]) =
+            make_float4(L.v[j][0], L.v[j][1], L.v[j][2], L.v[j][3]);
+    }
+    sh.sl[rbuf][threadIdx.x & (IKC - 1)] = L.s * sh.lc[mslot][threadIdx.x & (IKC - 1)];
+}
+
+template <int MT, int G>
+__device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& a, int b, long long off, int cnt, ItemLds<MT>& sh)
+{
+    constexpr int MP = 16 * MT;
+    constexpr int NTG = item_ntile<MT, G>();
+    const int lane = threadIdx.x & 63;
+    const int kq = lane >> 4;
+    const int col = lane & 15;
+
+    // padding columns (ms .. RS) are never written by the staging: zero once
+    for (int q = lane; q < 2 * (IKC + 1) * ItemLds<MT>::RS; q += 64)
+        (&sh.rows[0][0][0])[q] = 0.0f;
+    dbl4 acc[NTG > 0 ? NTG : 1];
+#pragma unroll
+    for (int f = 0; f < NTG; ++f)
+        acc[f] = dbl4{0.0, 0.0, 0.0, 0.0};
+    double sts[MT];
+#pragma unroll
+    for (int X = 0; X < MT; ++X)
+        sts[X] = 0.0;
+
+    const int nchunk = (cnt + IKC - 1) / IKC;
+    int mid;
+    double mc;
+    // prologue: list entries of chunks 0..2 in LDS, chunk 0 staged, chunk 1
+    // gathered into registers
+    item_meta_load(a, off, cnt, 0, mid, mc);
+    item_meta_store<MT>(sh, 0, mid, mc);
+    item_meta_load(a, off, cnt, IKC, mid, mc);
+    item_meta_store<MT>(sh, 1, mid, mc);
+    item_meta_load(a, off, cnt, 2 * IKC, mid, mc);
+    item_meta_store<MT>(sh, 2, mid, mc);
+    ItemRegs<MT> L;
+    item_gather<MT>(a, sh, 0, L);
+    item_stage<MT>(a, sh, 0, 0, L);
+    item_gather<MT>(a, sh, 1, L);
+    for (int c = 0; c < nchunk; ++c) {
+        // chunk c+3's list entries: loaded now, stored after the MFMAs
+        item_meta_load(a, off, cnt, (c + 3) * IKC, mid, mc);
+        __builtin_amdgcn_sched_barrier(0);
+        const int buf = c & 1;
+#pragma unroll
+        for (int s = 0; s < IKC / 4; ++s) {
+            const int r = 4 * s + kq;
+            const double l = sh.lc[c & 3][r];
+            double frag[MT];
+#pragma unroll
+            for (int X = 0; X < MT; ++X)
+                frag[X] = (double)sh.rows[buf][r][16 * X + col] * l;
+            item_tiles<MT, G, 0>(frag, acc);
+            if (G == 0) {
+                const double sl = sh.sl[buf][r];
+#pragma unroll
+                for (int X = 0; X < MT; ++X)
+                    sts[X] += frag[X] * sl;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // stage chunk c+1 (gathered during chunk c-1 or the prologue) and
+        // gather chunk c+2; past the end these touch the zero row only
+        item_stage<MT>(a, sh, (c + 1) & 1, (c + 1) & 3, L);
+        item_gather<MT>(a, sh, (c + 2) & 3, L);
+        item_meta_store<MT>(sh, (c + 3) & 3, mid, mc);
+    }
+    // outputs in the factor kernel's register-block layout, both triangles
+    // (accumulator register i of lane l holds D[4 i + l / 16][l % 16])
+    double* gout = a.gram + (size_t)b * MP * MP;
+#pragma unroll
+    for (int t = 0; t < NTG; ++t) {
+        const int f = G + ING * t;
+        const int I = tri_I(MT, f), J = tri_J(MT, f);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 16 * I + 4 * i + kq;
+            const int cc = 16 * J + col;
+            gout[gram_index<MT>(r, cc)] = acc[t][i];
+            if (I != J)
+                gout[gram_index<MT>(cc, r)] = acc[t][i];
+        }
+    }
+    if (G == 0) {
+        double* stsout = a.sts + (size_t)b * MP;
+#pragma unroll
+        for (int X = 0; X < MT; ++X) {
+            double v = sts[X];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (kq == 0)
+                stsout[16 * X + col] = v;
+        }
+    }
+}
+
+// wave of group g (the groups with no tile do nothing)
+template <int MT, int G>
+__device__ __attribute__((always_inline)) inline void item_dispatch(int g, const CalcArgs& a, int b, long long off,
+                                                                    int cnt, ItemLds<MT>& sh)
+{
+    if constexpr (G < ING) {
+        if (g == G) {
+            if constexpr (item_ntile<MT, G>() > 0)
+                item_wave<MT, G>(a, b, off, cnt, sh);
+        } else {
+            item_dispatch<MT, G + 1>(g, a, b, off, cnt, sh);
+        }
+    }
+}
+
+template <int MT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gram_item_kernel(CalcArgs a)
+{
+    // blocks are dispatched to the 8 XCDs round-robin; each XCD takes a
+    // contiguous range of (segment, group) items, the ING groups of a
+    // segment adjacent (neighbouring nodes and the groups of one segment
+    // share that XCD's L2)
+    const int nitem = a.nbatch * a.nseg * ING;
+    const int per = (nitem + 7) / 8;
+    const int it = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (it >= nitem)
+        return;
+    const int b = it / ING;
+    const int g = it - b * ING;
+    const int node = a.node_list ? a.node_list[a.node0 + b / a.nseg] : a.node0 + b / a.nseg;
+    const int seg = b % a.nseg;
+    long long off;
+    int cnt;
+    if (a.pertype) {
+        off = a.offsets[(size_t)node * a.ntypes + seg];
+        cnt = a.counts[(size_t)node * a.ntypes + seg];
+    } else {
+        off = a.offsets[(size_t)node * a.ntypes];
+        cnt = (int)(a.offsets[(size_t)(node + 1) * a.ntypes] - off);
+    }
+    if (cnt == 0)
+        return;
+    __shared__ ItemLds<MT> sh;
+    item_dispatch<MT, 0>(g, a, b, off, cnt, sh);
+}
+
 // ------------------------------------------------------------------ factor
 // 256 threads as a 16 x 16 grid; thread (ti, tj) owns the B x B block of rows
 // ti*B.., columns tj*B.. of the padded 16B x 16B matrix (B = ceil(m / 16));
@@ -878,6 +1131,18 @@ hipError_t launch_gram(hipStream_t s, const CalcArgs& a)
     dim3 grid((unsigned)(8 * ((a.nbatch * a.nseg + 7) / 8)));
     if (a.nbatch == 0)
         return hipSuccess;
+    if (MT <= 6) {
+        const dim3 igrid((unsigned)(8 * ((a.nbatch * a.nseg * ING + 7) / 8)));
+        switch (MT) {
+        case 1: hipLaunchKernelGGL(gram_item_kernel<1>, igrid, dim3(64), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(gram_item_kernel<2>, igrid, dim3(64), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(gram_item_kernel<3>, igrid, dim3(64), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(gram_item_kernel<4>, igrid, dim3(64), 0, s, a); break;
+        case 5: hipLaunchKernelGGL(gram_item_kernel<5>, igrid, dim3(64), 0, s, a); break;
+        default: hipLaunchKernelGGL(gram_item_kernel<6>, igrid, dim3(64), 0, s, a); break;
+        }
+        return hipGetLastError();
+    }
     switch (MT) {
     case 1: hipLaunchKernelGGL(gram_kernel<1>, grid, dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL(gram_kernel<2>, grid, dim3(256), 0, s, a); break;

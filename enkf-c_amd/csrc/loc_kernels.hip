@@ -113,20 +113,29 @@ __device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, con
         range_walk_nd<2>(tree, q, range, emit);
 }
 
-// Thread mapping: one thread per (node, obstype); consecutive threads take
-// consecutive nodes of a row of the stride grid and the same obstype, so a
-// wave walks neighbouring paths of one tree.
-__device__ inline bool thread_node(int nnode, int ntypes, int& t, int& node)
+// Thread mapping: one thread per (node, obstype); a wave takes an 8 x 8 tile
+// of the stride grid and one obstype, so its lanes walk nearly the same
+// paths of one tree and their node loads coalesce.  ni is the grid's row
+// length (nnode = ni nj); tiles are row-major within a type.
+__device__ inline bool thread_node(int nnode, int ni, int ntypes, int& t, int& node)
 {
+    const int nj = nnode / ni;
+    const int ti = (ni + 7) / 8, tj = (nj + 7) / 8;
+    const long long per_type = (long long)ti * tj * 64;
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)nnode * ntypes)
+    if (gid >= per_type * ntypes)
         return false;
-    t = (int)(gid / nnode);
-    node = (int)(gid - (long long)t * nnode);
+    t = (int)(gid / per_type);
+    const long long rem = gid - (long long)t * per_type;
+    const int tile = (int)(rem / 64), lane = (int)(rem % 64);
+    const int i = (tile % ti) * 8 + lane % 8, j = (tile / ti) * 8 + lane / 8;
+    if (i >= ni || j >= nj)
+        return false;
+    node = j * ni + i;
     return true;
 }
 
-__global__ void __launch_bounds__(64) loc_count_kernel(int nnode, const double* __restrict__ qxyz,
+__global__ void __launch_bounds__(64) loc_count_kernel(int nnode, int ni, const double* __restrict__ qxyz,
                                                         const unsigned char* __restrict__ isglobal,
                                                         const unsigned char* __restrict__ visible,
                                                         const KdNode* __restrict__ nodes,
@@ -135,7 +144,7 @@ __global__ void __launch_bounds__(64) loc_count_kernel(int nnode, const double* 
                                                         int* __restrict__ raw)
 {
     int t, node;
-    if (!thread_node(nnode, ntypes, t, node))
+    if (!thread_node(nnode, ni, ntypes, t, node))
         return;
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
@@ -161,7 +170,7 @@ __global__ void __launch_bounds__(64) loc_count_kernel(int nnode, const double* 
 
 // NLOBSMAX candidates of a (node, type): rows and squared distances in the
 // reference's visiting order
-__global__ void __launch_bounds__(64) loc_rawfill_kernel(int nnode, const double* __restrict__ qxyz,
+__global__ void __launch_bounds__(64) loc_rawfill_kernel(int nnode, int ni, const double* __restrict__ qxyz,
                                                          const unsigned char* __restrict__ isglobal,
                                                          const unsigned char* __restrict__ visible,
                                                          const KdNode* __restrict__ nodes,
@@ -170,7 +179,7 @@ __global__ void __launch_bounds__(64) loc_rawfill_kernel(int nnode, const double
                                                          double* __restrict__ rd2)
 {
     int t, node;
-    if (!thread_node(nnode, ntypes, t, node))
+    if (!thread_node(nnode, ni, ntypes, t, node))
         return;
     const TypeInfo& ti = types[t];
     if (isglobal[node] || !ti.searchable || !(visible == nullptr || visible[t]) || ti.nnode == 0 ||
@@ -261,7 +270,7 @@ __global__ void __launch_bounds__(256) loc_sort_kernel(int nnode, const TypeInfo
         counts[item] = cnt;
 }
 
-__global__ void __launch_bounds__(64) loc_fill_kernel(int nnode, const double* __restrict__ qxyz,
+__global__ void __launch_bounds__(64) loc_fill_kernel(int nnode, int ni, const double* __restrict__ qxyz,
                                                        const unsigned char* __restrict__ isglobal,
                                                        const unsigned char* __restrict__ visible,
                                                        const KdNode* __restrict__ nodes,
@@ -274,7 +283,7 @@ __global__ void __launch_bounds__(64) loc_fill_kernel(int nnode, const double* _
                                                        const double* __restrict__ sd2)
 {
     int t, node;
-    if (!thread_node(nnode, ntypes, t, node))
+    if (!thread_node(nnode, ni, ntypes, t, node))
         return;
     double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
     const bool glob = isglobal[node] != 0;
@@ -392,6 +401,13 @@ __global__ void __launch_bounds__(kScanThreads) scan_final_kernel(const int* __r
         out[n] = partials[nb];
 }
 
+// grid of the walk kernels: one 64-thread block per 8 x 8 tile and obstype
+// (thread_node)
+inline dim3 walk_grid(int ni, int nj, int ntypes)
+{
+    return dim3((unsigned)((long long)ntypes * ((ni + 7) / 8) * ((nj + 7) / 8)));
+}
+
 }  // namespace
 
 hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
@@ -401,7 +417,7 @@ hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, c
     const long long nt = (long long)ni * nj * ntypes;
     if (nt <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(loc_count_kernel, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, s, ni * nj, qxyz, isglobal,
+    hipLaunchKernelGGL(loc_count_kernel, walk_grid(ni, nj, ntypes), dim3(64), 0, s, ni * nj, ni, qxyz, isglobal,
                        visible, nodes, types, ntypes, status, counts, raw);
     return hipGetLastError();
 }
@@ -416,7 +432,7 @@ hipError_t launch_loc_nlobsmax(hipStream_t s, int ni, int nj, const double* qxyz
         return hipSuccess;
     if (nt > 0x7fffffffLL)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(loc_rawfill_kernel, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, s, ni * nj, qxyz, isglobal,
+    hipLaunchKernelGGL(loc_rawfill_kernel, walk_grid(ni, nj, ntypes), dim3(64), 0, s, ni * nj, ni, qxyz, isglobal,
                        visible, nodes, types, ntypes, rawoff, rrow, rd2);
     hipLaunchKernelGGL(loc_sort_kernel, dim3((unsigned)nt), dim3(256), 0, s, ni * nj, types, ntypes, status, raw,
                        rawoff, rrow, rd2, srow, sd2, counts, err);
@@ -431,7 +447,7 @@ hipError_t launch_loc_fill(hipStream_t s, int ni, int nj, const double* qxyz, co
     const long long nt = (long long)ni * nj * ntypes;
     if (nt <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(loc_fill_kernel, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, s, ni * nj, qxyz, isglobal,
+    hipLaunchKernelGGL(loc_fill_kernel, walk_grid(ni, nj, ntypes), dim3(64), 0, s, ni * nj, ni, qxyz, isglobal,
                        visible, nodes, types, ntypes, status, obsids, offsets, ids, lc, rawoff, raw, srow, sd2);
     return hipGetLastError();
 }
