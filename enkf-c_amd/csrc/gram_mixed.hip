@@ -135,7 +135,7 @@ __device__ inline void mix_wave_chunk(const MixLds<MT>& sh, int buf, flt4 (&acc)
                                       double (&sts)[(MT + 3) / 4])
 {
     const int lane = threadIdx.x & 63;
-    const int kg = lane >> 4;      // which 8 obs of the K = 32 step
+    const int kg = lane >> 4;      // which 4 obs of the K = 16 step
     const int col = lane & 15;
 #pragma unroll
     for (int ks = 0; ks < KC / 32; ++ks) {
@@ -278,6 +278,268 @@ __global__ void __launch_bounds__(256) gram_mixed_kernel(CalcArgs a)
     }
 }
 
+// ------------------------------------------------------------------ one wavefront per work item
+//
+// For m <= 96: independent wavefronts, no workgroup barrier.  A wavefront
+// computes one of MNG = 2 groups of the tri_count(MT) tiles (f = g, g + 2,
+// ...; 11 and 10 tiles for m = 96, FP32 accumulators) of one (node, obstype
+// segment); both waves of a segment stage the whole chunk (the second finds
+// the rows in L2).  Chunks of MKC = 16 local obs (one K = 16 MFMA step,
+// v_mfma_f32_16x16x16_bf16, which halves the gathered registers in flight
+// against K = 32) are gathered into registers one chunk ahead -- item
+// u = lane + 64 j is member u % MP of obs rows 4 (u / MP) .. + 3 -- and
+// staged member-major as BF16 hi
+// and lo planes in a private LDS area after the MFMAs of the previous chunk
+// (the wave's own program order keeps the two apart).  The taper is applied
+// in FP32 (x = S * (float) lc; the split that follows already carries about
+// 2^-17 relative error); S^T s is summed in FP32 over a chunk and in FP64
+// over chunks.
+constexpr int MKC = 16;
+constexpr int MNG = 2;         // tile groups (wavefronts) per segment
+constexpr int MKR = MKC + 8;   // bf16 per member row (48 bytes: 8-byte aligned, staggered banks)
+typedef short sh4 __attribute__((ext_vector_type(4)));
+
+template <int MT>
+struct MixItemLds {
+    static constexpr int MP = 16 * MT;
+    alignas(16) __bf16 hi[MP][MKR];
+    alignas(16) __bf16 lo[MP][MKR];
+    float sl[MKC];             // s * lc of the staged chunk
+    double lc[4][MKC];         // tapers, chunk c in [c % 4]
+    int id[4][MKC];            // rows of S (-1 past the end), chunk c in [c % 4]
+};
+
+template <int MT>
+struct MixItemRegs {
+    float v[MT][4];            // items j: 4 obs rows of one member
+    double s;                  // s of row lane % MKC
+};
+
+template <int MT, int G>
+constexpr int mix_item_ntile()
+{
+    return (tri_count(MT) - G + MNG - 1) / MNG;
+}
+
+// the tiles f = G + MNG t of group G
+template <int MT, int G, int T>
+__device__ __attribute__((always_inline)) inline void mix_item_tiles(const sh4 (&h)[MT], const sh4 (&l)[MT],
+                                                                     flt4* acc)
+{
+    if constexpr (G + MNG * T < tri_count(MT)) {
+        constexpr int f = G + MNG * T;
+        constexpr int I = tri_I(MT, f), J = tri_J(MT, f);
+        acc[T] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[I], h[J], acc[T], 0, 0, 0);
+        acc[T] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[I], l[J], acc[T], 0, 0, 0);
+        acc[T] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(l[I], h[J], acc[T], 0, 0, 0);
+        mix_item_tiles<MT, G, T + 1>(h, l, acc);
+    }
+}
+
+__device__ __attribute__((always_inline)) inline void mix_item_meta_load(const CalcArgs& a, long long off, int cnt,
+                                                                         int k0, int& id, double& c)
+{
+    const int k = k0 + (int)(threadIdx.x & (MKC - 1));
+    const int i = a.ids[off + k];
+    const double l = a.lc[off + k];
+    const bool in = k < cnt;
+    id = in ? i : -1;
+    c = in ? l : 0.0;
+}
+
+template <int MT>
+__device__ __attribute__((always_inline)) inline void mix_item_meta_store(MixItemLds<MT>& sh, int slot, int id,
+                                                                          double c)
+{
+    // lanes l and l + MKC hold the same entry: every lane stores
+    sh.id[slot][threadIdx.x & (MKC - 1)] = id;
+    sh.lc[slot][threadIdx.x & (MKC - 1)] = c;
+}
+
+template <int MT>
+__device__ __attribute__((always_inline)) inline void mix_item_gather(const CalcArgs& a, const MixItemLds<MT>& sh,
+                                                                      int slot, MixItemRegs<MT>& L)
+{
+    constexpr int MP = 16 * MT;
+    // byte offsets from S in 32 bits (S is below 4 GiB for this kernel,
+    // launch_gram_mixed): one base register for all 8 MT loads
+    const char* base = reinterpret_cast<const char*>(a.S);
+    static_assert(MKC == 16, "4 row groups of 4 obs per chunk");
+#pragma unroll
+    for (int j = 0; j < MT; ++j) {
+        const int u = (int)threadIdx.x + 64 * j;
+        const int rg = u / MP;               // < MKC / 4 since 64 MT = 4 MP
+        const int e = min(u - rg * MP, a.m - 1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int id = sh.id[slot][4 * rg + k];
+            const unsigned row = (unsigned)(id >= 0 ? id : a.nobs);
+            const unsigned boff = (row * (unsigned)a.ms + (unsigned)e) * 4u;
+            L.v[j][k] = *reinterpret_cast<const float*>(base + boff);
+        }
+    }
+    const int id = sh.id[slot][threadIdx.x & (MKC - 1)];
+    L.s = a.sv[id >= 0 ? id : a.nobs];
+}
+
+template <int MT>
+__device__ __attribute__((always_inline)) inline void mix_item_stage(const CalcArgs& a, MixItemLds<MT>& sh, int mslot,
+                                                                     const MixItemRegs<MT>& L)
+{
+    constexpr int MP = 16 * MT;
+#pragma unroll
+    for (int j = 0; j < MT; ++j) {
+        const int u = (int)threadIdx.x + 64 * j;
+        const int rg = u / MP;
+        const int e = u - rg * MP;
+        const bool real = e < a.m;
+        bf16x4 h, l;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float x = real ? L.v[j][k] * (float)sh.lc[mslot][4 * rg + k] : 0.0f;
+            h[k] = (__bf16)x;
+            l[k] = (__bf16)(x - (float)h[k]);
+        }
+        *reinterpret_cast<bf16x4*>(&sh.hi[e][4 * rg]) = h;
+        *reinterpret_cast<bf16x4*>(&sh.lo[e][4 * rg]) = l;
+    }
+    sh.sl[threadIdx.x & (MKC - 1)] = (float)(L.s * sh.lc[mslot][threadIdx.x & (MKC - 1)]);
+}
+
+template <int MT, int G>
+__device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcArgs& a, int b, long long off, int cnt,
+                                                                    MixItemLds<MT>& sh)
+{
+    constexpr int MP = 16 * MT;
+    constexpr int NTG = mix_item_ntile<MT, G>();
+    const int lane = threadIdx.x & 63;
+    const int kg = lane >> 4;      // which 8 obs of the K = 32 step
+    const int col = lane & 15;
+
+    flt4 acc[NTG > 0 ? NTG : 1];
+#pragma unroll
+    for (int f = 0; f < NTG; ++f)
+        acc[f] = flt4{0.0f, 0.0f, 0.0f, 0.0f};
+    double sts[MT];
+#pragma unroll
+    for (int X = 0; X < MT; ++X)
+        sts[X] = 0.0;
+
+    const int nchunk = (cnt + MKC - 1) / MKC;
+    int mid;
+    double mc;
+    mix_item_meta_load(a, off, cnt, 0, mid, mc);
+    mix_item_meta_store<MT>(sh, 0, mid, mc);
+    mix_item_meta_load(a, off, cnt, MKC, mid, mc);
+    mix_item_meta_store<MT>(sh, 1, mid, mc);
+    mix_item_meta_load(a, off, cnt, 2 * MKC, mid, mc);
+    mix_item_meta_store<MT>(sh, 2, mid, mc);
+    MixItemRegs<MT> L;
+    mix_item_gather<MT>(a, sh, 0, L);
+    mix_item_stage<MT>(a, sh, 0, L);
+    mix_item_gather<MT>(a, sh, 1, L);
+    const int k0 = 4 * kg;
+    for (int c = 0; c < nchunk; ++c) {
+        mix_item_meta_load(a, off, cnt, (c + 3) * MKC, mid, mc);
+        __builtin_amdgcn_sched_barrier(0);
+        // operand of v_mfma_f32_16x16x16_bf16: lane l supplies 4 consecutive
+        // obs (k = 4 (l / 16) ..) of member row l % 16 of a tile
+        sh4 h[MT], l[MT];
+#pragma unroll
+        for (int X = 0; X < MT; ++X) {
+            h[X] = *reinterpret_cast<const sh4*>(&sh.hi[16 * X + col][k0]);
+            l[X] = *reinterpret_cast<const sh4*>(&sh.lo[16 * X + col][k0]);
+        }
+        mix_item_tiles<MT, G, 0>(h, l, acc);
+        if (G == 0) {
+            float slv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                slv[q] = sh.sl[k0 + q];
+#pragma unroll
+            for (int X = 0; X < MT; ++X) {
+                const bf16x4 hb = __builtin_bit_cast(bf16x4, h[X]);
+                const bf16x4 lb = __builtin_bit_cast(bf16x4, l[X]);
+                float t = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    t = fmaf((float)hb[q] + (float)lb[q], slv[q], t);
+                sts[X] += (double)t;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // stage chunk c+1 over chunk c (read above, in this wave's order),
+        // gather chunk c+2; past the end these touch the zero row only
+        mix_item_stage<MT>(a, sh, (c + 1) & 3, L);
+        mix_item_gather<MT>(a, sh, (c + 2) & 3, L);
+        mix_item_meta_store<MT>(sh, (c + 3) & 3, mid, mc);
+    }
+    // v_mfma_f32_16x16x16_bf16 result: register i of lane l holds
+    // D[4 (l / 16) + i][l % 16]
+    double* gout = a.gram + (size_t)b * MP * MP;
+#pragma unroll
+    for (int t = 0; t < NTG; ++t) {
+        const int f = G + MNG * t;
+        const int I = tri_I(MT, f), J = tri_J(MT, f);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 16 * I + 4 * kg + i;
+            const int cc = 16 * J + col;
+            const double v = (double)acc[t][i];
+            gout[gram_index<MT>(r, cc)] = v;
+            if (I != J)
+                gout[gram_index<MT>(cc, r)] = v;
+        }
+    }
+    if (G == 0) {
+        // the four lanes of a member (kg = 0..3) hold partial sums over
+        // their 4 obs of each chunk
+        double* stsout = a.sts + (size_t)b * MP;
+#pragma unroll
+        for (int X = 0; X < MT; ++X) {
+            double v = sts[X];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (kg == 0)
+                stsout[16 * X + col] = v;
+        }
+    }
+}
+
+template <int MT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gram_mixed_item_kernel(CalcArgs a)
+{
+    // blocks are dispatched to the 8 XCDs round-robin; each XCD takes a
+    // contiguous range of (segment, group) items
+    const int nitem = a.nbatch * a.nseg * MNG;
+    const int per = (nitem + 7) / 8;
+    const int it = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (it >= nitem)
+        return;
+    const int b = it / MNG;
+    const int g = it - b * MNG;
+    const int node = a.node_list ? a.node_list[a.node0 + b / a.nseg] : a.node0 + b / a.nseg;
+    const int seg = b % a.nseg;
+    long long off;
+    int cnt;
+    if (a.pertype) {
+        off = a.offsets[(size_t)node * a.ntypes + seg];
+        cnt = a.counts[(size_t)node * a.ntypes + seg];
+    } else {
+        off = a.offsets[(size_t)node * a.ntypes];
+        cnt = (int)(a.offsets[(size_t)(node + 1) * a.ntypes] - off);
+    }
+    if (cnt == 0)
+        return;
+    __shared__ MixItemLds<MT> sh;
+    if (g == 0) {
+        mix_item_wave<MT, 0>(a, b, off, cnt, sh);
+    } else {
+        if constexpr (mix_item_ntile<MT, 1>() > 0)
+            mix_item_wave<MT, 1>(a, b, off, cnt, sh);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
@@ -286,6 +548,20 @@ hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
     dim3 grid((unsigned)(8 * ((a.nbatch * a.nseg + 7) / 8)));
     if (a.nbatch == 0)
         return hipSuccess;
+    // the item kernel addresses S with 32-bit byte offsets
+    const bool small_s = (unsigned long long)(a.nobs + 1) * (unsigned long long)a.ms * 4ull < (1ull << 32);
+    if (MT <= 6 && small_s) {
+        const dim3 igrid((unsigned)(8 * ((a.nbatch * a.nseg * MNG + 7) / 8)));
+        switch (MT) {
+        case 1: hipLaunchKernelGGL(gram_mixed_item_kernel<1>, igrid, dim3(64), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(gram_mixed_item_kernel<2>, igrid, dim3(64), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(gram_mixed_item_kernel<3>, igrid, dim3(64), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(gram_mixed_item_kernel<4>, igrid, dim3(64), 0, s, a); break;
+        case 5: hipLaunchKernelGGL(gram_mixed_item_kernel<5>, igrid, dim3(64), 0, s, a); break;
+        default: hipLaunchKernelGGL(gram_mixed_item_kernel<6>, igrid, dim3(64), 0, s, a); break;
+        }
+        return hipGetLastError();
+    }
     switch (MT) {
     case 1: hipLaunchKernelGGL(gram_mixed_kernel<1>, grid, dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL(gram_mixed_kernel<2>, grid, dim3(256), 0, s, a); break;
