@@ -106,6 +106,10 @@ struct enkfg_ctx {
     int nnode = 0;
     DevBuf d_qxyz, d_glob, d_vis, d_counts, d_offsets, d_scantmp, d_ids, d_lc;
     DevBuf d_raw, d_rawoff, d_rrow, d_rd2, d_srow, d_sd2;   // NLOBSMAX candidates
+    // one-walk localisation: chunk pool (capacity from the last two-walk
+    // localisation), list heads, pool counter and overflow flag
+    DevBuf d_heads, d_cids, d_clc, d_cnext, d_lflag;
+    long long pool_cap = 0;
     bool lists_valid = false;
 
     // transforms and diagnostics
@@ -312,13 +316,49 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
     if (nlobsmax)
         HIPCHECK(c, c->d_raw.reserve(std::max<long long>(ncnt, 1) * sizeof(int)));
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHECK(c, launch_loc_count(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
-                                 c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(),
-                                 c->d_counts.as<int>(), nlobsmax ? c->d_raw.as<int>() : nullptr));
     size_t tmpb = 0;
     HIPCHECK(c, launch_scan_counts(c->stream, nullptr, nullptr, ncnt, nullptr, &tmpb));
     HIPCHECK(c, c->d_scantmp.reserve(tmpb));
     tmpb = c->d_scantmp.bytes;
+    if (!nlobsmax && c->pool_cap > 0) {
+        // one walk into the chunk pool, then the compaction; a pool too
+        // small for these lists falls back to the two walks below
+        HIPCHECK(c, c->d_heads.reserve(std::max<long long>(ncnt, 1) * sizeof(int)));
+        HIPCHECK(c, c->d_cids.reserve((size_t)c->pool_cap * kLocChunk * sizeof(int)));
+        HIPCHECK(c, c->d_clc.reserve((size_t)c->pool_cap * kLocChunk * sizeof(double)));
+        HIPCHECK(c, c->d_cnext.reserve((size_t)c->pool_cap * sizeof(int)));
+        HIPCHECK(c, c->d_lflag.reserve(2 * sizeof(int)));
+        HIPCHECK(c, hipMemsetAsync(c->d_lflag.p, 0, 2 * sizeof(int), c->stream));
+        HIPCHECK(c, launch_loc_walk(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(),
+                                    vis, c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt,
+                                    c->d_status.as<short>(), c->d_counts.as<int>(), c->d_heads.as<int>(),
+                                    c->d_cids.as<int>(), c->d_clc.as<double>(), c->d_cnext.as<int>(),
+                                    c->d_lflag.as<int>(), (int)c->pool_cap, c->d_lflag.as<int>() + 1));
+        HIPCHECK(c, launch_scan_counts(c->stream, c->d_counts.as<int>(), c->d_offsets.as<long long>(), ncnt,
+                                       c->d_scantmp.p, &tmpb));
+        long long total = 0;
+        int flags[2] = {0, 0};
+        HIPCHECK(c, hipMemcpyAsync(&total, c->d_offsets.as<long long>() + ncnt, sizeof total, hipMemcpyDeviceToHost,
+                                   c->stream));
+        HIPCHECK(c, hipMemcpyAsync(flags, c->d_lflag.p, sizeof flags, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        if (flags[1] == 0) {
+            HIPCHECK(c, c->d_ids.reserve((total + kListPad) * sizeof(int)));
+            HIPCHECK(c, c->d_lc.reserve((total + kListPad) * sizeof(double)));
+            HIPCHECK(c, launch_loc_compact(c->stream, ncnt, nt, c->d_types.as<TypeInfo>(), c->d_obsids.as<int>(),
+                                           c->d_counts.as<int>(), c->d_heads.as<int>(),
+                                           c->d_offsets.as<long long>(), c->d_cids.as<int>(), c->d_clc.as<double>(),
+                                           c->d_cnext.as<int>(), c->d_ids.as<int>(), c->d_lc.as<double>()));
+            HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+            HIPCHECK(c, hipStreamSynchronize(c->stream));
+            c->timings[0] = elapsed(c->ev[0], c->ev[1]);
+            c->lists_valid = true;
+            return ENKFG_OK;
+        }
+    }
+    HIPCHECK(c, launch_loc_count(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(), c->d_glob.as<unsigned char>(), vis,
+                                 c->d_nodes.as<KdNode>(), c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(),
+                                 c->d_counts.as<int>(), nlobsmax ? c->d_raw.as<int>() : nullptr));
     if (nlobsmax) {
         // NLOBSMAX types: candidates in visiting order, sorted by distance,
         // the first NLOBSMAX with status OK kept (observations.c:1899-1908)
@@ -366,6 +406,18 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->timings[0] = elapsed(c->ev[0], c->ev[1]);
     c->lists_valid = true;
+    // pool for the next localisation's single walk: the chunks these lists
+    // take, and a margin
+    if (!nlobsmax) {
+        std::vector<int> cnt((size_t)ncnt);
+        if (ncnt > 0)
+            HIPCHECK(c, hipMemcpy(cnt.data(), c->d_counts.p, (size_t)ncnt * sizeof(int), hipMemcpyDeviceToHost));
+        long long need = 0;
+        for (int n : cnt)
+            need += (n + kLocChunk - 1) / kLocChunk;
+        need += need / 64 + 64;
+        c->pool_cap = (need < 0x7fffffffLL) ? need : 0;   // 0: stay with the two walks
+    }
     return ENKFG_OK;
 }
 
@@ -430,7 +482,8 @@ void enkfg_destroy(enkfg_ctx* c)
         (void)hipStreamSynchronize(c->stream2);
     DevBuf* bufs[] = {&c->d_nodes, &c->d_types, &c->d_obsids, &c->d_type, &c->d_status, &c->d_estd, &c->d_sqrtrf,
                       &c->d_perm, &c->d_S, &c->d_s, &c->d_tmp, &c->d_tmp2, &c->d_qxyz, &c->d_glob, &c->d_vis, &c->d_counts, &c->d_offsets,
-                      &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_raw, &c->d_rawoff, &c->d_rrow, &c->d_rd2,
+                      &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_heads, &c->d_cids, &c->d_clc, &c->d_cnext, &c->d_lflag,
+                      &c->d_raw, &c->d_rawoff, &c->d_rrow, &c->d_rd2,
                       &c->d_srow, &c->d_sd2, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
                       &c->d_pdfs, &c->d_psrf, &c->d_gram, &c->d_sts, &c->d_err, &c->d_E, &c->d_infl, &c->d_nlev,
                       &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio, &c->d_adjg, &c->d_adjflag,

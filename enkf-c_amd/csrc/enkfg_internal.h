@@ -51,6 +51,16 @@ hipError_t launch_loc_nlobsmax(hipStream_t s, int ni, int nj, const double* qxyz
                                const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                                const short* status, const int* raw, const long long* rawoff, int* rrow, double* rd2,
                                int* srow, double* sd2, int* counts, int* err);
+// one-walk localisation (loc_kernels.hip): lists appended to a pool of
+// chunks of kLocChunk entries, then compacted into the CSR arrays
+constexpr int kLocChunk = 512;
+hipError_t launch_loc_walk(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
+                           const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
+                           const short* status, int* counts, int* heads, int* cids, double* clc, int* cnext, int* pool,
+                           int pool_cap, int* overflow);
+hipError_t launch_loc_compact(hipStream_t s, long long nitem, int ntypes, const TypeInfo* types, const int* obsids,
+                              const int* counts, const int* heads, const long long* offsets, const int* cids,
+                              const double* clc, const int* cnext, int* ids, double* lc);
 hipError_t launch_loc_fill(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                            const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                            const short* status, const int* obsids, const long long* offsets, int* ids,

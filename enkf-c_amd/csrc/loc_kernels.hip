@@ -401,6 +401,113 @@ __global__ void __launch_bounds__(kScanThreads) scan_final_kernel(const int* __r
         out[n] = partials[nb];
 }
 
+// ---------------------------------------------------------------- one walk
+//
+// Counting the lists before filling them walks every tree twice.  With the
+// capacity known from an earlier localisation, one walk suffices: a (node,
+// type) thread appends its entries to chunks of kLocChunk taken from a pool
+// by an atomic counter, linked in order (next[chunk]), and records its count
+// and first chunk; after the scan of the counts, loc_compact_kernel copies
+// each list into its place in the CSR arrays.  A walk that runs out of pool
+// raises the overflow flag; the host then redoes the localisation with the
+// two walks.  Types with NLOBSMAX take the two-walk path.
+__global__ void __launch_bounds__(64) loc_walk_kernel(int nnode, int ni, const double* __restrict__ qxyz,
+                                                       const unsigned char* __restrict__ isglobal,
+                                                       const unsigned char* __restrict__ visible,
+                                                       const KdNode* __restrict__ nodes,
+                                                       const TypeInfo* __restrict__ types, int ntypes,
+                                                       const short* __restrict__ status, int* __restrict__ counts,
+                                                       int* __restrict__ heads, int* __restrict__ cids,
+                                                       double* __restrict__ clc, int* __restrict__ cnext,
+                                                       int* __restrict__ pool, int pool_cap, int* __restrict__ overflow)
+{
+    int t, node;
+    if (!thread_node(nnode, ni, ntypes, t, node))
+        return;
+    const size_t item = (size_t)node * ntypes + t;
+    double q[3] = {qxyz[3 * (size_t)node], qxyz[3 * (size_t)node + 1], qxyz[3 * (size_t)node + 2]};
+    const bool glob = isglobal[node] != 0;
+    const TypeInfo& ti = types[t];
+    int n = 0;
+    int head = -1;
+    if (ti.searchable && (visible == nullptr || visible[t])) {
+        if (glob) {
+            n = ti.nids;   // obs_getglobal: all obs of the type, no status check (written by the compaction)
+            head = -2;
+        } else if (ti.nnode > 0) {
+            int cur = -1, pos = kLocChunk;
+            bool full = false;
+            range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int v, double distsq) {
+                const int row = ti.node0 + v;   // device row of the node's observation
+                if (status[row] != kStatusOK)
+                    return;
+                if (pos == kLocChunk && !full) {
+                    const int nc = atomicAdd(pool, 1);
+                    if (nc >= pool_cap) {
+                        full = true;
+                    } else {
+                        if (cur >= 0)
+                            cnext[cur] = nc;
+                        else
+                            head = nc;
+                        cur = nc;
+                        pos = 0;
+                    }
+                }
+                if (!full) {
+                    cids[(size_t)cur * kLocChunk + pos] = row;
+                    clc[(size_t)cur * kLocChunk + pos] = calclcoeff(ti, sqrt(distsq));
+                    pos++;
+                }
+                n++;
+            });
+            if (cur >= 0)
+                cnext[cur] = -1;
+            if (full)
+                atomicExch(overflow, 1);
+        }
+    }
+    counts[item] = n;
+    heads[item] = head;
+}
+
+// one wavefront per (node, type): the chunks of a list copied to their
+// place in the CSR arrays (lanes along the chunk: coalesced)
+__global__ void __launch_bounds__(256) loc_compact_kernel(long long nitem, int ntypes,
+                                                           const TypeInfo* __restrict__ types,
+                                                           const int* __restrict__ obsids,
+                                                           const int* __restrict__ counts,
+                                                           const int* __restrict__ heads,
+                                                           const long long* __restrict__ offsets,
+                                                           const int* __restrict__ cids, const double* __restrict__ clc,
+                                                           const int* __restrict__ cnext, int* __restrict__ ids,
+                                                           double* __restrict__ lc)
+{
+    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= nitem)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int n = counts[item];
+    const long long o = offsets[item];
+    int cur = heads[item];
+    if (cur == -2) {
+        const TypeInfo& ti = types[item % ntypes];
+        for (int i = lane; i < n; i += 64) {
+            ids[o + i] = obsids[ti.obsid0 + i];
+            lc[o + i] = 1.0;
+        }
+        return;
+    }
+    for (int k0 = 0; k0 < n && cur >= 0; k0 += kLocChunk) {
+        const int len = min(kLocChunk, n - k0);
+        for (int i = lane; i < len; i += 64) {
+            ids[o + k0 + i] = cids[(size_t)cur * kLocChunk + i];
+            lc[o + k0 + i] = clc[(size_t)cur * kLocChunk + i];
+        }
+        cur = cnext[cur];
+    }
+}
+
 // grid of the walk kernels: one 64-thread block per 8 x 8 tile and obstype
 // (thread_node)
 inline dim3 walk_grid(int ni, int nj, int ntypes)
@@ -475,6 +582,31 @@ hipError_t launch_scan_counts(hipStream_t s, const int* counts, long long* offse
     hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanThreads), 0, s, partials, (int)nb);
     hipLaunchKernelGGL(scan_final_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, counts, n, partials, offsets,
                        (int)nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_loc_walk(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
+                           const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
+                           const short* status, int* counts, int* heads, int* cids, double* clc, int* cnext, int* pool,
+                           int pool_cap, int* overflow)
+{
+    const long long nt = (long long)ni * nj * ntypes;
+    if (nt <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(loc_walk_kernel, walk_grid(ni, nj, ntypes), dim3(64), 0, s, ni * nj, ni, qxyz, isglobal,
+                       visible, nodes, types, ntypes, status, counts, heads, cids, clc, cnext, pool, pool_cap,
+                       overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_loc_compact(hipStream_t s, long long nitem, int ntypes, const TypeInfo* types, const int* obsids,
+                              const int* counts, const int* heads, const long long* offsets, const int* cids,
+                              const double* clc, const int* cnext, int* ids, double* lc)
+{
+    if (nitem <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(loc_compact_kernel, dim3((unsigned)((nitem + 3) / 4)), dim3(256), 0, s, nitem, ntypes, types,
+                       obsids, counts, heads, offsets, cids, clc, cnext, ids, lc);
     return hipGetLastError();
 }
 

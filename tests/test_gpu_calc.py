@@ -75,6 +75,38 @@ def test_localisation_status_visibility_global():
     E.close()
 
 
+def test_localisation_one_walk_and_pool_overflow():
+    """Localisations after the first walk the trees once, into a pool of
+    chunks sized by the previous lists, and compact them; a pool too small
+    for the new lists falls back to counting first.  Three localisations of
+    one engine: type 1 invisible (two walks), all visible with a global node
+    (the pool overflows: two walks again), and the same again (one walk,
+    with the global node's list written by the compaction) -- each against
+    the oracle, bit for bit."""
+    print("case localisation one walk / pool overflow", flush=True)
+    P = problem(seed=7, profiles=30, nobs=3000, locrad=90.0)
+    rng = np.random.default_rng(11)
+    status = P.obs_status.copy()
+    status[rng.random(P.nobs) < 0.05] = 7
+    E = engine_for(P, enkfg.MODE_ENKF, enkfg.SCHEME_DENKF)
+    E.set_obs_status(status)
+    ob = oracle_obs(P, status=status, status_build=P.obs_status)
+    lon = P.node_lon.copy()
+    lat = P.node_lat.copy()
+    lon[1, 2] = np.nan                              # a node without coordinates -> all obs of each type
+    for vis in (np.array([1, 0], np.uint8), np.array([1, 1], np.uint8), np.array([1, 1], np.uint8)):
+        E.localise(lon, lat, stride=P.stride, visible=vis)
+        counts = E.local_counts()
+        assert counts.sum() > 0
+        for k in range(lon.size):
+            ids_o, lc_o = ob.findlocal(lon.flat[k], lat.flat[k], visible=vis)
+            assert counts[k] == len(ids_o), "node %d count" % k
+            ids_g, lc_g = E.localobs(k, counts[k])
+            assert np.array_equal(ids_g[:counts[k]], ids_o), "node %d ids" % k
+            assert np.array_equal(lc_g[:counts[k]].view(np.uint64), lc_o.view(np.uint64))
+    E.close()
+
+
 @pytest.mark.parametrize("nmax", [1, 17, 60])
 def test_localisation_nlobsmax(nmax):
     """NLOBSMAX (observations.c:1899-1908): candidates sorted by distance,
