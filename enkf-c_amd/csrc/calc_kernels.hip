@@ -332,8 +332,10 @@ __global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
 // ------------------------------------------------------------------ Gram, independent wavefronts
 //
 // Each wavefront computes one group of upper-triangle 16 x 16 tiles (tiles
-// f = g, g + ING, ... of tri_count(MT); ING = 3 groups of 7 tiles for m =
-// 96) of one (node, obstype segment), with no workgroup barrier: the wave
+// f = g, g + NG, ... of tri_count(MT); NG = 3 groups of 7 tiles or NG = 2
+// groups of 11 and 10 for m = 96, chosen per obs-count bucket by the
+// host's autotuner) of one (node, obstype segment), with no workgroup
+// barrier: the wave
 // stages its own chunks of IKC local obs through a private LDS area, so the
 // waves of one segment share nothing but L2 (the other waves' gathers of the
 // same rows hit there).  Per chunk: the gathers of chunk c+1 and the
@@ -343,7 +345,6 @@ __global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
 // order (the accesses may alias).  The taper is applied in FP64 as an
 // operand is formed, (double) S * lc, as the reference forms Sloc.
 constexpr int IKC = 8;         // obs per chunk: two k-steps
-constexpr int ING = 3;         // tile groups (wavefronts) per segment
 
 template <int MT>
 struct ItemLds {
@@ -362,22 +363,22 @@ struct ItemRegs {
     double s;                  // s of row lane % IKC
 };
 
-// number of tiles of group G
-template <int MT, int G>
+// number of tiles of group G of NG
+template <int MT, int NG, int G>
 constexpr int item_ntile()
 {
-    return (tri_count(MT) - G + ING - 1) / ING;
+    return (tri_count(MT) - G + NG - 1) / NG;
 }
 
-// the tiles f = G + ING t of group G: acc[t] += frag[I] frag[J]^T
-template <int MT, int G, int T>
+// the tiles f = G + NG t of group G: acc[t] += frag[I] frag[J]^T
+template <int MT, int NG, int G, int T>
 __device__ __attribute__((always_inline)) inline void item_tiles(const double (&frag)[MT], dbl4* acc)
 {
-    if constexpr (G + ING * T < tri_count(MT)) {
-        constexpr int f = G + ING * T;
+    if constexpr (G + NG * T < tri_count(MT)) {
+        constexpr int f = G + NG * T;
         constexpr int I = tri_I(MT, f), J = tri_J(MT, f);
         acc[T] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[I], frag[J], acc[T], 0, 0, 0);
-        item_tiles<MT, G, T + 1>(frag, acc);
+        item_tiles<MT, NG, G, T + 1>(frag, acc);
     }
 }
 
@@ -443,11 +444,11 @@ __device__ __attribute__((always_inline)) inline void item_stage(const CalcArgs&
     sh.sl[rbuf][threadIdx.x & (IKC - 1)] = L.s * sh.lc[mslot][threadIdx.x & (IKC - 1)];
 }
 
-template <int MT, int G>
+template <int MT, int NG, int G>
 __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& a, int b, long long off, int cnt, ItemLds<MT>& sh)
 {
     constexpr int MP = 16 * MT;
-    constexpr int NTG = item_ntile<MT, G>();
+    constexpr int NTG = item_ntile<MT, NG, G>();
     const int lane = threadIdx.x & 63;
     const int kq = lane >> 4;
     const int col = lane & 15;
@@ -492,7 +493,7 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
 #pragma unroll
             for (int X = 0; X < MT; ++X)
                 frag[X] = (double)sh.rows[buf][r][16 * X + col] * l;
-            item_tiles<MT, G, 0>(frag, acc);
+            item_tiles<MT, NG, G, 0>(frag, acc);
             if (G == 0) {
                 const double sl = sh.sl[buf][r];
 #pragma unroll
@@ -512,7 +513,7 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
     double* gout = a.gram + (size_t)b * MP * MP;
 #pragma unroll
     for (int t = 0; t < NTG; ++t) {
-        const int f = G + ING * t;
+        const int f = G + NG * t;
         const int I = tri_I(MT, f), J = tri_J(MT, f);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -537,34 +538,26 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
 }
 
 // wave of group g (the groups with no tile do nothing)
-template <int MT, int G>
+template <int MT, int NG, int G>
 __device__ __attribute__((always_inline)) inline void item_dispatch(int g, const CalcArgs& a, int b, long long off,
                                                                     int cnt, ItemLds<MT>& sh)
 {
-    if constexpr (G < ING) {
+    if constexpr (G < NG) {
         if (g == G) {
-            if constexpr (item_ntile<MT, G>() > 0)
-                item_wave<MT, G>(a, b, off, cnt, sh);
+            if constexpr (item_ntile<MT, NG, G>() > 0)
+                item_wave<MT, NG, G>(a, b, off, cnt, sh);
         } else {
-            item_dispatch<MT, G + 1>(g, a, b, off, cnt, sh);
+            item_dispatch<MT, NG, G + 1>(g, a, b, off, cnt, sh);
         }
     }
 }
 
-template <int MT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gram_item_kernel(CalcArgs a)
+// item `it` = (segment b, group g) of the batch
+template <int MT, int NG>
+__device__ __attribute__((always_inline)) inline void item_run(const CalcArgs& a, int it, ItemLds<MT>& sh)
 {
-    // blocks are dispatched to the 8 XCDs round-robin; each XCD takes a
-    // contiguous range of (segment, group) items, the ING groups of a
-    // segment adjacent (neighbouring nodes and the groups of one segment
-    // share that XCD's L2)
-    const int nitem = a.nbatch * a.nseg * ING;
-    const int per = (nitem + 7) / 8;
-    const int it = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (it >= nitem)
-        return;
-    const int b = it / ING;
-    const int g = it - b * ING;
+    const int b = it / NG;
+    const int g = it - b * NG;
     const int node = a.node_list ? a.node_list[a.node0 + b / a.nseg] : a.node0 + b / a.nseg;
     const int seg = b % a.nseg;
     long long off;
@@ -576,10 +569,24 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gr
         off = a.offsets[(size_t)node * a.ntypes];
         cnt = (int)(a.offsets[(size_t)(node + 1) * a.ntypes] - off);
     }
-    if (cnt == 0)
-        return;
+    if (cnt > 0)
+        item_dispatch<MT, NG, 0>(g, a, b, off, cnt, sh);
+}
+
+template <int MT, int NG>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gram_item_kernel(CalcArgs a)
+{
+    // one item per wave.  Blocks are dispatched to the 8 XCDs round-robin
+    // (and handed out as earlier ones finish: the hardware dispatcher is the
+    // work queue); each XCD takes a contiguous range of (segment, group)
+    // items, the NG groups of a segment adjacent, so neighbouring nodes and
+    // the groups of one segment share that XCD's L2
     __shared__ ItemLds<MT> sh;
-    item_dispatch<MT, 0>(g, a, b, off, cnt, sh);
+    const int nitem = a.nbatch * a.nseg * NG;
+    const int per = (nitem + 7) / 8;
+    const int it = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (it < nitem)
+        item_run<MT, NG>(a, it, sh);
 }
 
 // ------------------------------------------------------------------ factor
@@ -771,7 +778,7 @@ __device__ inline double load_gram(const CalcArgs& a, int bnode, int seg, double
     for (int s = 0; s < a.nseg; ++s) {
         if (seg >= 0 && s != seg)
             continue;
-        const int node = a.node0 + bnode;
+        const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
         const int cnt = a.pertype ? a.counts[(size_t)node * a.ntypes + s] : 1;
         if (cnt == 0)
             continue;
@@ -815,7 +822,7 @@ __device__ inline double dot_gram(const CalcArgs& a, int bnode, int seg, const d
     for (int s = 0; s < a.nseg; ++s) {
         if (seg >= 0 && s != seg)
             continue;
-        const int node = a.node0 + bnode;
+        const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
         const int cnt = a.pertype ? a.counts[(size_t)node * a.ntypes + s] : 1;
         if (cnt == 0)
             continue;
@@ -846,7 +853,7 @@ __device__ inline void matvec_sts(const CalcArgs& a, int bnode, const double (&x
     for (int c = 0; c < B; ++c)
         v[c] = 0.0;
     for (int s = 0; s < a.nseg; ++s) {
-        const int node = a.node0 + bnode;
+        const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
         const int cnt = a.pertype ? a.counts[(size_t)node * a.ntypes + s] : 1;
         if (cnt == 0)
             continue;
@@ -907,7 +914,7 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
 {
     __shared__ FactorShared<B> sh;
     const int bnode = blockIdx.x;
-    const int node = a.node0 + bnode;
+    const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
     const int tid = threadIdx.x;
     const int m = a.m, md = a.md;
     const bool dotT = (a.mode == ENKFG_MODE_ENKF || a.mode == ENKFG_MODE_HYBRID);
@@ -1125,6 +1132,15 @@ hipError_t launch_scatter_rows(hipStream_t s, const double* src, const int* perm
     return launch_permute<double, true>(s, src, perm, dst, nrow, width, stride);
 }
 
+template <int M>
+void launch_gram_item(hipStream_t s, dim3 grid, const CalcArgs& a, int ng)
+{
+    if (ng == 2)
+        hipLaunchKernelGGL((gram_item_kernel<M, 2>), grid, dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((gram_item_kernel<M, 3>), grid, dim3(64), 0, s, a);
+}
+
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a)
 {
     const int MT = (a.m + 15) / 16;
@@ -1132,14 +1148,16 @@ hipError_t launch_gram(hipStream_t s, const CalcArgs& a)
     if (a.nbatch == 0)
         return hipSuccess;
     if (MT <= 6) {
-        const dim3 igrid((unsigned)(8 * ((a.nbatch * a.nseg * ING + 7) / 8)));
+        const int ng = (a.tile_groups == 2) ? 2 : 3;
+        const long long nitem = (long long)a.nbatch * a.nseg * ng;
+        const dim3 igrid((unsigned)(8 * ((nitem + 7) / 8)));
         switch (MT) {
-        case 1: hipLaunchKernelGGL(gram_item_kernel<1>, igrid, dim3(64), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(gram_item_kernel<2>, igrid, dim3(64), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(gram_item_kernel<3>, igrid, dim3(64), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(gram_item_kernel<4>, igrid, dim3(64), 0, s, a); break;
-        case 5: hipLaunchKernelGGL(gram_item_kernel<5>, igrid, dim3(64), 0, s, a); break;
-        default: hipLaunchKernelGGL(gram_item_kernel<6>, igrid, dim3(64), 0, s, a); break;
+        case 1: launch_gram_item<1>(s, igrid, a, ng); break;
+        case 2: launch_gram_item<2>(s, igrid, a, ng); break;
+        case 3: launch_gram_item<3>(s, igrid, a, ng); break;
+        case 4: launch_gram_item<4>(s, igrid, a, ng); break;
+        case 5: launch_gram_item<5>(s, igrid, a, ng); break;
+        default: launch_gram_item<6>(s, igrid, a, ng); break;
         }
         return hipGetLastError();
     }

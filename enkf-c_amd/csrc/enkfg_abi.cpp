@@ -75,6 +75,11 @@ void ll2xyz(double lon_deg, double lat_deg, double* out)
 
 }  // namespace
 
+// obs-count buckets of the calc batching: lower bounds of the largest
+// segment (node, obstype list) of a node
+constexpr int kBuckets = 4;
+constexpr int kBucketLo[kBuckets] = {0, 256, 1024, 4096};
+
 struct enkfg_ctx {
     enkfg_config cfg{};
     std::string err;
@@ -111,6 +116,13 @@ struct enkfg_ctx {
     DevBuf d_heads, d_cids, d_clc, d_cnext, d_lflag;
     long long pool_cap = 0;
     bool lists_valid = false;
+
+    // calc batching: nodes ordered by obs-count bucket (heaviest first), and
+    // the autotuned Gram tile groups per [precision][bucket] (0 = not tuned)
+    DevBuf d_order;
+    std::vector<int> h_order;
+    int bucket_nodes[kBuckets] = {0, 0, 0, 0};
+    int tile_groups[2][kBuckets] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 
     // transforms and diagnostics
     int ni_s = 0, nj_s = 0, stride = 0;
@@ -485,7 +497,7 @@ void enkfg_destroy(enkfg_ctx* c)
                       &c->d_scantmp, &c->d_ids, &c->d_lc, &c->d_heads, &c->d_cids, &c->d_clc, &c->d_cnext, &c->d_lflag,
                       &c->d_raw, &c->d_rawoff, &c->d_rrow, &c->d_rd2,
                       &c->d_srow, &c->d_sd2, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
-                      &c->d_pdfs, &c->d_psrf, &c->d_gram, &c->d_sts, &c->d_err, &c->d_E, &c->d_infl, &c->d_nlev,
+                      &c->d_pdfs, &c->d_psrf, &c->d_gram, &c->d_order, &c->d_sts, &c->d_err, &c->d_E, &c->d_infl, &c->d_nlev,
                       &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio, &c->d_adjg, &c->d_adjflag,
                       &c->d_adjlist, &c->d_adjz, &c->d_adjw, &c->d_gy, &c->d_gR};
     for (DevBuf* b : bufs)
@@ -869,10 +881,103 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     a.psrf = c->d_psrf.as<float>();
     a.nnode = nn;
     a.err = c->d_err.as<int>();
+    // Node order and batches.  Buckets by the node's largest segment (the
+    // work of its heaviest Gram item), [0, 256), [256, 1024), [1024, 4096)
+    // and [4096, ...) local obs: the nodes are processed bucket by bucket,
+    // heaviest first, in grid order within a bucket (neighbours, whose lists
+    // overlap, stay together for the L2); a batch never spans two buckets,
+    // and each bucket's Gram tile shape comes from the autotuner below.
+    std::vector<long long> all((size_t)nn * nt + 1);
+    HIPCHECK(c, hipMemcpy(all.data(), c->d_offsets.p, all.size() * sizeof(long long), hipMemcpyDeviceToHost));
+    std::vector<unsigned char> bucket_of((size_t)nn);
+    int bcount[kBuckets] = {0, 0, 0, 0};
+    for (int k = 0; k < nn; ++k) {
+        long long mx = 0;
+        if (pertype) {
+            for (int t = 0; t < nt; ++t)
+                mx = std::max(mx, all[(size_t)k * nt + t + 1] - all[(size_t)k * nt + t]);
+        } else {
+            mx = all[(size_t)(k + 1) * nt] - all[(size_t)k * nt];
+        }
+        int bk = kBuckets - 1;
+        while (bk > 0 && mx < kBucketLo[bk])
+            --bk;
+        bucket_of[k] = (unsigned char)bk;
+        bcount[bk]++;
+    }
+    int bstart[kBuckets + 1];   // bucket kBuckets-1 first
+    bstart[0] = 0;
+    for (int q = 0; q < kBuckets; ++q)
+        bstart[q + 1] = bstart[q] + bcount[kBuckets - 1 - q];
+    c->h_order.resize((size_t)nn);
+    {
+        int pos[kBuckets];
+        for (int q = 0; q < kBuckets; ++q)
+            pos[kBuckets - 1 - q] = bstart[q];
+        for (int k = 0; k < nn; ++k)
+            c->h_order[(size_t)pos[bucket_of[k]]++] = k;
+    }
+    for (int q = 0; q < kBuckets; ++q)
+        c->bucket_nodes[q] = bcount[q];
+
+    // Autotuner: the item Gram kernels (m <= 96) run a bucket's segments
+    // with 2 or 3 tile groups (wavefronts) per segment.  The first batch of
+    // a bucket that is not yet tuned for this precision and holds at least
+    // kTuneMin nodes runs both and keeps the faster (both write the same
+    // Gram matrices, bit for bit); the choice holds for the engine's life.
+    // Once every bucket large enough to tune is tuned and all present
+    // buckets share one choice, the nodes run in plain grid order (better L2
+    // reuse between neighbours) with that choice.
+    const bool mixed = c->cfg.precision == ENKFG_PREC_MIXED;
+    const bool items = (m + 15) / 16 <= 6 &&
+                       (!mixed || (unsigned long long)(c->nobs + 1) * (unsigned long long)a.ms * 4ull < (1ull << 32));
+    const int kTuneMin = std::min(nbatch, 256);
+    int* tg = c->tile_groups[mixed ? 1 : 0];
+    const int tg_default = mixed ? 2 : 3;
+    bool plain = !items;
+    if (items) {
+        bool tuned = true, same = true;
+        int common = 0;
+        for (int q = 0; q < kBuckets; ++q) {
+            if (bcount[q] == 0)
+                continue;
+            if (tg[q] == 0 && bcount[q] >= kTuneMin)
+                tuned = false;
+            const int g = tg[q] ? tg[q] : tg_default;
+            if (common == 0)
+                common = g;
+            else if (g != common)
+                same = false;
+        }
+        plain = tuned && same;
+    }
+    struct Batch {
+        int node0, n, bucket;
+    };
+    std::vector<Batch> batches;
+    if (plain) {
+        for (int k = 0; k < nn; k += nbatch)
+            batches.push_back({k, std::min(nbatch, nn - k), -1});
+        a.node_list = nullptr;
+    } else {
+        HIPCHECK(c, c->d_order.reserve(std::max<size_t>((size_t)nn, 1) * sizeof(int)));
+        if (nn > 0)
+            HIPCHECK(c, hipMemcpy(c->d_order.p, c->h_order.data(), (size_t)nn * sizeof(int), hipMemcpyHostToDevice));
+        a.node_list = c->d_order.as<int>();
+        for (int q = 0; q < kBuckets; ++q)
+            for (int k = bstart[q]; k < bstart[q + 1]; k += nbatch)
+                batches.push_back({k, std::min(nbatch, bstart[q + 1] - k), kBuckets - 1 - q});
+    }
+    const int nb = (int)batches.size();
+    int plain_groups = 0;
+    if (plain && items)
+        for (int q = 0; q < kBuckets; ++q)
+            if (bcount[q] > 0 && plain_groups == 0)
+                plain_groups = tg[q] ? tg[q] : tg_default;
+
     // batch b: Gram on `stream` into buffer b % 2, factor on `stream2`; the
     // Gram of batch b + 2 waits for the factor of batch b (same buffer)
-    const int nb = (nn + nbatch - 1) / nbatch;
-    while ((int)c->bev.size() < 4 * nb) {
+    while ((int)c->bev.size() < 4 * nb + 2) {
         hipEvent_t e;
         HIPCHECK(c, hipEventCreate(&e));
         c->bev.push_back(e);
@@ -880,17 +985,39 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     for (int b = 0; b < nb; ++b) {
         const int buf = b & 1;
         hipEvent_t* e = &c->bev[4 * (size_t)b];
-        a.node0 = b * nbatch;
-        a.nbatch = std::min(nbatch, nn - a.node0);
+        a.node0 = batches[b].node0;
+        a.nbatch = batches[b].n;
         a.gram = c->d_gram.as<double>() + buf * gram_elems;
         a.sts = c->d_sts.as<double>() + buf * sts_elems;
+        const int bk = batches[b].bucket;
         if (b >= 2)
             HIPCHECK(c, hipStreamWaitEvent(c->stream, c->bev[4 * (size_t)(b - 2) + 3], 0));
         HIPCHECK(c, hipEventRecord(e[0], c->stream));
-        if (c->cfg.precision == ENKFG_PREC_MIXED)
-            HIPCHECK(c, launch_gram_mixed(c->stream, a));
-        else
-            HIPCHECK(c, launch_gram(c->stream, a));
+        if (bk < 0) {
+            a.tile_groups = plain_groups;
+            HIPCHECK(c, mixed ? launch_gram_mixed(c->stream, a) : launch_gram(c->stream, a));
+        } else if (items && tg[bk] == 0 && a.nbatch >= kTuneMin) {
+            float best = 0.0f;
+            int pick = 0;
+            hipEvent_t t0 = c->bev[4 * (size_t)nb], t1 = c->bev[4 * (size_t)nb + 1];
+            for (int cand : {3, 2}) {
+                a.tile_groups = cand;
+                HIPCHECK(c, hipEventRecord(t0, c->stream));
+                HIPCHECK(c, mixed ? launch_gram_mixed(c->stream, a) : launch_gram(c->stream, a));
+                HIPCHECK(c, hipEventRecord(t1, c->stream));
+                HIPCHECK(c, hipEventSynchronize(t1));
+                float ms = 0.0f;
+                HIPCHECK(c, hipEventElapsedTime(&ms, t0, t1));
+                if (pick == 0 || ms < best) {
+                    best = ms;
+                    pick = cand;
+                }
+            }
+            tg[bk] = pick;
+        } else {
+            a.tile_groups = tg[bk];
+            HIPCHECK(c, mixed ? launch_gram_mixed(c->stream, a) : launch_gram(c->stream, a));
+        }
         HIPCHECK(c, hipEventRecord(e[1], c->stream));
         HIPCHECK(c, hipStreamWaitEvent(c->stream2, e[1], 0));
         HIPCHECK(c, hipEventRecord(e[2], c->stream2));
@@ -899,15 +1026,15 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream2));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    double tg = 0.0, tf = 0.0;
+    double tgram = 0.0, tf = 0.0;
     for (int b = 0; b < nb; ++b) {
-        tg += elapsed(c->bev[4 * (size_t)b], c->bev[4 * (size_t)b + 1]);
+        tgram += elapsed(c->bev[4 * (size_t)b], c->bev[4 * (size_t)b + 1]);
         tf += elapsed(c->bev[4 * (size_t)b + 2], c->bev[4 * (size_t)b + 3]);
     }
-    c->timings[1] = tg;
+    c->timings[1] = tgram;
     c->timings[2] = tf;
     c->timings[4] = (double)nb;
-    c->timings[5] = elapsed(c->ev[0], c->bev[4 * (size_t)(nb - 1) + 3]);
+    c->timings[5] = nb > 0 ? elapsed(c->ev[0], c->bev[4 * (size_t)(nb - 1) + 3]) : 0.0;
     int err = 0;
     HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
     if (err != 0)
@@ -915,12 +1042,8 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
 
     // calcstats (transforms.c:44-50) from the counts
     std::vector<long long> off((size_t)nn + 1);
-    {
-        std::vector<long long> all((size_t)nn * nt + 1);
-        HIPCHECK(c, hipMemcpy(all.data(), c->d_offsets.p, all.size() * sizeof(long long), hipMemcpyDeviceToHost));
-        for (int k = 0; k <= nn; ++k)
-            off[k] = all[(size_t)k * nt];
-    }
+    for (int k = 0; k <= nn; ++k)
+        off[k] = all[(size_t)k * nt];
     enkfg_stats st{};
     for (int k = 0; k < nn; ++k) {
         const int p = (int)(off[k + 1] - off[k]);
@@ -995,6 +1118,23 @@ int enkfg_get_timings(enkfg_ctx* c, double* ms, int n)
         return ENKFG_ERR_ARG;
     for (int k = 0; k < n && k < 6; ++k)
         ms[k] = c->timings[k];
+    return ENKFG_OK;
+}
+
+int enkfg_get_tuning(enkfg_ctx* c, int* out, int n)
+{
+    if (check_ctx(c) || out == nullptr || n < 1)
+        return ENKFG_ERR_ARG;
+    std::vector<int> v;
+    v.push_back(kBuckets);
+    for (int q = 0; q < kBuckets; ++q) {
+        v.push_back(kBucketLo[q]);
+        v.push_back(c->bucket_nodes[q]);
+        v.push_back(c->tile_groups[0][q]);
+        v.push_back(c->tile_groups[1][q]);
+    }
+    for (int k = 0; k < n && k < (int)v.size(); ++k)
+        out[k] = v[k];
     return ENKFG_OK;
 }
 

@@ -119,6 +119,9 @@ struct CalcArgs {
     const double* adj_g;
     double* adj_z;
     double* adj_w;
+    // Gram item kernels (m <= 96): tile groups (wavefronts) per segment, 2 or
+    // 3; any other value takes the kernel's default (3 FP64, 2 mixed)
+    int tile_groups;
 };
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a);
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a);   // gram_mixed.hip

@@ -18,6 +18,8 @@
 // v_mfma_f32_16x16x32_bf16 -- 8 consecutive obs of one member -- is one
 // 16-byte LDS read; the gathers are arranged so that each lane holds 4
 // consecutive obs of one member and stores them as one 8-byte write.
+#include <algorithm>
+
 #include "enkfg_internal.h"
 #include "gram_layout.h"
 
@@ -281,10 +283,11 @@ __global__ void __launch_bounds__(256) gram_mixed_kernel(CalcArgs a)
 // ------------------------------------------------------------------ one wavefront per work item
 //
 // For m <= 96: independent wavefronts, no workgroup barrier.  A wavefront
-// computes one of MNG = 2 groups of the tri_count(MT) tiles (f = g, g + 2,
-// ...; 11 and 10 tiles for m = 96, FP32 accumulators) of one (node, obstype
-// segment); both waves of a segment stage the whole chunk (the second finds
-// the rows in L2).  Chunks of MKC = 16 local obs (one K = 16 MFMA step,
+// computes one of NG groups of the tri_count(MT) tiles (f = g, g + NG, ...;
+// NG = 2: 11 and 10 tiles for m = 96, NG = 3: 7 each, FP32 accumulators;
+// NG is chosen per obs-count bucket by the host's autotuner) of one (node,
+// obstype segment); every wave of a segment stages the whole chunk (the
+// others find the rows in L2).  Chunks of MKC = 16 local obs (one K = 16 MFMA step,
 // v_mfma_f32_16x16x16_bf16, which halves the gathered registers in flight
 // against K = 32) are gathered into registers one chunk ahead -- item
 // u = lane + 64 j is member u % MP of obs rows 4 (u / MP) .. + 3 -- and
@@ -295,7 +298,6 @@ __global__ void __launch_bounds__(256) gram_mixed_kernel(CalcArgs a)
 // 2^-17 relative error); S^T s is summed in FP32 over a chunk and in FP64
 // over chunks.
 constexpr int MKC = 16;
-constexpr int MNG = 2;         // tile groups (wavefronts) per segment
 constexpr int MKR = MKC + 8;   // bf16 per member row (48 bytes: 8-byte aligned, staggered banks)
 typedef short sh4 __attribute__((ext_vector_type(4)));
 
@@ -315,24 +317,24 @@ struct MixItemRegs {
     double s;                  // s of row lane % MKC
 };
 
-template <int MT, int G>
+template <int MT, int NG, int G>
 constexpr int mix_item_ntile()
 {
-    return (tri_count(MT) - G + MNG - 1) / MNG;
+    return (tri_count(MT) - G + NG - 1) / NG;
 }
 
-// the tiles f = G + MNG t of group G
-template <int MT, int G, int T>
+// the tiles f = G + NG t of group G
+template <int MT, int NG, int G, int T>
 __device__ __attribute__((always_inline)) inline void mix_item_tiles(const sh4 (&h)[MT], const sh4 (&l)[MT],
                                                                      flt4* acc)
 {
-    if constexpr (G + MNG * T < tri_count(MT)) {
-        constexpr int f = G + MNG * T;
+    if constexpr (G + NG * T < tri_count(MT)) {
+        constexpr int f = G + NG * T;
         constexpr int I = tri_I(MT, f), J = tri_J(MT, f);
         acc[T] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[I], h[J], acc[T], 0, 0, 0);
         acc[T] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[I], l[J], acc[T], 0, 0, 0);
         acc[T] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(l[I], h[J], acc[T], 0, 0, 0);
-        mix_item_tiles<MT, G, T + 1>(h, l, acc);
+        mix_item_tiles<MT, NG, G, T + 1>(h, l, acc);
     }
 }
 
@@ -406,12 +408,12 @@ __device__ __attribute__((always_inline)) inline void mix_item_stage(const CalcA
     sh.sl[threadIdx.x & (MKC - 1)] = (float)(L.s * sh.lc[mslot][threadIdx.x & (MKC - 1)]);
 }
 
-template <int MT, int G>
+template <int MT, int NG, int G>
 __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcArgs& a, int b, long long off, int cnt,
                                                                     MixItemLds<MT>& sh)
 {
     constexpr int MP = 16 * MT;
-    constexpr int NTG = mix_item_ntile<MT, G>();
+    constexpr int NTG = mix_item_ntile<MT, NG, G>();
     const int lane = threadIdx.x & 63;
     const int kg = lane >> 4;      // which 8 obs of the K = 32 step
     const int col = lane & 15;
@@ -450,7 +452,7 @@ __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcAr
             h[X] = *reinterpret_cast<const sh4*>(&sh.hi[16 * X + col][k0]);
             l[X] = *reinterpret_cast<const sh4*>(&sh.lo[16 * X + col][k0]);
         }
-        mix_item_tiles<MT, G, 0>(h, l, acc);
+        mix_item_tiles<MT, NG, G, 0>(h, l, acc);
         if (G == 0) {
             float slv[4];
 #pragma unroll
@@ -479,7 +481,7 @@ __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcAr
     double* gout = a.gram + (size_t)b * MP * MP;
 #pragma unroll
     for (int t = 0; t < NTG; ++t) {
-        const int f = G + MNG * t;
+        const int f = G + NG * t;
         const int I = tri_I(MT, f), J = tri_J(MT, f);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -506,18 +508,26 @@ __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcAr
     }
 }
 
-template <int MT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gram_mixed_item_kernel(CalcArgs a)
+template <int MT, int NG, int G>
+__device__ __attribute__((always_inline)) inline void mix_item_dispatch(int g, const CalcArgs& a, int b, long long off,
+                                                                        int cnt, MixItemLds<MT>& sh)
 {
-    // blocks are dispatched to the 8 XCDs round-robin; each XCD takes a
-    // contiguous range of (segment, group) items
-    const int nitem = a.nbatch * a.nseg * MNG;
-    const int per = (nitem + 7) / 8;
-    const int it = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (it >= nitem)
-        return;
-    const int b = it / MNG;
-    const int g = it - b * MNG;
+    if constexpr (G < NG) {
+        if (g == G) {
+            if constexpr (mix_item_ntile<MT, NG, G>() > 0)
+                mix_item_wave<MT, NG, G>(a, b, off, cnt, sh);
+        } else {
+            mix_item_dispatch<MT, NG, G + 1>(g, a, b, off, cnt, sh);
+        }
+    }
+}
+
+// item `it` = (segment b, group g) of the batch
+template <int MT, int NG>
+__device__ __attribute__((always_inline)) inline void mix_item_run(const CalcArgs& a, int it, MixItemLds<MT>& sh)
+{
+    const int b = it / NG;
+    const int g = it - b * NG;
     const int node = a.node_list ? a.node_list[a.node0 + b / a.nseg] : a.node0 + b / a.nseg;
     const int seg = b % a.nseg;
     long long off;
@@ -529,18 +539,36 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gr
         off = a.offsets[(size_t)node * a.ntypes];
         cnt = (int)(a.offsets[(size_t)(node + 1) * a.ntypes] - off);
     }
-    if (cnt == 0)
-        return;
+    if (cnt > 0)
+        mix_item_dispatch<MT, NG, 0>(g, a, b, off, cnt, sh);
+}
+
+template <int MT, int NG>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gram_mixed_item_kernel(CalcArgs a)
+{
+    // one item per wave.  Blocks are dispatched to the 8 XCDs round-robin
+    // (and handed out as earlier ones finish: the hardware dispatcher is the
+    // work queue); each XCD takes a contiguous range of (segment, group)
+    // items, the NG groups of a segment adjacent, so neighbouring nodes and
+    // the groups of one segment share that XCD's L2
     __shared__ MixItemLds<MT> sh;
-    if (g == 0) {
-        mix_item_wave<MT, 0>(a, b, off, cnt, sh);
-    } else {
-        if constexpr (mix_item_ntile<MT, 1>() > 0)
-            mix_item_wave<MT, 1>(a, b, off, cnt, sh);
-    }
+    const int nitem = a.nbatch * a.nseg * NG;
+    const int per = (nitem + 7) / 8;
+    const int it = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (it < nitem)
+        mix_item_run<MT, NG>(a, it, sh);
 }
 
 }  // namespace
+
+template <int M>
+void launch_mix_item(hipStream_t s, dim3 grid, const CalcArgs& a, int ng)
+{
+    if (ng == 2)
+        hipLaunchKernelGGL((gram_mixed_item_kernel<M, 2>), grid, dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((gram_mixed_item_kernel<M, 3>), grid, dim3(64), 0, s, a);
+}
 
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
 {
@@ -551,14 +579,16 @@ hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
     // the item kernel addresses S with 32-bit byte offsets
     const bool small_s = (unsigned long long)(a.nobs + 1) * (unsigned long long)a.ms * 4ull < (1ull << 32);
     if (MT <= 6 && small_s) {
-        const dim3 igrid((unsigned)(8 * ((a.nbatch * a.nseg * MNG + 7) / 8)));
+        const int ng = (a.tile_groups == 3) ? 3 : 2;
+        const long long nitem = (long long)a.nbatch * a.nseg * ng;
+        const dim3 igrid((unsigned)(8 * ((nitem + 7) / 8)));
         switch (MT) {
-        case 1: hipLaunchKernelGGL(gram_mixed_item_kernel<1>, igrid, dim3(64), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(gram_mixed_item_kernel<2>, igrid, dim3(64), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(gram_mixed_item_kernel<3>, igrid, dim3(64), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(gram_mixed_item_kernel<4>, igrid, dim3(64), 0, s, a); break;
-        case 5: hipLaunchKernelGGL(gram_mixed_item_kernel<5>, igrid, dim3(64), 0, s, a); break;
-        default: hipLaunchKernelGGL(gram_mixed_item_kernel<6>, igrid, dim3(64), 0, s, a); break;
+        case 1: launch_mix_item<1>(s, igrid, a, ng); break;
+        case 2: launch_mix_item<2>(s, igrid, a, ng); break;
+        case 3: launch_mix_item<3>(s, igrid, a, ng); break;
+        case 4: launch_mix_item<4>(s, igrid, a, ng); break;
+        case 5: launch_mix_item<5>(s, igrid, a, ng); break;
+        default: launch_mix_item<6>(s, igrid, a, ng); break;
         }
         return hipGetLastError();
     }

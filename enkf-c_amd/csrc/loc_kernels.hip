@@ -117,12 +117,20 @@ __device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, con
 // of the stride grid and one obstype, so its lanes walk nearly the same
 // paths of one tree and their node loads coalesce.  ni is the grid's row
 // length (nnode = ni nj); tiles are row-major within a type.
+constexpr int kWalkRun = 16;   // consecutive walk items per XCD run
+
 __device__ inline bool thread_node(int nnode, int ni, int ntypes, int& t, int& node)
 {
     const int nj = nnode / ni;
     const int ti = (ni + 7) / 8, tj = (nj + 7) / 8;
     const long long per_type = (long long)ti * tj * 64;
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // blocks go to the 8 XCDs round-robin; each XCD takes runs of kWalkRun
+    // consecutive (type, tile) items, so neighbouring walks share its L2,
+    // and the runs are dealt round-robin, so dense and sparse regions spread
+    // over all XCDs (walk_grid rounds the grid up to a multiple of 8 runs)
+    const unsigned s = blockIdx.x / 8, x = blockIdx.x % 8;
+    const long long item = ((long long)(s / kWalkRun) * 8 + x) * kWalkRun + s % kWalkRun;
+    const long long gid = item * blockDim.x + threadIdx.x;
     if (gid >= per_type * ntypes)
         return false;
     t = (int)(gid / per_type);
@@ -509,10 +517,11 @@ __global__ void __launch_bounds__(256) loc_compact_kernel(long long nitem, int n
 }
 
 // grid of the walk kernels: one 64-thread block per 8 x 8 tile and obstype
-// (thread_node)
+// (thread_node), rounded up to a multiple of 8 runs
 inline dim3 walk_grid(int ni, int nj, int ntypes)
 {
-    return dim3((unsigned)((long long)ntypes * ((ni + 7) / 8) * ((nj + 7) / 8)));
+    const long long n = (long long)ntypes * ((ni + 7) / 8) * ((nj + 7) / 8);
+    return dim3((unsigned)(8 * kWalkRun * ((n + 8 * kWalkRun - 1) / (8 * kWalkRun))));
 }
 
 }  // namespace

@@ -32,7 +32,7 @@ EXPORTED = [
     "enkfg_set_obs_status", "enkfg_set_S", "enkfg_get_S", "enkfg_calc_transforms", "enkfg_get_transforms",
     "enkfg_get_diag", "enkfg_get_stats", "enkfg_localise", "enkfg_get_local_counts", "enkfg_get_localobs",
     "enkfg_set_transforms", "enkfg_update_fields", "enkfg_update_bg", "enkfg_ens_attach", "enkfg_ens_update",
-    "enkfg_get_timings", "enkfg_adjoint",
+    "enkfg_get_timings", "enkfg_get_tuning", "enkfg_adjoint",
 ]
 
 
@@ -108,6 +108,7 @@ def lib():
         L.enkfg_ens_attach.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Field), vp]
         L.enkfg_ens_update.argtypes = [vp, C.c_int, vp]
         L.enkfg_get_timings.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
+        L.enkfg_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
         L.enkfg_adjoint.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
         _lib = L
     return _lib
@@ -278,6 +279,15 @@ class Engine:
         gi = C.c_double()
         self._check(lib().enkfg_adjoint(self.h, C.c_void_p(dev_dJ_dmean), _p(gy), _p(gR), C.byref(gi)))
         return gy, gR, gi.value
+
+    def tuning(self):
+        """Obs-count buckets of the calc batching: a list of dicts (lower
+        bound of the largest (node, obstype) list, nodes at the last calc,
+        Gram tile groups chosen for FP64 and mixed precision, 0 = not tuned)."""
+        t = (C.c_int * 64)()
+        self._check(lib().enkfg_get_tuning(self.h, t, 64))
+        return [dict(lo=t[1 + 4 * q], nodes=t[2 + 4 * q], groups_fp64=t[3 + 4 * q], groups_mixed=t[4 + 4 * q])
+                for q in range(t[0])]
 
     def launches(self):
         """Gram (and factor) launches of the last calc_transforms."""

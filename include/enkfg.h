@@ -238,6 +238,18 @@ int enkfg_adjoint(enkfg_ctx* ctx, const float* dev_dJ_dmean, double* dJ_dy, doub
  * the time they take together). */
 int enkfg_get_timings(enkfg_ctx* ctx, double* ms, int n);
 
+/* Calc batching and the autotuner's choices (no counterpart in the
+ * reference, whose calc hands out columns through mpiqueue,
+ * transforms.c:700-716).  enkfg_calc_transforms orders the stride nodes by
+ * obs-count bucket (the largest (node, obstype) list: [0, 256), [256, 1024),
+ * [1024, 4096), [4096, ...)), heaviest bucket first, and runs each bucket's
+ * Gram matrices with the tile groups per list (2 or 3 wavefronts) that the
+ * first sufficiently large batch of the bucket measured fastest.  Writes up
+ * to n ints: out[0] = number of buckets B, then per bucket the lower bound,
+ * the nodes in it at the last calc, and the tile groups chosen for FP64 and
+ * for mixed precision (0 = not tuned yet). */
+int enkfg_get_tuning(enkfg_ctx* ctx, int* out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,43 @@ def test_localisation_one_walk_and_pool_overflow():
     E.close()
 
 
+@pytest.mark.parametrize("precision", [enkfg.PREC_FP64, enkfg.PREC_MIXED])
+def test_calc_buckets_and_autotune_bitwise(precision):
+    """The calc orders the nodes by obs-count bucket and tunes each bucket's
+    Gram tile groups (2 or 3 wavefronts per list) on its first large batch,
+    running both.  A node's Gram matrix does not depend on the order or the
+    tile groups, so a second calc on the same engine (tuned choices) gives
+    T, w and the diagnostics of the first (tuning) calc bit for bit; the
+    buckets account for every node."""
+    print("case calc buckets / autotune precision=%d" % precision, flush=True)
+    P = problem(seed=4, profiles=30, nobs=8000, locrad=150.0)
+    E = enkfg.Engine(P.nmem, mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF, precision=precision,
+                     pertype_diag=True, keep_fp64=True)
+    E.set_obstypes(P.obstypes)
+    E.set_obs(P.obs_type, P.obs_status, P.obs_lon, P.obs_lat, P.obs_estd)
+    E.set_S(P.HA, P.innov, standardised=False)
+    E.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
+    T1, w1, T641, w641 = E.transforms(fp64=True)
+    d1 = E.diag()
+    tu = E.tuning()
+    print("  buckets", tu, flush=True)
+    assert sum(b["nodes"] for b in tu) == P.node_lon.size
+    key = "groups_fp64" if precision == enkfg.PREC_FP64 else "groups_mixed"
+    assert all(b[key] in (0, 2, 3) for b in tu)
+    assert any(b[key] in (2, 3) for b in tu), "no bucket was tuned"
+    assert len([b for b in tu if b["nodes"] > 0]) >= 2, "the problem should span several buckets"
+    E.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
+    T2, w2, T642, w642 = E.transforms(fp64=True)
+    d2 = E.diag()
+    assert E.tuning() == tu
+    assert np.array_equal(T641.view(np.uint64), T642.view(np.uint64))
+    assert np.array_equal(w641.view(np.uint64), w642.view(np.uint64))
+    assert np.array_equal(T1.view(np.uint32), T2.view(np.uint32))
+    for x, y in zip(d1, d2):
+        assert np.array_equal(x, y)
+    E.close()
+
+
 @pytest.mark.parametrize("nmax", [1, 17, 60])
 def test_localisation_nlobsmax(nmax):
     """NLOBSMAX (observations.c:1899-1908): candidates sorted by distance,
@@ -161,6 +198,7 @@ CASES = [
     (enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, 24, 24, 1.0, 0),
     (enkfg.MODE_ENKF, enkfg.SCHEME_ETKF, 24, 24, 1.0, 0),
     (enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, 96, 96, 0.5, 30),
+    (enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, 37, 37, 1.0, 10),      # odd m: the last pivot pairs with padding
     (enkfg.MODE_ENKF, enkfg.SCHEME_ETKF, 96, 96, 1.0, 30),
     (enkfg.MODE_ENOI, enkfg.SCHEME_DENKF, 40, 40, 1.0, 0),
     (enkfg.MODE_HYBRID, enkfg.SCHEME_DENKF, 48, 32, 1.0, 20),
