@@ -181,9 +181,31 @@ def main():
                     tcalc=tcalc / K, tupd=tupd / K, wall=wall, stats=st, launches=nl, t_tree=t_tree,
                     band=(Tb, wb, pdb), tuning=tuning)
 
+    def run_graph(precision):
+        """calc passes of the same workload with the Gram/factor batches as
+        one captured HIP graph (ENKFG_FLAG_GRAPH): two warmup passes (tuning,
+        capture), then timed replays; returns ms per calc pass"""
+        E = enkfg.Engine(m, mode=enkfg.MODE_ENKF, scheme=scheme_id, precision=precision, pertype_diag=True,
+                         keep_fp64=False, graph=True)
+        E.set_obstypes(X.obstypes)
+        E.set_obs(X.obs_type, X.obs_status, X.obs_lon, X.obs_lat, X.obs_estd)
+        E.set_S(X.HA, X.innov, standardised=False)
+        for _ in range(2):
+            E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
+        E.sync()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            E.calc_transforms(X.node_lon, X.node_lat, stride=X.stride)
+        E.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        log("  [%s, graph] calc %.1f ms per pass (%d replays)" % (pname[precision], ms, args.steps))
+        E.close()
+        return ms
+
     pname = {enkfg.PREC_FP64: "fp64", enkfg.PREC_MIXED: "mixed"}
     R = run(enkfg.PREC_FP64)
     Rm = run(enkfg.PREC_MIXED) if args.precision == "both" else None
+    graph_ms = run_graph(enkfg.PREC_FP64)
     K = args.steps
     phases = R["phases"]
     st = R["stats"]
@@ -241,7 +263,11 @@ def main():
                kd_tree_build_s=R["t_tree"],
                roofline=roof,
                update_roofline=upd_roof,
-               calc_buckets=R["tuning"])
+               calc_buckets=R["tuning"],
+               graph=dict(note="the same FP64 calc pass with the Gram/factor batches replayed as one captured "
+                               "HIP graph (ENKFG_FLAG_GRAPH)", calc_ms=graph_ms,
+                          local_analyses_per_s=n_an / (graph_ms * 1e-3),
+                          speedup_vs_stream_launches=1e3 * R["tcalc"] / graph_ms))
     if Rm is not None:
         Tb, wb, pdb = R["band"]
         Tm, wm, pdm = Rm["band"]

@@ -123,6 +123,12 @@ struct enkfg_ctx {
     std::vector<int> h_order;
     int bucket_nodes[kBuckets] = {0, 0, 0, 0};
     int tile_groups[2][kBuckets] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    // captured batch loop (ENKFG_FLAG_GRAPH): the executable graph, the
+    // signature of what it was captured for, and its dependency events
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<long long> graph_sig;
+    std::vector<hipEvent_t> gev;
+    int graph_launches = 0;   // replays of the current graph (statistics)
 
     // transforms and diagnostics
     int ni_s = 0, nj_s = 0, stride = 0;
@@ -509,6 +515,10 @@ void enkfg_destroy(enkfg_ctx* c)
         (void)hipEventDestroy(e);
     if (c->stream != nullptr)
         (void)hipStreamDestroy(c->stream);
+    if (c->graph_exec != nullptr)
+        (void)hipGraphExecDestroy(c->graph_exec);
+    for (hipEvent_t e : c->gev)
+        (void)hipEventDestroy(e);
     if (c->stream2 != nullptr)
         (void)hipStreamDestroy(c->stream2);
     delete c;
@@ -975,6 +985,96 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
             if (bcount[q] > 0 && plain_groups == 0)
                 plain_groups = tg[q] ? tg[q] : tg_default;
 
+    // Graph mode: once no bucket of this run needs tuning, the batch loop is
+    // captured as one HIP graph (kernel nodes and event dependencies only)
+    // and replayed while its signature -- the batches, the tile groups and
+    // every buffer the kernels read or write -- is unchanged.
+    bool need_tune = false;
+    for (const Batch& bt : batches)
+        if (items && bt.bucket >= 0 && tg[bt.bucket] == 0 && bt.n >= kTuneMin)
+            need_tune = true;
+    if ((c->cfg.flags & ENKFG_FLAG_GRAPH) && !need_tune && nb > 0) {
+        std::vector<long long> sig = {nb, (long long)mixed, plain_groups, (long long)(intptr_t)a.node_list,
+                                      (long long)(intptr_t)a.counts, (long long)(intptr_t)a.offsets,
+                                      (long long)(intptr_t)a.ids, (long long)(intptr_t)a.lc,
+                                      (long long)(intptr_t)a.S, (long long)(intptr_t)a.sv,
+                                      (long long)(intptr_t)c->d_gram.p, (long long)(intptr_t)c->d_sts.p,
+                                      (long long)(intptr_t)a.T, (long long)(intptr_t)a.w,
+                                      (long long)(intptr_t)a.T64, (long long)(intptr_t)a.w64,
+                                      (long long)(intptr_t)a.pnlobs, (long long)(intptr_t)a.pdfs,
+                                      (long long)(intptr_t)a.psrf, (long long)(intptr_t)a.err, nn, nbatch};
+        for (const Batch& bt : batches) {
+            sig.push_back(bt.node0);
+            sig.push_back(bt.n);
+            sig.push_back(bt.bucket >= 0 ? tg[bt.bucket] : -1);
+        }
+        if (c->graph_exec == nullptr || sig != c->graph_sig) {
+            if (c->graph_exec != nullptr) {
+                HIPCHECK(c, hipGraphExecDestroy(c->graph_exec));
+                c->graph_exec = nullptr;
+            }
+            while ((int)c->gev.size() < 2 * nb + 2) {
+                hipEvent_t e;
+                HIPCHECK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                c->gev.push_back(e);
+            }
+            hipGraph_t graph = nullptr;
+            HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            // stream2 joins the capture through the fork event
+            hipError_t ce = hipEventRecord(c->gev[2 * nb], c->stream);
+            if (ce == hipSuccess)
+                ce = hipStreamWaitEvent(c->stream2, c->gev[2 * nb], 0);
+            for (int b = 0; b < nb && ce == hipSuccess; ++b) {
+                const int buf = b & 1;
+                a.node0 = batches[b].node0;
+                a.nbatch = batches[b].n;
+                a.gram = c->d_gram.as<double>() + buf * gram_elems;
+                a.sts = c->d_sts.as<double>() + buf * sts_elems;
+                a.tile_groups = batches[b].bucket >= 0 ? tg[batches[b].bucket] : plain_groups;
+                if (b >= 2)
+                    ce = hipStreamWaitEvent(c->stream, c->gev[2 * (b - 2) + 1], 0);
+                if (ce == hipSuccess)
+                    ce = mixed ? launch_gram_mixed(c->stream, a) : launch_gram(c->stream, a);
+                if (ce == hipSuccess)
+                    ce = hipEventRecord(c->gev[2 * b], c->stream);
+                if (ce == hipSuccess)
+                    ce = hipStreamWaitEvent(c->stream2, c->gev[2 * b], 0);
+                if (ce == hipSuccess)
+                    ce = launch_factor(c->stream2, a);
+                if (ce == hipSuccess)
+                    ce = hipEventRecord(c->gev[2 * b + 1], c->stream2);
+            }
+            // join stream2 back before the capture ends
+            if (ce == hipSuccess)
+                ce = hipEventRecord(c->gev[2 * nb + 1], c->stream2);
+            if (ce == hipSuccess)
+                ce = hipStreamWaitEvent(c->stream, c->gev[2 * nb + 1], 0);
+            const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+            if (ce != hipSuccess || ee != hipSuccess) {
+                if (graph != nullptr)
+                    (void)hipGraphDestroy(graph);
+                return fail(c, ENKFG_ERR_HIP, "graph capture of the calc batches: %s",
+                            hipGetErrorString(ce != hipSuccess ? ce : ee));
+            }
+            const hipError_t ie = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                c->graph_exec = nullptr;
+                return fail(c, ENKFG_ERR_HIP, "graph instantiation: %s", hipGetErrorString(ie));
+            }
+            c->graph_sig = sig;
+            c->graph_launches = 0;
+        }
+        HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+        HIPCHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->graph_launches++;
+        c->timings[1] = elapsed(c->ev[2], c->ev[3]);
+        c->timings[2] = 0.0;
+        c->timings[4] = (double)nb;
+        c->timings[5] = elapsed(c->ev[0], c->ev[3]);
+    } else {
     // batch b: Gram on `stream` into buffer b % 2, factor on `stream2`; the
     // Gram of batch b + 2 waits for the factor of batch b (same buffer)
     while ((int)c->bev.size() < 4 * nb + 2) {
@@ -1035,6 +1135,7 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     c->timings[2] = tf;
     c->timings[4] = (double)nb;
     c->timings[5] = nb > 0 ? elapsed(c->ev[0], c->bev[4 * (size_t)(nb - 1) + 3]) : 0.0;
+    }
     int err = 0;
     HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
     if (err != 0)

@@ -22,7 +22,7 @@ OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_LINALG, ERR_STATE, ERR_UNSUPPORTED, ERR_NON
 MODE_ENKF, MODE_ENOI, MODE_HYBRID = 1, 2, 3
 SCHEME_DENKF, SCHEME_ETKF = 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
-FLAG_PERTYPE_DIAG, FLAG_KEEP_FP64 = 1, 2
+FLAG_PERTYPE_DIAG, FLAG_KEEP_FP64, FLAG_GRAPH = 1, 2, 4
 UPD_OUTPUTINC, UPD_HYBRID_CONDITION = 32, 1024
 NLOBSMAX_NONE = 2147483647
 MAXLOCRAD = 8
@@ -151,12 +151,13 @@ class Engine:
     """One enkfg context (one device, one stream)."""
 
     def __init__(self, nmem, nmem_dynamic=None, mode=MODE_ENKF, scheme=SCHEME_DENKF, alpha=1.0, gamma=1.0,
-                 precision=PREC_FP64, device=0, pertype_diag=True, keep_fp64=False):
+                 precision=PREC_FP64, device=0, pertype_diag=True, keep_fp64=False, graph=False):
         self.nmem = nmem
         self.md = nmem if nmem_dynamic is None else nmem_dynamic
         self.mode = mode
         cfg = Config(mode, scheme, precision, nmem, self.md, alpha, gamma, device,
-                     (FLAG_PERTYPE_DIAG if pertype_diag else 0) | (FLAG_KEEP_FP64 if keep_fp64 else 0))
+                     (FLAG_PERTYPE_DIAG if pertype_diag else 0) | (FLAG_KEEP_FP64 if keep_fp64 else 0)
+                     | (FLAG_GRAPH if graph else 0))
         h = C.c_void_p()
         r = lib().enkfg_create(C.byref(h), C.byref(cfg))
         if r != OK:

@@ -64,6 +64,11 @@ extern "C" {
                                       USE_MPIQUEUE build computes them
                                       (calc/transforms.c:853-879) */
 #define ENKFG_FLAG_KEEP_FP64 2     /* keep pre-cast FP64 T and w */
+#define ENKFG_FLAG_GRAPH 4         /* run the calc's Gram/factor batches as
+                                      one captured HIP graph, replayed while
+                                      the batches and buffers are unchanged
+                                      (timings then give the Gram and factor
+                                      phases as one, in [1]) */
 
 #define ENKFG_NLOBSMAX_NONE 2147483647
 #define ENKFG_MAXLOCRAD 8

@@ -144,6 +144,46 @@ def test_calc_buckets_and_autotune_bitwise(precision):
     E.close()
 
 
+@pytest.mark.parametrize("precision", [enkfg.PREC_FP64, enkfg.PREC_MIXED])
+def test_calc_graph_replay_bitwise(precision):
+    """ENKFG_FLAG_GRAPH: the calc's Gram/factor batches are captured as one
+    HIP graph and replayed.  Every replay -- with eager work (localisation,
+    result reads, a new S) between them -- must equal an eager engine's
+    result bit for bit."""
+    print("case calc graph replay precision=%d" % precision, flush=True)
+    P = problem(seed=4, profiles=30, nobs=8000, locrad=150.0)
+
+    def make(graph):
+        E = enkfg.Engine(P.nmem, mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF, precision=precision,
+                         pertype_diag=True, keep_fp64=True, graph=graph)
+        E.set_obstypes(P.obstypes)
+        E.set_obs(P.obs_type, P.obs_status, P.obs_lon, P.obs_lat, P.obs_estd)
+        E.set_S(P.HA, P.innov, standardised=False)
+        return E
+
+    def result(E):
+        E.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
+        T, w, T64, w64 = E.transforms(fp64=True)
+        return [T64.view(np.uint64), w64.view(np.uint64), T.view(np.uint32)] + list(E.diag())
+
+    E0, E1 = make(False), make(True)
+    ref = result(E0)
+    for k in range(4):
+        got = result(E1)
+        for x, y in zip(got, ref):
+            assert np.array_equal(x, y), "replay %d" % k
+    # new S (same buffers): the replayed graph reads the current data
+    E0.set_S(P.HA * 0.75, P.innov, standardised=False)
+    E1.set_S(P.HA * 0.75, P.innov, standardised=False)
+    ref = result(E0)
+    for k in range(2):
+        got = result(E1)
+        for x, y in zip(got, ref):
+            assert np.array_equal(x, y), "replay after set_S %d" % k
+    E0.close()
+    E1.close()
+
+
 @pytest.mark.parametrize("nmax", [1, 17, 60])
 def test_localisation_nlobsmax(nmax):
     """NLOBSMAX (observations.c:1899-1908): candidates sorted by distance,
