@@ -238,7 +238,11 @@ def main():
         roof = upd_roof
     elif dom == "gram":
         ach = F_gram / (phases[1] * 1e-3)
-        kname = ("gram_item_kernel<%d>" if m <= 96 else "gram_kernel<%d>") % ((m + 15) // 16)
+        # the item kernel with the tile groups the autotuner chose for the
+        # tile's largest bucket
+        big = max(R["tuning"], key=lambda b: b["nodes"])
+        kname = ("gram_item_kernel<%d, %d>" % ((m + 15) // 16, big["groups_fp64"] or 3) if m <= 96
+                 else "gram_kernel<%d>" % ((m + 15) // 16))
         nl = R["launches"]
         roof = dict(bound="mfma", achieved=ach / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s", frac=ach / PEAK_FP64,
                     traffic=profiled_traffic(tag, kname), kernel=kname + " (FP64 MFMA)",
