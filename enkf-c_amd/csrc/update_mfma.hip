@@ -113,6 +113,42 @@ __device__ inline float bilin(const Bilin& b, float v00, float v10, float v01, f
     return b.icopy ? ta : ta * b.wi1 + tb * b.wi2;
 }
 
+// sum over the 16 lanes of a row (DPP: quad swaps, then row rotations by 4
+// and 8); every lane of the row receives the total
+template <int CTRL>
+__device__ __attribute__((always_inline)) inline double dpp_f64(double x)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __attribute__((always_inline)) inline double row16_sum(double x)
+{
+    x += dpp_f64<0xb1>(x);    // quad_perm(1, 0, 3, 2)
+    x += dpp_f64<0x4e>(x);    // quad_perm(2, 3, 0, 1)
+    x += dpp_f64<0x124>(x);   // row_ror:4
+    x += dpp_f64<0x128>(x);   // row_ror:8
+    return x;
+}
+
+// the records of field group f0 into registers, coalesced: float q of field
+// f is lane-contiguous; fields past the end re-read the last one
+template <int MT, int NL>
+__device__ __attribute__((always_inline)) inline void load_group(const UpdArgs& a, long long cell, int f0,
+                                                                 float (&v)[NL])
+{
+    constexpr int MP = 16 * MT;
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+        const int q = (int)threadIdx.x + 256 * u;
+        const int f = q / MP, e = q - f * MP;
+        const int fg = min(f0 + min(f, FG - 1), a.nfield - 1);
+        v[u] = a.E[((long long)fg * a.ncell + cell) * a.m + min(e, a.m - 1)];
+    }
+}
+
 template <int MT, bool MIX>
 __device__ inline void store_t(UpdLds<MT, MIX>& sh, int e1, int e2, float t)
 {
@@ -146,6 +182,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
     const bool hybrid = (a.mode == ENKFG_MODE_HYBRID);
     const double k_d = (md > 1) ? sqrt((double)(m - 1) / (double)(md - 1)) : 1.0;
     const int nl = a.nlevels[(long long)j * a.mni + i];
+    // the first field group's records are loaded while T_cell is formed; each
+    // later group's while the previous one is processed
+    constexpr int NL = (FG * MP + 255) / 256;
+    float vnext[NL];
+    load_group<MT, NL>(a, cell, 0, vnext);
 
     // ---- 1. T_cell and w_cell (zero outside md x m)
     {
@@ -220,29 +261,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
     }
 
     for (int f0 = 0; f0 < a.nfield; f0 += FG) {
-        __syncthreads();   // T is in LDS; the previous group's reads are done
-        // ---- 2a. the group's records, coalesced: float q of field f is
-        // lane-contiguous; fields past the end re-read the last one
-        {
-            constexpr int NL = (FG * MP + 255) / 256;
-            float v[NL];
+        // per-field parameters of this group, loaded ahead of the next
+        // group's records (so that waiting for them does not wait for those)
+        const int fst = min(f0 + 16 * w + (lane & 15), a.nfield - 1);   // statistics field of this lane
+        const int lev_s = a.flevel[fst], surf_s = a.fsurf[fst];
+        float infl0_r[4];
+        double ratio_r[4];
+        int log_r[4];
 #pragma unroll
-            for (int u = 0; u < NL; ++u) {
-                const int q = tid + 256 * u;
-                const int f = q / MP, e = q - f * MP;
-                const int fg = min(f0 + min(f, FG - 1), a.nfield - 1);
-                v[u] = a.E[((long long)fg * a.ncell + cell) * m + min(e, m - 1)];
-            }
-#pragma unroll
-            for (int u = 0; u < NL; ++u) {
-                const int q = tid + 256 * u;
-                const int f = q / MP, e = q - f * MP;
-                if (f < FG)
-                    sh.at[e][f] = (e < m) ? v[u] : 0.0f;
-            }
-            for (int q = tid; q < (KP - MP) * FG; q += 256)
-                sh.at[MP + q / FG][q % FG] = 0.0f;
+        for (int r = 0; r < 4; ++r) {
+            const int fo = min(f0 + 16 * w + 4 * (lane >> 4) + r, a.nfield - 1);   // output fields of this lane
+            infl0_r[r] = a.finfl[fo];
+            ratio_r[r] = a.finfratio[fo];
+            log_r[r] = a.fapplylog[fo];
         }
+        __syncthreads();   // T is in LDS; the previous group's reads are done
+        // ---- 2a. the group's records (loaded earlier) into LDS
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int q = tid + 256 * u;
+            const int f = q / MP, e = q - f * MP;
+            if (f < FG)
+                sh.at[e][f] = (e < m) ? vnext[u] : 0.0f;
+        }
+        for (int q = tid; q < (KP - MP) * FG; q += 256)
+            sh.at[MP + q / FG][q % FG] = 0.0f;
+        if (f0 + FG < a.nfield)
+            load_group<MT, NL>(a, cell, f0 + FG, vnext);
         __syncthreads();
         // ---- 2b. per-field statistics: lane (f, g) covers members
         // e = g (mod 4); the 4 lanes of a field are lanes f, f + 16, f + 32,
@@ -262,7 +307,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
             eb = min(eb, __shfl_xor(eb, 16));
             eb = min(eb, __shfl_xor(eb, 32));
             int z = 0;
-            if (!live || is_masked(nl, a.flevel[min(fgl, a.nfield - 1)], a.fsurf[min(fgl, a.nfield - 1)], a.surfk))
+            if (!live || is_masked(nl, lev_s, surf_s, a.surfk))
                 z = 1;
             else if (eb < md)
                 z = 1;
@@ -370,14 +415,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
                 if (e < md)
                     v2a += (double)(va * va);
             }
-            v2a += __shfl_xor(v2a, 1);
-            v2a += __shfl_xor(v2a, 2);
-            v2a += __shfl_xor(v2a, 4);
-            v2a += __shfl_xor(v2a, 8);
+            v2a = row16_sum(v2a);
             if (fgl >= a.nfield)
                 continue;
-            const float inflation0 = a.finfl[fgl];
-            const double inf_ratio = a.finfratio[fgl];
+            const float inflation0 = infl0_r[r];
+            const double inf_ratio = ratio_r[r];
             float inflation = inflation0;
             if (!isnan(inf_ratio) && v2a > 0.0) {
                 inflation = (float)(sqrt(sh.v2f[fl] / v2a) * inf_ratio + 1.0 - inf_ratio);
@@ -389,7 +431,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
             const double v1a = sh.v1a[fl], v1f = sh.v1f[fl];
             float* rec = a.E + ((long long)fgl * a.ncell + cell) * m;
             const bool outinc = a.outputinc != 0;
-            const bool applylog = a.fapplylog[fgl] != 0;
+            const bool applylog = log_r[r] != 0;
 #pragma unroll
             for (int J = 0; J < MT; ++J) {
                 const int e = 16 * J + col;
