@@ -53,7 +53,7 @@ template <int MT, bool MIX>
 struct UpdLds {
     using G = UpdGeom<MT, MIX>;
     float at[G::KP][G::AS];                        // at[e][f]: values, then anomalies (0 for e >= m)
-    float tt[MIX ? 1 : G::KP][MIX ? 1 : G::TS];    // tt[e2][e1] = T_cell[e1][e2] (FP32 path)
+    float tt[MIX ? 1 : G::KP][MIX ? 1 : G::TS];    // tt[e2][e1 ^ ((e2 / 4) % 16)] = T_cell[e1][e2] (FP32 path)
     alignas(16) _Float16 th[MIX ? G::MP : 1][MIX ? G::HS : 8];   // th[e1][e2] = hi(T_cell[e1][e2]) (mixed)
     alignas(16) _Float16 tl[MIX ? G::MP : 1][MIX ? G::HS : 8];   // lo part
     float wc[G::KP];
@@ -157,7 +157,9 @@ __device__ inline void store_t(UpdLds<MT, MIX>& sh, int e1, int e2, float t)
         sh.th[e1][e2] = h;
         sh.tl[e1][e2] = (_Float16)(t - (float)h);
     } else {
-        sh.tt[e2][e1] = t;
+        // column swizzle within 16-member blocks: the lanes of a store hold
+        // e2 four apart, all in one bank without it
+        sh.tt[e2][e1 ^ ((e2 >> 2) & 15)] = t;
     }
 }
 
@@ -371,7 +373,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
                 const float av = sh.at[k][fa];
 #pragma unroll
                 for (int J = 0; J < MT; ++J)
-                    acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sh.tt[k][16 * J + col], acc[J], 0, 0, 0);
+                    acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sh.tt[k][16 * J + (col ^ (s & 15))], acc[J], 0,
+                                                                  0, 0);
             }
         } else {
             const float sc = sh.scale[fa];
