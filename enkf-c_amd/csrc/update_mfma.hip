@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
         if ((m & 3) == 0) {
             // quads of 4 consecutive e2: row e1 = q / (m / 4) by a multiply-high
             // (exact for these small q)
-            constexpr int UB = 4;
+            constexpr int UB = 5;   // md*m/4 = 2304 quads at m = 96: two rounds of loads
             const int d = m >> 2;
             const unsigned magic = 0xffffffffu / (unsigned)d + 1u;
             const int n = md * d;
@@ -300,11 +300,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
             const int g = lane >> 4;
             const int fgl = f0 + fl;
             const bool live = fgl < a.nfield;
+            // first bad member and the FP64 sum of the dynamic members in one
+            // pass (the sum is used only when no dynamic member is bad)
             int eb = m;
+            double p = 0.0;
             for (int e = g; e < m; e += 4) {
                 const float v = sh.at[e][fl];
                 if (eb == m && (!isfinite(v) || fabsf(v) > kMaxVal))
                     eb = e;
+                if (e < md)
+                    p += (double)v;
             }
             eb = min(eb, __shfl_xor(eb, 16));
             eb = min(eb, __shfl_xor(eb, 32));
@@ -316,9 +321,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
             else if (eb < m)
                 z = 2;
             // FP64 mean of the dynamic members (partials combined in lane order)
-            double p = 0.0;
-            for (int e = g; e < md; e += 4)
-                p += (double)sh.at[e][fl];
             double s0 = __shfl(p, lane & 15), s1 = __shfl(p, (lane & 15) + 16);
             double s2 = __shfl(p, (lane & 15) + 32), s3 = __shfl(p, (lane & 15) + 48);
             const double v1_f = (((s0 + s1) + s2) + s3) / (double)md;
