@@ -68,6 +68,27 @@ def test_update_fields_parity(stride, periodic, outputinc, surfk):
     E.close()
 
 
+@pytest.mark.parametrize("nmem", [30, 17])
+def test_update_member_count_not_multiple_of_four(nmem):
+    """m % 4 != 0: the scalar interpolation of T_cell (the quad path needs
+    whole float4 rows) and a partial last member tile."""
+    print("case update nmem=%d" % nmem, flush=True)
+    P = synth.make_problem(seed=31 + nmem, nmem=nmem, nobs=1000, locrad=60.0, nj=28, ni=35, stride=3, land_frac=0.1)
+    T, w = transforms(P, 1, 1, nmem, alpha=0.9)
+    bufs, fl = fields_of(P, nmem)
+    bufs[0][2, 4, 6] = np.nan
+    ref = [b.copy() for b in bufs]
+    gpu = [b.copy() for b in bufs]
+    infl_o = O.update_fields(1, nmem, nmem, 3, 0, 0, P.nlevels, T, w, fl, ref, 0, P.nj - 1, want_infl=True)
+    E = enkfg.Engine(nmem, mode=enkfg.MODE_ENKF)
+    E.set_transforms(T, w, 3)
+    infl_g = E.update_fields(P.nlevels, 3, 0, 0, fl, gpu, 0, P.nj - 1, want_infl=True)
+    for a, b in zip(gpu, ref):
+        check(a, b)
+    assert np.abs(infl_g - infl_o).max() <= 1e-5
+    E.close()
+
+
 def test_update_tile_rows():
     print("case update j-tile", flush=True)
     P = synth.make_problem(seed=21, nmem=24, nobs=1000, locrad=60.0, nj=33, ni=40, stride=3)
@@ -86,7 +107,7 @@ def test_update_tile_rows():
     E.close()
 
 
-@pytest.mark.parametrize("m,md", [(40, 28), (144, 96)])
+@pytest.mark.parametrize("m,md", [(40, 28), (144, 96), (37, 25)])
 def test_update_hybrid_with_conditioning(m, md):
     print("case update hybrid m=%d md=%d" % (m, md), flush=True)
     P = synth.make_problem(seed=5, nmem=m, nobs=1200, locrad=60.0, nj=30, ni=36, stride=3)
