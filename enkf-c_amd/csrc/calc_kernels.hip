@@ -344,7 +344,12 @@ __global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
 // executes its LDS instructions in order, and the compiler keeps them in
 // order (the accesses may alias).  The taper is applied in FP64 as an
 // operand is formed, (double) S * lc, as the reference forms Sloc.
-constexpr int IKC = 8;         // obs per chunk: two k-steps
+// obs per chunk: four k-steps.  16 rather than 8: the calc pass on the tile
+// takes 888 instead of 904 ms -- the wave's LDS (16 KB) holds the kernel to
+// about 9 waves per CU, and the factor kernels of the previous batch, on the
+// other stream, then overlap it better (32: 984 ms, 222 VGPRs).
+constexpr int IKC = 16;
+static_assert(kListPad >= 4 * IKC, "list padding too small for the item Gram pipeline");
 
 template <int MT>
 struct ItemLds {
