@@ -117,17 +117,21 @@ __device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, con
 // of the stride grid and one obstype, so its lanes walk nearly the same
 // paths of one tree and their node loads coalesce.  ni is the grid's row
 // length (nnode = ni nj); tiles are row-major within a type.
-constexpr int kWalkRun = 16;   // consecutive walk items per XCD run
+constexpr int kWalkRun = 64;   // consecutive walk items (one 8 x 8 block of tiles) per XCD run
 
 __device__ inline bool thread_node(int nnode, int ni, int ntypes, int& t, int& node)
 {
     const int nj = nnode / ni;
     const int ti = (ni + 7) / 8, tj = (nj + 7) / 8;
-    const long long per_type = (long long)ti * tj * 64;
-    // blocks go to the 8 XCDs round-robin; each XCD takes runs of kWalkRun
-    // consecutive (type, tile) items, so neighbouring walks share its L2,
-    // and the runs are dealt round-robin, so dense and sparse regions spread
-    // over all XCDs (walk_grid rounds the grid up to a multiple of 8 runs)
+    const int sti = (ti + 7) / 8, stj = (tj + 7) / 8;   // 8 x 8 blocks of tiles
+    const long long per_type = (long long)sti * stj * 64 * 64;
+    // Tiles are ordered in 8 x 8 blocks of tiles (64 x 64 nodes), row-major
+    // within and between blocks.  Blocks go to the 8 XCDs round-robin; each
+    // XCD takes runs of kWalkRun consecutive (type, tile) items -- one block
+    // of tiles -- so the walks in flight on one L2 cover a compact patch of
+    // the grid, and the runs are dealt round-robin, so dense and sparse
+    // regions spread over all XCDs (walk_grid rounds the grid up to a
+    // multiple of 8 runs; items past the grid's edge return)
     const unsigned s = blockIdx.x / 8, x = blockIdx.x % 8;
     const long long item = ((long long)(s / kWalkRun) * 8 + x) * kWalkRun + s % kWalkRun;
     const long long gid = item * blockDim.x + threadIdx.x;
@@ -136,7 +140,11 @@ __device__ inline bool thread_node(int nnode, int ni, int ntypes, int& t, int& n
     t = (int)(gid / per_type);
     const long long rem = gid - (long long)t * per_type;
     const int tile = (int)(rem / 64), lane = (int)(rem % 64);
-    const int i = (tile % ti) * 8 + lane % 8, j = (tile / ti) * 8 + lane / 8;
+    const int sb = tile / 64, wi = tile % 64;
+    const int tx = (sb % sti) * 8 + wi % 8, ty = (sb / sti) * 8 + wi / 8;
+    if (tx >= ti || ty >= tj)
+        return false;
+    const int i = tx * 8 + lane % 8, j = ty * 8 + lane / 8;
     if (i >= ni || j >= nj)
         return false;
     node = j * ni + i;
@@ -520,7 +528,7 @@ __global__ void __launch_bounds__(256) loc_compact_kernel(long long nitem, int n
 // (thread_node), rounded up to a multiple of 8 runs
 inline dim3 walk_grid(int ni, int nj, int ntypes)
 {
-    const long long n = (long long)ntypes * ((ni + 7) / 8) * ((nj + 7) / 8);
+    const long long n = (long long)ntypes * (((ni + 7) / 8 + 7) / 8) * (((nj + 7) / 8 + 7) / 8) * 64;
     return dim3((unsigned)(8 * kWalkRun * ((n + 8 * kWalkRun - 1) / (8 * kWalkRun))));
 }
 
