@@ -43,7 +43,7 @@ def profiled_traffic(workload_tag, kernel):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == "`%s`" % kernel:
+        if len(cells) == 5 and cells[0].startswith("`%s`" % kernel):
             try:
                 return float(cells[4]) * 1e9
             except ValueError:
@@ -238,11 +238,9 @@ def main():
         roof = upd_roof
     elif dom == "gram":
         ach = F_gram / (phases[1] * 1e-3)
-        # the item kernel with the tile groups the autotuner chose for the
-        # tile's largest bucket
-        big = max(R["tuning"], key=lambda b: b["nodes"])
-        kname = ("gram_item_kernel<%d, %d>" % ((m + 15) // 16, big["groups_fp64"] or 3) if m <= 96
-                 else "gram_kernel<%d>" % ((m + 15) // 16))
+        # the item kernel in every tile shape the autotuner chose (per
+        # obs-count bucket): the FP64 Gram phase is their launches together
+        kname = ("gram_item_kernel<%d, *>" % ((m + 15) // 16) if m <= 96 else "gram_kernel<%d>" % ((m + 15) // 16))
         nl = R["launches"]
         roof = dict(bound="mfma", achieved=ach / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s", frac=ach / PEAK_FP64,
                     traffic=profiled_traffic(tag, kname), kernel=kname + " (FP64 MFMA)",

@@ -1097,23 +1097,21 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
             a.tile_groups = plain_groups;
             HIPCHECK(c, mixed ? launch_gram_mixed(c->stream, a) : launch_gram(c->stream, a));
         } else if (items && tg[bk] == 0 && a.nbatch >= kTuneMin) {
-            float best = 0.0f;
-            int pick = 0;
+            // the default shape first; the other is kept only when it is
+            // clearly faster (3%), so that run-to-run noise does not flip
+            // the choice
+            const int cands[2] = {tg_default, 5 - tg_default};
+            float ms[2] = {0.0f, 0.0f};
             hipEvent_t t0 = c->bev[4 * (size_t)nb], t1 = c->bev[4 * (size_t)nb + 1];
-            for (int cand : {3, 2}) {
-                a.tile_groups = cand;
+            for (int k = 0; k < 2; ++k) {
+                a.tile_groups = cands[k];
                 HIPCHECK(c, hipEventRecord(t0, c->stream));
                 HIPCHECK(c, mixed ? launch_gram_mixed(c->stream, a) : launch_gram(c->stream, a));
                 HIPCHECK(c, hipEventRecord(t1, c->stream));
                 HIPCHECK(c, hipEventSynchronize(t1));
-                float ms = 0.0f;
-                HIPCHECK(c, hipEventElapsedTime(&ms, t0, t1));
-                if (pick == 0 || ms < best) {
-                    best = ms;
-                    pick = cand;
-                }
+                HIPCHECK(c, hipEventElapsedTime(&ms[k], t0, t1));
             }
-            tg[bk] = pick;
+            tg[bk] = (ms[1] < 0.97f * ms[0]) ? cands[1] : cands[0];
         } else {
             a.tile_groups = tg[bk];
             HIPCHECK(c, mixed ? launch_gram_mixed(c->stream, a) : launch_gram(c->stream, a));
