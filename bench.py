@@ -99,8 +99,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        log("bench.py: this build runs on one GPU; the path shards by stride-row bands "
-            "(see DESIGN.md), but no multi-GPU run is part of it.  Use --gpus 1.")
+        log("bench.py measures one GPU: the path would shard by stride-row bands balanced by local "
+            "obs counts, but only the one-GPU run is measured.  Use --gpus 1.")
         sys.exit(2)
 
     import numpy as np

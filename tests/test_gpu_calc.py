@@ -3,7 +3,7 @@ diagnostics) against the CPU oracle, through the C ABI.
 
 Tolerances: local lists (ids and taper coefficients) are compared bit for
 bit; pre-cast FP64 T and w within 1e-10 relative to max|T|, max|w| (the
-BASELINE plain-FP64 bar); the FP32 outputs within 2 ulp of the oracle's cast;
+bar set for the plain FP64 path); the FP32 outputs within 2 ulp of the oracle's cast;
 dfs/srf (stored as float by the reference) within 1e-6 relative.
 """
 import numpy as np
