@@ -109,6 +109,8 @@ struct enkfg_ctx {
 
     // localisation
     int nnode = 0;
+    std::vector<double> node_lon, node_lat;   // node grid whose coordinates are on the device
+    int node_geo = 0;
     DevBuf d_qxyz, d_glob, d_vis, d_counts, d_offsets, d_scantmp, d_ids, d_lc;
     DevBuf d_raw, d_rawoff, d_rrow, d_rd2, d_srow, d_sd2;   // NLOBSMAX candidates
     // one-walk localisation: chunk pool (capacity from the last two-walk
@@ -280,6 +282,18 @@ int node_inputs(enkfg_ctx* c, const enkfg_nodes* nodes)
     c->ni_s = nodes->ni;
     c->nj_s = nodes->nj;
     c->stride = nodes->stride;
+    const int nt = (int)c->types.size();
+    if (nodes->visible != nullptr) {
+        HIPCHECK(c, c->d_vis.reserve(std::max(nt, 1)));
+        HIPCHECK(c, hipMemcpy(c->d_vis.p, nodes->visible, nt, hipMemcpyHostToDevice));
+    }
+    // the same node grid as the last call (bit for bit): its coordinates are
+    // on the device already
+    if (c->node_lon.size() == (size_t)nn && c->node_geo == nodes->geographic &&
+        std::memcmp(c->node_lon.data(), nodes->lon, (size_t)nn * sizeof(double)) == 0 &&
+        std::memcmp(c->node_lat.data(), nodes->lat, (size_t)nn * sizeof(double)) == 0)
+        return ENKFG_OK;
+    c->node_lon.clear();   // valid again only after the upload below
     std::vector<double> q(3 * (size_t)nn);
     std::vector<unsigned char> g(nn);
     for (long long k = 0; k < nn; ++k) {
@@ -299,11 +313,9 @@ int node_inputs(enkfg_ctx* c, const enkfg_nodes* nodes)
     HIPCHECK(c, c->d_glob.reserve(nn));
     HIPCHECK(c, hipMemcpy(c->d_qxyz.p, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(c->d_glob.p, g.data(), nn, hipMemcpyHostToDevice));
-    const int nt = (int)c->types.size();
-    if (nodes->visible != nullptr) {
-        HIPCHECK(c, c->d_vis.reserve(std::max(nt, 1)));
-        HIPCHECK(c, hipMemcpy(c->d_vis.p, nodes->visible, nt, hipMemcpyHostToDevice));
-    }
+    c->node_lon.assign(nodes->lon, nodes->lon + nn);
+    c->node_lat.assign(nodes->lat, nodes->lat + nn);
+    c->node_geo = nodes->geographic;
     return ENKFG_OK;
 }
 
