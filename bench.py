@@ -171,11 +171,11 @@ def main():
         K = args.steps
         st = E.stats()
         nl = E.launches()
+        tuning = E.tuning()   # bucket counts of the timed tile calc
         # transforms of the comparison band (after the timed region)
         E.calc_transforms(*sample, stride=X.stride)
         Tb, wb, _, _ = E.transforms()
         pnb, pdb, _ = E.diag()
-        tuning = E.tuning()
         E.close()
         return dict(calc_rate=n_an / (tcalc / K), upd_rate=nfield * ncell * md / (tupd / K), phases=phases / K,
                     tcalc=tcalc / K, tupd=tupd / K, wall=wall, stats=st, launches=nl, t_tree=t_tree,
