@@ -1,10 +1,11 @@
 // factor_wave.hip -- the per-column factorisation of the DEnKF and EnOI
 // analyses by unblocked Gauss-Jordan elimination.
 //
-// A work item is one symmetric positive definite M = I + G0 of a stride-grid
-// node: slot 0 the node's full Gram matrix (transforms, w, total
-// diagnostics), slot t + 1 the Gram matrix of obstype t alone (per-type
-// diagnostics, calc/transforms.c:843-879).  The 256 lanes of the workgroup
+// A work item of factor_wave_kernel is the symmetric positive definite
+// M = I + G0 of a stride-grid node, G0 its full Gram matrix (transforms, w,
+// total diagnostics).  The per-obstype diagnostics (calc/transforms.c:843-879),
+// which need only tr(M_t^-1) and tr(G_t), are factor_trace_kernel's work
+// items, one per (node, obstype).  The 256 lanes of the workgroup
 // form a 16 x 16 grid; lane (lr, lc) holds the MT x MT block of rows lr MT..,
 // columns lc MT.. of the padded 16 MT x 16 MT matrix in registers (padding
 // kept as identity, which every step leaves intact): 36 doubles per lane for
@@ -62,9 +63,7 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
     const int lane = threadIdx.x;
     const int lr = lane >> 4, lc = lane & 15;
     const int nt = a.ntypes;
-    const int nslot = a.pertype ? nt + 1 : 1;
-    const int bnode = (int)(blockIdx.x / nslot);
-    const int slot = (int)(blockIdx.x - (unsigned)bnode * nslot);
+    const int bnode = (int)blockIdx.x;
     if (bnode >= a.nbatch)
         return;
     const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
@@ -81,45 +80,36 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
         }
         return;
     }
-    if (slot == 0 && !a.pertype && !adjoint) {
+    if (!a.pertype && !adjoint) {
         if (lane < nt) {
             a.pnlobs[(size_t)lane * a.nnode + node] = a.counts[(size_t)node * nt + lane];
             a.pdfs[(size_t)lane * a.nnode + node] = 0.0f;
             a.psrf[(size_t)lane * a.nnode + node] = 0.0f;
         }
     }
-    int pslot = p;
-    if (slot > 0) {
-        pslot = a.counts[(size_t)node * nt + (slot - 1)];
-        if (lane == 0)
-            a.pnlobs[(size_t)(slot - 1) * a.nnode + node] = pslot;
-    }
-    if (pslot == 0) {
-        const int d = (slot == 0) ? nt : slot - 1;
+    if (p == 0) {
         if (lane == 0) {
-            a.pnlobs[(size_t)d * a.nnode + node] = 0;
-            a.pdfs[(size_t)d * a.nnode + node] = 0.0f;
-            a.psrf[(size_t)d * a.nnode + node] = 0.0f;
+            a.pnlobs[(size_t)nt * a.nnode + node] = 0;
+            a.pdfs[(size_t)nt * a.nnode + node] = 0.0f;
+            a.psrf[(size_t)nt * a.nnode + node] = 0.0f;
         }
-        if (slot == 0) {
-            // empty column: T = I, w = 0 (transforms.c:766-780)
-            if (dotT)
-                for (int k = lane; k < md * m; k += 256) {
-                    const int e1 = k / m, e2 = k - e1 * m;
-                    a.T[(size_t)node * md * m + k] = (e1 == e2) ? 1.0f : 0.0f;
-                    if (a.T64 != nullptr)
-                        a.T64[(size_t)node * md * m + k] = (e1 == e2) ? 1.0 : 0.0;
-                }
-            for (int e = lane; e < m; e += 256) {
-                a.w[(size_t)node * m + e] = 0.0f;
-                if (a.w64 != nullptr)
-                    a.w64[(size_t)node * m + e] = 0.0;
+        // empty column: T = I, w = 0 (transforms.c:766-780)
+        if (dotT)
+            for (int k = lane; k < md * m; k += 256) {
+                const int e1 = k / m, e2 = k - e1 * m;
+                a.T[(size_t)node * md * m + k] = (e1 == e2) ? 1.0f : 0.0f;
+                if (a.T64 != nullptr)
+                    a.T64[(size_t)node * md * m + k] = (e1 == e2) ? 1.0 : 0.0;
             }
+        for (int e = lane; e < m; e += 256) {
+            a.w[(size_t)node * m + e] = 0.0f;
+            if (a.w64 != nullptr)
+                a.w64[(size_t)node * m + e] = 0.0;
         }
         return;
     }
 
-    // ---- M = I + G0 (G0 summed over the slot's segments), tr(G0)
+    // ---- M = I + G0 (G0 summed over the node's segments), tr(G0)
     double x[RR][RC];
 #pragma unroll
     for (int r = 0; r < RR; ++r)
@@ -128,8 +118,6 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
             x[r][c] = (lr * RR + r == lc * RC + c) ? 1.0 : 0.0;
     double trg = 0.0;
     for (int s = 0; s < a.nseg; ++s) {
-        if (slot > 0 && s != slot - 1)
-            continue;
         if (a.pertype && a.counts[(size_t)node * nt + s] == 0)
             continue;
         // this lane's rows and columns are block (lr, lc) of the gram_index
@@ -252,15 +240,6 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
     trx = group_sum(trx, sh.prow[0], lane);
     const double dfs = (double)m - trx;
 
-    if (slot > 0) {
-        if (lane == 0) {
-            const float fd = (float)dfs;
-            a.pdfs[(size_t)(slot - 1) * a.nnode + node] = fd;
-            a.psrf[(size_t)(slot - 1) * a.nnode + node] =
-                ((double)fd > kDfsMin) ? (float)(sqrt(trg / (double)fd) - 1.0) : 0.0f;
-        }
-        return;
-    }
     if (lane == 0) {
         const float fd = (float)dfs;
         a.pnlobs[(size_t)nt * a.nnode + node] = p;
@@ -326,6 +305,169 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
     }
 }
 
+
+// ---- per-obstype diagnostics: dfs_t = m - tr(M_t^-1), M_t = I + G_t
+//
+// Only the trace of the inverse is needed (calc/transforms.c:865-871), and M_t
+// is symmetric, as is every intermediate matrix of the sweep operator: pivot
+// k sets A_ij -= A_ik A_kj / d for i, j != k, A_ik = A_ik / d, A_kj = A_kj / d
+// and A_kk = -1 / d, which ends, after all pivots, with -M^-1.  So only the
+// 136 blocks on and below the diagonal of the 16 x 16 grid of MT x MT blocks
+// are kept, one per lane of a 3-wave workgroup (lanes 136-191 hold none), and
+// each pivot publishes one vector in LDS: column k, read by a block (br, bc)
+// at rows br MT.. (A_ik) and at rows bc MT.. (A_kj = A_jk).  Three waves and
+// one vector per matrix, against four waves and two vectors for the full
+// elimination.
+constexpr int kTriBlocks = 136;   // 16 * 17 / 2
+
+// sum over the 192 lanes (three waves), in order through `tmp`
+__device__ inline double group_sum3(double v, double* tmp, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1)
+        v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((lane & 63) == 0)
+        tmp[lane >> 6] = v;
+    __syncthreads();
+    const double s = (tmp[0] + tmp[1]) + tmp[2];
+    __syncthreads();
+    return s;
+}
+
+template <int MT>
+__global__ void __launch_bounds__(192) factor_trace_kernel(CalcArgs a)
+{
+    constexpr int MP = 16 * MT;
+    __shared__ double col[2][MP];
+    __shared__ double tmp[4];
+    const int lane = threadIdx.x;
+    const int nt = a.ntypes;
+    const int bnode = (int)(blockIdx.x / nt);
+    const int t = (int)(blockIdx.x - (unsigned)bnode * nt);
+    if (bnode >= a.nbatch)
+        return;
+    const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
+    const int m = a.m;
+    const int pt = a.counts[(size_t)node * nt + t];
+    const size_t o = (size_t)t * a.nnode + node;
+    if (pt == 0) {
+        if (lane == 0) {
+            a.pnlobs[o] = 0;
+            a.pdfs[o] = 0.0f;
+            a.psrf[o] = 0.0f;
+        }
+        return;
+    }
+    if (lane == 0)
+        a.pnlobs[o] = pt;
+
+    // block (br, bc), br >= bc, of lane br (br + 1) / 2 + bc; lanes past the
+    // last block compute on block (0, 0)'s column data but never publish,
+    // load or count
+    const bool act = lane < kTriBlocks;
+    int br = 0;
+    if (act)
+        while ((br + 1) * (br + 2) / 2 <= lane)
+            ++br;
+    const int bc = act ? lane - br * (br + 1) / 2 : 0;
+
+    // ---- M_t = I + G_t on this lane's block, tr(G_t)
+    double x[MT][MT];
+    double trg = 0.0;
+    const double* g = a.gram + ((size_t)bnode * a.nseg + t) * MP * MP + gram_index<MT>(br * MT, bc * MT);
+#pragma unroll
+    for (int r = 0; r < MT; ++r) {
+        const int i = br * MT + r;
+#pragma unroll
+        for (int c = 0; c < MT; ++c) {
+            const int j = bc * MT + c;
+            double v = (i == j) ? 1.0 : 0.0;
+            if (act && i < m && j < m) {
+                const double gv = g[r * MT + c];
+                v += gv;
+                if (i == j)
+                    trg += gv;
+            }
+            x[r][c] = v;
+        }
+    }
+    trg = group_sum3(trg, tmp, lane);
+
+    // column k of the current matrix, from the blocks (br >= kb, bc == kb)
+    // (their column rk) and (br == kb, bc < kb) (their row rk)
+    auto publish = [&](double* dst, int kb, int rk) {
+        if (!act)
+            return;
+        if (bc == kb) {
+#pragma unroll
+            for (int r = 0; r < MT; ++r)
+                dst[br * MT + r] = x[r][rk];
+        } else if (br == kb) {
+#pragma unroll
+            for (int c = 0; c < MT; ++c)
+                dst[bc * MT + c] = x[rk][c];
+        }
+    };
+
+    bool bad = false;
+    int buf = 0;
+    publish(col[0], 0, 0);
+    for (int kb = 0; kb < 16; ++kb) {
+#pragma unroll
+        for (int rk = 0; rk < MT; ++rk) {
+            const int k = kb * MT + rk;
+            if (k >= m)
+                break;   // the remaining pivots are identity padding
+            __syncthreads();
+            const double d = col[buf][k];
+            bad |= !(d > 0.0);
+            const double pinv = 1.0 / d;
+            double u[MT], ws[MT];
+#pragma unroll
+            for (int r = 0; r < MT; ++r)
+                u[r] = col[buf][br * MT + r];             // A_ik
+#pragma unroll
+            for (int c = 0; c < MT; ++c)
+                ws[c] = col[buf][bc * MT + c] * pinv;     // A_kj / d
+#pragma unroll
+            for (int r = 0; r < MT; ++r)
+#pragma unroll
+                for (int c = 0; c < MT; ++c) {
+                    double v = fma(-u[r], ws[c], x[r][c]);
+                    if (br == kb && r == rk)
+                        v = ws[c];
+                    if (bc == kb && c == rk)
+                        v = u[r] * pinv;
+                    if (br == kb && bc == kb && r == rk && c == rk)
+                        v = -pinv;
+                    x[r][c] = v;
+                }
+            // column k + 1 into the other buffer: every wave has passed this
+            // pivot's barrier, so none still reads it
+            if (k + 1 < m)
+                publish(col[buf ^ 1], (k + 1) / MT, (rk + 1) % MT);
+            buf ^= 1;
+        }
+    }
+    if (bad)
+        atomicExch(a.err, ENKFG_ERR_LINALG);
+
+    // ---- dfs = m - tr(M^-1) = m + tr(-M^-1), srf with the DFS_MIN guard
+    double s = 0.0;
+    if (act && br == bc) {
+#pragma unroll
+        for (int r = 0; r < MT; ++r)
+            if (br * MT + r < m)
+                s += x[r][r];
+    }
+    s = group_sum3(s, tmp, lane);
+    if (lane == 0) {
+        const float fd = (float)((double)m + s);
+        a.pdfs[o] = fd;
+        a.psrf[o] = ((double)fd > kDfsMin) ? (float)(sqrt(trg / (double)fd) - 1.0) : 0.0f;
+    }
+}
+
 }  // namespace
 
 bool factor_wave_supported(const CalcArgs& a)
@@ -339,8 +481,7 @@ hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a)
     if (a.nbatch == 0)
         return hipSuccess;
     const int MT = (a.m + 15) / 16;
-    const int nslot = a.pertype ? a.ntypes + 1 : 1;
-    const dim3 grid((unsigned)((long long)a.nbatch * nslot)), block(256);
+    const dim3 grid((unsigned)a.nbatch), block(256);
     switch (MT) {
     case 1: hipLaunchKernelGGL(factor_wave_kernel<1>, grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL(factor_wave_kernel<2>, grid, block, 0, s, a); break;
@@ -348,6 +489,20 @@ hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a)
     case 4: hipLaunchKernelGGL(factor_wave_kernel<4>, grid, block, 0, s, a); break;
     case 5: hipLaunchKernelGGL(factor_wave_kernel<5>, grid, block, 0, s, a); break;
     case 6: hipLaunchKernelGGL(factor_wave_kernel<6>, grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !a.pertype || a.ntypes <= 0)
+        return e;
+    // per-obstype diagnostics, one work item per (node, obstype)
+    const dim3 tgrid((unsigned)((long long)a.nbatch * a.ntypes)), tblock(192);
+    switch (MT) {
+    case 1: hipLaunchKernelGGL(factor_trace_kernel<1>, tgrid, tblock, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(factor_trace_kernel<2>, tgrid, tblock, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(factor_trace_kernel<3>, tgrid, tblock, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(factor_trace_kernel<4>, tgrid, tblock, 0, s, a); break;
+    case 5: hipLaunchKernelGGL(factor_trace_kernel<5>, tgrid, tblock, 0, s, a); break;
+    case 6: hipLaunchKernelGGL(factor_trace_kernel<6>, tgrid, tblock, 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
