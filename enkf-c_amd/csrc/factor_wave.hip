@@ -320,6 +320,18 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
 // elimination.
 constexpr int kTriBlocks = 136;   // 16 * 17 / 2
 
+// 1/d for a pivot d >= 1 (the pivots of I + S^T S are Schur complements of
+// at least 1): the hardware estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - d r), within an ulp of 1/d
+__device__ inline double recip_pivot(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
 // sum over the 192 lanes (three waves), in order through `tmp`
 __device__ inline double group_sum3(double v, double* tmp, int lane)
 {
@@ -335,7 +347,7 @@ __device__ inline double group_sum3(double v, double* tmp, int lane)
 }
 
 template <int MT>
-__global__ void __launch_bounds__(192) factor_trace_kernel(CalcArgs a)
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) factor_trace_kernel(CalcArgs a)
 {
     constexpr int MP = 16 * MT;
     __shared__ double col[2][MP];
@@ -421,7 +433,7 @@ __global__ void __launch_bounds__(192) factor_trace_kernel(CalcArgs a)
             __syncthreads();
             const double d = col[buf][k];
             bad |= !(d > 0.0);
-            const double pinv = 1.0 / d;
+            const double pinv = recip_pivot(d);
             double u[MT], ws[MT];
 #pragma unroll
             for (int r = 0; r < MT; ++r)
@@ -429,19 +441,20 @@ __global__ void __launch_bounds__(192) factor_trace_kernel(CalcArgs a)
 #pragma unroll
             for (int c = 0; c < MT; ++c)
                 ws[c] = col[buf][bc * MT + c] * pinv;     // A_kj / d
+            // row k and column k by the same rank-1 update: with u_k = d - 1,
+            // A_kj - (d - 1) A_kj / d = A_kj / d; with A_kk / d taken as
+            // (d - 1) / d, A_ik - A_ik (d - 1) / d = A_ik / d; then A_kk = -1/d
+            if (br == kb)
+                u[rk] = d - 1.0;
+            if (bc == kb)
+                ws[rk] = (d - 1.0) * pinv;
 #pragma unroll
             for (int r = 0; r < MT; ++r)
 #pragma unroll
-                for (int c = 0; c < MT; ++c) {
-                    double v = fma(-u[r], ws[c], x[r][c]);
-                    if (br == kb && r == rk)
-                        v = ws[c];
-                    if (bc == kb && c == rk)
-                        v = u[r] * pinv;
-                    if (br == kb && bc == kb && r == rk && c == rk)
-                        v = -pinv;
-                    x[r][c] = v;
-                }
+                for (int c = 0; c < MT; ++c)
+                    x[r][c] = fma(-u[r], ws[c], x[r][c]);
+            if (br == kb && bc == kb)
+                x[rk][rk] = -pinv;
             // column k + 1 into the other buffer: every wave has passed this
             // pivot's barrier, so none still reads it
             if (k + 1 < m)
