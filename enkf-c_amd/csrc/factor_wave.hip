@@ -481,6 +481,250 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
     }
 }
 
+
+// ---- a node's own matrix by the same symmetric sweep (no adjoint)
+//
+// As factor_trace_kernel, on the 136 lower-triangle blocks of M = I + G0 (G0
+// summed over the node's segments), but with the pivot's row, column and
+// diagonal set explicitly and an IEEE division, so that M^-1 keeps the
+// accuracy of the Gauss-Jordan elimination.  Then, with X = -M^-1 in the
+// blocks:
+//   T   = (I + M^-1) / 2 on rows < m_d, relaxed and cast (transforms.c:814-842):
+//         each block written as itself and, off the diagonal, transposed;
+//   w   = M^-1 S^T s: each block's row products and, off the diagonal, its
+//         transpose's, staged in LDS by (row block, other block) and summed in
+//         block order by one lane per row;
+//   dfs = m - tr(M^-1), srf = sqrt(tr(G0) / dfs) - 1.
+template <int MT>
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) factor_sym_kernel(CalcArgs a)
+{
+    constexpr int MP = 16 * MT;
+    __shared__ double col[2][MP];
+    __shared__ double part[16][16][MT];   // [row block][other block][row in block]
+    __shared__ double vs[MP];             // S^T s over the node's segments
+    __shared__ double tmp[4];
+    const int lane = threadIdx.x;
+    const int nt = a.ntypes;
+    const int bnode = (int)blockIdx.x;
+    if (bnode >= a.nbatch)
+        return;
+    const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
+    const int m = a.m, md = a.md;
+    const bool dotT = (a.mode == ENKFG_MODE_ENKF || a.mode == ENKFG_MODE_HYBRID);
+    const long long off = a.offsets[(size_t)node * nt];
+    const int p = (int)(a.offsets[(size_t)(node + 1) * nt] - off);
+    if (!a.pertype) {
+        if (lane < nt) {
+            a.pnlobs[(size_t)lane * a.nnode + node] = a.counts[(size_t)node * nt + lane];
+            a.pdfs[(size_t)lane * a.nnode + node] = 0.0f;
+            a.psrf[(size_t)lane * a.nnode + node] = 0.0f;
+        }
+    }
+    if (p == 0) {
+        if (lane == 0) {
+            a.pnlobs[(size_t)nt * a.nnode + node] = 0;
+            a.pdfs[(size_t)nt * a.nnode + node] = 0.0f;
+            a.psrf[(size_t)nt * a.nnode + node] = 0.0f;
+        }
+        // empty column: T = I, w = 0 (transforms.c:766-780)
+        if (dotT)
+            for (int k = lane; k < md * m; k += 192) {
+                const int e1 = k / m, e2 = k - e1 * m;
+                a.T[(size_t)node * md * m + k] = (e1 == e2) ? 1.0f : 0.0f;
+                if (a.T64 != nullptr)
+                    a.T64[(size_t)node * md * m + k] = (e1 == e2) ? 1.0 : 0.0;
+            }
+        for (int e = lane; e < m; e += 192) {
+            a.w[(size_t)node * m + e] = 0.0f;
+            if (a.w64 != nullptr)
+                a.w64[(size_t)node * m + e] = 0.0;
+        }
+        return;
+    }
+
+    const bool act = lane < kTriBlocks;
+    int br = 0;
+    if (act)
+        while ((br + 1) * (br + 2) / 2 <= lane)
+            ++br;
+    const int bc = act ? lane - br * (br + 1) / 2 : 0;
+
+    // ---- M = I + G0 on this lane's block, tr(G0)
+    double x[MT][MT];
+#pragma unroll
+    for (int r = 0; r < MT; ++r)
+#pragma unroll
+        for (int c = 0; c < MT; ++c)
+            x[r][c] = (br * MT + r == bc * MT + c) ? 1.0 : 0.0;
+    double trg = 0.0;
+    for (int sg = 0; sg < a.nseg; ++sg) {
+        if (a.pertype && a.counts[(size_t)node * nt + sg] == 0)
+            continue;
+        const double* g = a.gram + ((size_t)bnode * a.nseg + sg) * MP * MP + gram_index<MT>(br * MT, bc * MT);
+#pragma unroll
+        for (int r = 0; r < MT; ++r) {
+            const int i = br * MT + r;
+#pragma unroll
+            for (int c = 0; c < MT; ++c) {
+                const int j = bc * MT + c;
+                if (act && i < m && j < m) {
+                    const double v = g[r * MT + c];
+                    x[r][c] += v;
+                    if (i == j)
+                        trg += v;
+                }
+            }
+        }
+    }
+    trg = group_sum3(trg, tmp, lane);
+
+    auto publish = [&](double* dst, int kb, int rk) {
+        if (!act)
+            return;
+        if (bc == kb) {
+#pragma unroll
+            for (int r = 0; r < MT; ++r)
+                dst[br * MT + r] = x[r][rk];
+        } else if (br == kb) {
+#pragma unroll
+            for (int c = 0; c < MT; ++c)
+                dst[bc * MT + c] = x[rk][c];
+        }
+    };
+
+    // ---- the sweep: X <- -M^-1
+    bool bad = false;
+    int buf = 0;
+    publish(col[0], 0, 0);
+    for (int kb = 0; kb < 16; ++kb) {
+#pragma unroll
+        for (int rk = 0; rk < MT; ++rk) {
+            const int k = kb * MT + rk;
+            if (k >= m)
+                break;   // the remaining pivots are identity padding
+            __syncthreads();
+            const double d = col[buf][k];
+            bad |= !(d > 0.0);
+            const double pinv = 1.0 / d;
+            double u[MT], ws[MT];
+#pragma unroll
+            for (int r = 0; r < MT; ++r)
+                u[r] = col[buf][br * MT + r];             // A_ik
+#pragma unroll
+            for (int c = 0; c < MT; ++c)
+                ws[c] = col[buf][bc * MT + c] * pinv;     // A_kj / d
+#pragma unroll
+            for (int r = 0; r < MT; ++r)
+#pragma unroll
+                for (int c = 0; c < MT; ++c) {
+                    double v = fma(-u[r], ws[c], x[r][c]);
+                    if (br == kb && r == rk)
+                        v = ws[c];                          // A_kj / d
+                    if (bc == kb && c == rk)
+                        v = u[r] * pinv;                    // A_ik / d
+                    if (br == kb && bc == kb && r == rk && c == rk)
+                        v = -pinv;
+                    x[r][c] = v;
+                }
+            if (k + 1 < m)
+                publish(col[buf ^ 1], (k + 1) / MT, (rk + 1) % MT);
+            buf ^= 1;
+        }
+    }
+    if (bad)
+        atomicExch(a.err, ENKFG_ERR_LINALG);
+
+    // S^T s into LDS (ordered before its reads by group_sum3's barriers)
+    if (lane < MP) {
+        double v = 0.0;
+        if (lane < m)
+            for (int sg = 0; sg < a.nseg; ++sg) {
+                if (a.pertype && a.counts[(size_t)node * nt + sg] == 0)
+                    continue;
+                v += a.sts[((size_t)bnode * a.nseg + sg) * MP + lane];
+            }
+        vs[lane] = v;
+    }
+
+    // ---- dfs = m - tr(M^-1) = m + tr(X), srf
+    double sd = 0.0;
+    if (act && br == bc) {
+#pragma unroll
+        for (int r = 0; r < MT; ++r)
+            if (br * MT + r < m)
+                sd += x[r][r];
+    }
+    sd = group_sum3(sd, tmp, lane);
+    if (lane == 0) {
+        const float fd = (float)((double)m + sd);
+        a.pnlobs[(size_t)nt * a.nnode + node] = p;
+        a.pdfs[(size_t)nt * a.nnode + node] = fd;
+        a.psrf[(size_t)nt * a.nnode + node] = (float)(sqrt(trg / (double)fd) - 1.0);
+    }
+
+    // ---- w = M^-1 S^T s
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < MT; ++r) {
+            double y = 0.0;
+#pragma unroll
+            for (int c = 0; c < MT; ++c)
+                y = fma(-x[r][c], vs[bc * MT + c], y);
+            part[br][bc][r] = y;
+        }
+        if (br != bc) {
+#pragma unroll
+            for (int c = 0; c < MT; ++c) {
+                double y = 0.0;
+#pragma unroll
+                for (int r = 0; r < MT; ++r)
+                    y = fma(-x[r][c], vs[br * MT + r], y);
+                part[bc][br][c] = y;
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < m) {
+        const int rb = lane / MT, rr = lane - rb * MT;
+        double y = 0.0;
+        for (int sl = 0; sl < 16; ++sl)
+            y += part[rb][sl][rr];
+        a.w[(size_t)node * m + lane] = (float)y;
+        if (a.w64 != nullptr)
+            a.w64[(size_t)node * m + lane] = y;
+    }
+
+    // ---- DEnKF T = (I + M^-1) / 2, relaxation T <- (T - I) alpha + I, FP32
+    if (dotT && act) {
+#pragma unroll
+        for (int r = 0; r < MT; ++r)
+#pragma unroll
+            for (int c = 0; c < MT; ++c) {
+                const double mi = -x[r][c];
+                int e1 = br * MT + r, e2 = bc * MT + c;
+                if (e1 < md && e2 < m) {
+                    const double dlt = (e1 == e2) ? 1.0 : 0.0;
+                    double t = 0.5 * mi + 0.5 * dlt;
+                    t = (t - dlt) * a.alpha + dlt;
+                    const size_t o = ((size_t)node * md + e1) * m + e2;
+                    a.T[o] = (float)t;
+                    if (a.T64 != nullptr)
+                        a.T64[o] = t;
+                }
+                e1 = bc * MT + c;
+                e2 = br * MT + r;
+                if (br != bc && e1 < md && e2 < m) {
+                    double t = 0.5 * mi + 0.5 * 0.0;
+                    t = (t - 0.0) * a.alpha + 0.0;
+                    const size_t o = ((size_t)node * md + e1) * m + e2;
+                    a.T[o] = (float)t;
+                    if (a.T64 != nullptr)
+                        a.T64[o] = t;
+                }
+            }
+    }
+}
+
 }  // namespace
 
 bool factor_wave_supported(const CalcArgs& a)
@@ -494,15 +738,28 @@ hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a)
     if (a.nbatch == 0)
         return hipSuccess;
     const int MT = (a.m + 15) / 16;
-    const dim3 grid((unsigned)a.nbatch), block(256);
-    switch (MT) {
-    case 1: hipLaunchKernelGGL(factor_wave_kernel<1>, grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(factor_wave_kernel<2>, grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(factor_wave_kernel<3>, grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL(factor_wave_kernel<4>, grid, block, 0, s, a); break;
-    case 5: hipLaunchKernelGGL(factor_wave_kernel<5>, grid, block, 0, s, a); break;
-    case 6: hipLaunchKernelGGL(factor_wave_kernel<6>, grid, block, 0, s, a); break;
-    default: return hipErrorInvalidValue;
+    const dim3 grid((unsigned)a.nbatch);
+    if (a.adj_g != nullptr) {
+        // adjoint: z = M^-1 g and w by the Gauss-Jordan elimination
+        switch (MT) {
+        case 1: hipLaunchKernelGGL(factor_wave_kernel<1>, grid, dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(factor_wave_kernel<2>, grid, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(factor_wave_kernel<3>, grid, dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(factor_wave_kernel<4>, grid, dim3(256), 0, s, a); break;
+        case 5: hipLaunchKernelGGL(factor_wave_kernel<5>, grid, dim3(256), 0, s, a); break;
+        case 6: hipLaunchKernelGGL(factor_wave_kernel<6>, grid, dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (MT) {
+        case 1: hipLaunchKernelGGL(factor_sym_kernel<1>, grid, dim3(192), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(factor_sym_kernel<2>, grid, dim3(192), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(factor_sym_kernel<3>, grid, dim3(192), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(factor_sym_kernel<4>, grid, dim3(192), 0, s, a); break;
+        case 5: hipLaunchKernelGGL(factor_sym_kernel<5>, grid, dim3(192), 0, s, a); break;
+        case 6: hipLaunchKernelGGL(factor_sym_kernel<6>, grid, dim3(192), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !a.pertype || a.ntypes <= 0)

@@ -275,6 +275,32 @@ def test_calc_transforms_parity(mode, scheme, m, md, alpha, profiles):
     E.close()
 
 
+
+def test_calc_without_pertype_diagnostics():
+    """ENKFG_FLAG_PERTYPE_DIAG off: the per-type rows of the diagnostics hold
+    the local obs counts with zero dfs and srf; T, w and the all-types row
+    match the oracle as with the flag on."""
+    m = 96
+    print("case calc DEnKF m=96 per-type diagnostics off", flush=True)
+    P = problem(seed=7, nmem=m, profiles=30, nobs=2500, locrad=75.0, nj=45, ni=60)
+    ob = oracle_obs(P)
+    S, s = O.standardise(m, ob, P.HA, P.innov)
+    ref = O.calc_transforms(enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, m, m, 1.0, ob, S, s, P.node_lon.ravel(),
+                            P.node_lat.ravel())
+    E = engine_for(P, enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, pertype=False)
+    E.set_S(P.HA, P.innov, standardised=False)
+    E.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
+    T, w, T64, w64 = E.transforms(fp64=True)
+    pn, pd, ps = E.diag()
+    nt = len(P.obstypes)
+    assert relerr(w64, ref["w64"]) < 1e-10 and relerr(T64, ref["T64"]) < 1e-10
+    assert np.array_equal(pn, ref["pnlobs"])
+    assert not np.any(pd[:nt]) and not np.any(ps[:nt])
+    assert np.allclose(pd[nt], ref["pdfs"][nt], rtol=1e-6, atol=1e-7)
+    nz = ref["pdfs"][nt] != 0
+    assert np.allclose(ps[nt][nz], ref["psrf"][nt][nz], rtol=1e-5, atol=1e-6)
+    E.close()
+
 def test_calc_empty_columns_and_stats():
     print("case calc empty columns", flush=True)
     P = problem(seed=11, nobs=40, locrad=20.0, nj=30, ni=30)
