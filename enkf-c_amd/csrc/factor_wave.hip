@@ -485,23 +485,22 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
 // ---- a node's own matrix by the same symmetric sweep (no adjoint)
 //
 // As factor_trace_kernel, on the 136 lower-triangle blocks of M = I + G0 (G0
-// summed over the node's segments), but with the pivot's row, column and
-// diagonal set explicitly and an IEEE division, so that M^-1 keeps the
-// accuracy of the Gauss-Jordan elimination.  Then, with X = -M^-1 in the
-// blocks:
-//   T   = (I + M^-1) / 2 on rows < m_d, relaxed and cast (transforms.c:814-842):
-//         each block written as itself and, off the diagonal, transposed;
+// summed over the node's segments), with an IEEE division of the pivot.
+// Then, with X = -M^-1 in the blocks:
+//   T   = (I + M^-1) / 2 on rows < m_d, relaxed and cast (transforms.c:814-842),
+//         staged one block row at a time in LDS and written as whole rows;
 //   w   = M^-1 S^T s: each block's row products and, off the diagonal, its
 //         transpose's, staged in LDS by (row block, other block) and summed in
 //         block order by one lane per row;
 //   dfs = m - tr(M^-1), srf = sqrt(tr(G0) / dfs) - 1.
 template <int MT>
-__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) factor_sym_kernel(CalcArgs a)
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) factor_sym_kernel(CalcArgs a)
 {
     constexpr int MP = 16 * MT;
     __shared__ double col[2][MP];
     __shared__ double part[16][16][MT];   // [row block][other block][row in block]
     __shared__ double vs[MP];             // S^T s over the node's segments
+    __shared__ double slab[MT][MP];       // MT rows of M^-1 for the T output
     __shared__ double tmp[4];
     const int lane = threadIdx.x;
     const int nt = a.ntypes;
@@ -557,22 +556,27 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) f
         for (int c = 0; c < MT; ++c)
             x[r][c] = (br * MT + r == bc * MT + c) ? 1.0 : 0.0;
     double trg = 0.0;
-    for (int sg = 0; sg < a.nseg; ++sg) {
-        if (a.pertype && a.counts[(size_t)node * nt + sg] == 0)
-            continue;
-        const double* g = a.gram + ((size_t)bnode * a.nseg + sg) * MP * MP + gram_index<MT>(br * MT, bc * MT);
+    // every lane reads whole blocks of the padded scratch (lanes past the
+    // last block read block (0, 0)); the entries outside m x m, or of those
+    // lanes, are selected away.  Block rows outer, segments inner (each
+    // element sums its segments in order), so that one row of loads is in
+    // flight beside the accumulators.
+    const double* gb = a.gram + (size_t)bnode * a.nseg * MP * MP + gram_index<MT>(br * MT, bc * MT);
 #pragma unroll
-        for (int r = 0; r < MT; ++r) {
-            const int i = br * MT + r;
+    for (int r = 0; r < MT; ++r) {
+        const int i = br * MT + r;
+        for (int sg = 0; sg < a.nseg; ++sg) {
+            if (a.pertype && a.counts[(size_t)node * nt + sg] == 0)
+                continue;
+            const double* g = gb + (size_t)sg * MP * MP + r * MT;
 #pragma unroll
             for (int c = 0; c < MT; ++c) {
                 const int j = bc * MT + c;
-                if (act && i < m && j < m) {
-                    const double v = g[r * MT + c];
-                    x[r][c] += v;
-                    if (i == j)
-                        trg += v;
-                }
+                const double v = g[c];
+                const bool in = act && i < m && j < m;
+                x[r][c] += in ? v : 0.0;
+                if (r == c)
+                    trg += (in && i == j) ? v : 0.0;
             }
         }
     }
@@ -613,19 +617,20 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) f
 #pragma unroll
             for (int c = 0; c < MT; ++c)
                 ws[c] = col[buf][bc * MT + c] * pinv;     // A_kj / d
+            // row and column k folded into the rank-1 update as in
+            // factor_trace_kernel (about d eps of extra absolute rounding in
+            // them); the diagonal, whose folded form would lose d^2 eps, is set
+            if (br == kb)
+                u[rk] = d - 1.0;
+            if (bc == kb)
+                ws[rk] = (d - 1.0) * pinv;
 #pragma unroll
             for (int r = 0; r < MT; ++r)
 #pragma unroll
-                for (int c = 0; c < MT; ++c) {
-                    double v = fma(-u[r], ws[c], x[r][c]);
-                    if (br == kb && r == rk)
-                        v = ws[c];                          // A_kj / d
-                    if (bc == kb && c == rk)
-                        v = u[r] * pinv;                    // A_ik / d
-                    if (br == kb && bc == kb && r == rk && c == rk)
-                        v = -pinv;
-                    x[r][c] = v;
-                }
+                for (int c = 0; c < MT; ++c)
+                    x[r][c] = fma(-u[r], ws[c], x[r][c]);
+            if (br == kb && bc == kb)
+                x[rk][rk] = -pinv;
             if (k + 1 < m)
                 publish(col[buf ^ 1], (k + 1) / MT, (rk + 1) % MT);
             buf ^= 1;
@@ -694,34 +699,41 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) f
             a.w64[(size_t)node * m + lane] = y;
     }
 
-    // ---- DEnKF T = (I + M^-1) / 2, relaxation T <- (T - I) alpha + I, FP32
-    if (dotT && act) {
+    // ---- DEnKF T = (I + M^-1) / 2, relaxation T <- (T - I) alpha + I, FP32:
+    // M^-1 one block row at a time through LDS (rows R MT.., from the blocks
+    // (R, bc) and, transposed, (br, R)), then written as whole rows of T
+    if (dotT) {
+        for (int R = 0; R < 16 && R * MT < md; ++R) {
+            __syncthreads();   // the previous slab has been written out
+            if (act && br == R) {
 #pragma unroll
-        for (int r = 0; r < MT; ++r)
+                for (int r = 0; r < MT; ++r)
 #pragma unroll
-            for (int c = 0; c < MT; ++c) {
-                const double mi = -x[r][c];
-                int e1 = br * MT + r, e2 = bc * MT + c;
-                if (e1 < md && e2 < m) {
+                    for (int c = 0; c < MT; ++c)
+                        slab[r][bc * MT + c] = -x[r][c];
+            }
+            if (act && bc == R && br != R) {
+#pragma unroll
+                for (int c = 0; c < MT; ++c)
+#pragma unroll
+                    for (int r = 0; r < MT; ++r)
+                        slab[c][br * MT + r] = -x[r][c];
+            }
+            __syncthreads();
+            for (int k = lane; k < MT * m; k += 192) {
+                const int rr = k / m, e2 = k - rr * m;
+                const int e1 = R * MT + rr;
+                if (e1 < md) {
                     const double dlt = (e1 == e2) ? 1.0 : 0.0;
-                    double t = 0.5 * mi + 0.5 * dlt;
+                    double t = 0.5 * slab[rr][e2] + 0.5 * dlt;
                     t = (t - dlt) * a.alpha + dlt;
                     const size_t o = ((size_t)node * md + e1) * m + e2;
                     a.T[o] = (float)t;
                     if (a.T64 != nullptr)
                         a.T64[o] = t;
                 }
-                e1 = bc * MT + c;
-                e2 = br * MT + r;
-                if (br != bc && e1 < md && e2 < m) {
-                    double t = 0.5 * mi + 0.5 * 0.0;
-                    t = (t - 0.0) * a.alpha + 0.0;
-                    const size_t o = ((size_t)node * md + e1) * m + e2;
-                    a.T[o] = (float)t;
-                    if (a.T64 != nullptr)
-                        a.T64[o] = t;
-                }
             }
+        }
     }
 }
 
