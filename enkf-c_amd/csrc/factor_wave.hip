@@ -422,7 +422,6 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
     };
 
     bool bad = false;
-    int buf = 0;
     publish(col[0], 0, 0);
     for (int kb = 0; kb < 16; ++kb) {
 #pragma unroll
@@ -431,6 +430,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
             if (k >= m)
                 break;   // the remaining pivots are identity padding
             __syncthreads();
+            const int buf = k & 1;   // pivot k reads buffer k % 2 (a constant per unrolled step for even MT)
             const double d = col[buf][k];
             bad |= !(d > 0.0);
             const double pinv = recip_pivot(d);
@@ -459,7 +459,6 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
             // pivot's barrier, so none still reads it
             if (k + 1 < m)
                 publish(col[buf ^ 1], (k + 1) / MT, (rk + 1) % MT);
-            buf ^= 1;
         }
     }
     if (bad)
@@ -598,7 +597,6 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
 
     // ---- the sweep: X <- -M^-1
     bool bad = false;
-    int buf = 0;
     publish(col[0], 0, 0);
     for (int kb = 0; kb < 16; ++kb) {
 #pragma unroll
@@ -607,6 +605,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
             if (k >= m)
                 break;   // the remaining pivots are identity padding
             __syncthreads();
+            const int buf = k & 1;   // pivot k reads buffer k % 2 (a constant per unrolled step for even MT)
             const double d = col[buf][k];
             bad |= !(d > 0.0);
             const double pinv = 1.0 / d;
@@ -633,7 +632,6 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
                 x[rk][rk] = -pinv;
             if (k + 1 < m)
                 publish(col[buf ^ 1], (k + 1) / MT, (rk + 1) % MT);
-            buf ^= 1;
         }
     }
     if (bad)
