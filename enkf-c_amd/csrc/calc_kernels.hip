@@ -914,6 +914,92 @@ __device__ inline void write_T(const CalcArgs& a, int node, const double (&x)[B]
     }
 }
 
+// The scaled coupled Denman-Beavers iteration of factor_kernel for B > 6,
+// where Y, Z and Y^-1 do not fit in registers beside x: they live in the
+// batch node's slice of a.etkf_ws, each thread's B x B block at
+// ws[(k * B * B + r * B + c) * 256 + tid] for matrix k (0 Y, 1 Z, 2 Y^-1),
+// and only the matrix being inverted or combined is held in x.  On entry x
+// holds M^-1 = Y0^-1 and logdetA = log det M.  Writes T = Z (-> M^-1/2).
+template <int B>
+__device__ inline void etkf_ws_iterate(const CalcArgs& a, int bnode, int node, double logdetA, double (&x)[B][B],
+                                       FactorShared<B>& sh)
+{
+    const int tid = threadIdx.x;
+    const int ti = tid >> 4, tj = tid & 15;
+    const int m = a.m;
+    double* ws = a.etkf_ws + (size_t)bnode * etkf_ws_per_node(m);
+    auto at = [&](int k, int r, int c) -> double& { return ws[((size_t)(k * B * B + r * B + c)) * 256 + tid]; };
+    // Y^-1 = x; Y = M (= I + G0); Z = I
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            at(2, r, c) = x[r][c];
+            at(1, r, c) = (ti * B + r == tj * B + c) ? 1.0 : 0.0;
+        }
+    load_gram<B>(a, bnode, -1, 1.0, x, sh);
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            at(0, r, c) = x[r][c];
+    double logdetY = logdetA, logdetZ = 0.0;
+    bool last = false;
+    for (int it = 0; it < 60; ++it) {
+        if (it > 0) {
+            // Y^-1
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c)
+                    x[r][c] = at(0, r, c);
+            logdetY = gj_invert<B, true>(x, m, sh);
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c)
+                    at(2, r, c) = x[r][c];
+        }
+        // x = Z^-1 (Z = I at the first step)
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+                x[r][c] = at(1, r, c);
+        if (it > 0)
+            logdetZ = gj_invert<B, true>(x, m, sh);
+        const double mu = last ? 1.0 : exp(-(logdetY + logdetZ) / (2.0 * m));
+        const double hm = 0.5 * mu, hi = 0.5 / mu;
+        double dn = 0.0, nn = 0.0;
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const double y = at(0, r, c), z = at(1, r, c);
+                const double yn = hm * y + hi * x[r][c];
+                const double zn = hm * z + hi * at(2, r, c);
+                dn += (zn - z) * (zn - z);
+                nn += zn * zn;
+                at(0, r, c) = yn;
+                at(1, r, c) = zn;
+            }
+        dn = block_sum(dn, sh);
+        nn = block_sum(nn, sh);
+        if (last)
+            break;
+        if (dn <= 1e-20 * nn)
+            last = true;   // converged quadratically: one more unscaled step
+    }
+    if (!last && tid == 0)
+        sh.flag = 1;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            x[r][c] = at(1, r, c);
+    write_T<B>(a, node, x, 1.0, 0.0, sh);
+}
+
 template <int B, bool ETKF>
 __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
 {
@@ -976,6 +1062,8 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
     if (dotT) {
         if constexpr (!ETKF) {
             write_T<B>(a, node, x, 0.5, 0.5, sh);
+        } else if constexpr (B > 6) {
+            etkf_ws_iterate<B>(a, bnode, node, logdetA, x, sh);
         } else {
             // scaled coupled Denman-Beavers: Y -> M^1/2, Z -> M^-1/2; x holds
             // M^-1 = Y0^-1 already.  Z lives in shared-by-owner registers.
@@ -1029,6 +1117,8 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
                 if (dn <= 1e-20 * nn)
                     last = true;   // converged quadratically: one more unscaled step
             }
+            if (!last && tid == 0)
+                sh.flag = 1;       // no convergence in 60 steps: reported, not written silently
             // symmetrise Z into T: Z(e1,e2) and Z(e2,e1) live in different
             // threads; exchange through shared memory row by row is avoided by
             // writing Z as is (symmetric to rounding).

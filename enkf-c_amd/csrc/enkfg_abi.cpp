@@ -134,12 +134,18 @@ struct enkfg_ctx {
 
     // transforms and diagnostics
     int ni_s = 0, nj_s = 0, stride = 0;
-    DevBuf d_T, d_w, d_T64, d_w64, d_pnlobs, d_pdfs, d_psrf, d_gram, d_sts, d_err;
+    DevBuf d_T, d_w, d_T64, d_w64, d_pnlobs, d_pdfs, d_psrf, d_gram, d_sts, d_err, d_etkf;
     bool transforms_valid = false, fp64_valid = false, diag_valid = false;
     enkfg_stats stats{};
 
     // update
     DevBuf d_E, d_infl, d_nlev, d_flevel, d_fsurf, d_flog, d_finfl, d_fratio;
+    // RANDOMISE: the context's drand48 stream (48-bit state) and the noise
+    // vectors of the randomised fields of the last batch
+    unsigned long long rand48 = 0;
+    DevBuf d_noise;
+    // analysed observations: cell groups and per-obs inflation parameters
+    DevBuf d_he_order, d_he_gcell, d_he_goff, d_he_infl, d_he_ratio;
     // adjoint
     DevBuf d_adjg, d_adjflag, d_adjlist, d_adjz, d_adjw, d_gy, d_gR;
     float* attached_E = nullptr;
@@ -739,8 +745,9 @@ int enkfg_set_S(enkfg_ctx* c, const float* S, const double* s, int standardised)
         // into device row order, rows padded to ms floats, plus the zero row
         HIPCHECK(c, launch_gather_rows(c->stream, c->d_S.as<float>(), c->d_perm.as<int>(), c->d_tmp.as<float>(),
                                        c->nobs, m, ms));
-        std::swap(c->d_S.p, c->d_tmp.p);
-        std::swap(c->d_S.bytes, c->d_tmp.bytes);
+        // copied back rather than swapped, so that d_S keeps its address and
+        // a captured calc graph that reads it stays valid across set_S calls
+        HIPCHECK(c, hipMemcpyAsync(c->d_S.p, c->d_tmp.p, nrow * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         HIPCHECK(c, launch_gather_rows(c->stream, c->d_s.as<double>(), c->d_perm.as<int>(), c->d_tmp2.as<double>(),
                                        c->nobs, 1, 1));
         HIPCHECK(c, hipMemcpyAsync(c->d_s.p, c->d_tmp2.p, (c->nobs + 1) * sizeof(double), hipMemcpyDeviceToDevice,
@@ -847,8 +854,6 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     const bool dotT = c->cfg.mode != ENKFG_MODE_ENOI;
     const bool keep64 = (c->cfg.flags & ENKFG_FLAG_KEEP_FP64) != 0;
     const bool pertype = (c->cfg.flags & ENKFG_FLAG_PERTYPE_DIAG) != 0 && nt > 0;
-    if (c->cfg.scheme == ENKFG_SCHEME_ETKF && dotT && m > 96)
-        return fail(c, ENKFG_ERR_UNSUPPORTED, "ETKF on the GPU needs nmem <= 96 (got %d)", m);
     HIPCHECK(c, c->d_T.reserve(std::max<size_t>(dotT ? (size_t)nn * md * m : 1, 1) * sizeof(float)));
     HIPCHECK(c, c->d_w.reserve((size_t)nn * m * sizeof(float)));
     if (keep64) {
@@ -866,6 +871,14 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     int nbatch = (int)std::max<size_t>(1, std::min<size_t>(budget / per_node, (size_t)nn));
     if (nbatch > 65536)
         nbatch = 65536;
+    // ETKF with m > 96 keeps its iterates in device memory (etkf_ws): batches
+    // of at most 1 GiB of it
+    const bool etkf_ws = c->cfg.scheme == ENKFG_SCHEME_ETKF && dotT && m > 96;
+    if (etkf_ws) {
+        const size_t per = etkf_ws_per_node(m) * sizeof(double);
+        nbatch = std::max(1, std::min(nbatch, (int)(((size_t)1 << 30) / per)));
+        HIPCHECK(c, c->d_etkf.reserve((size_t)nbatch * per));
+    }
     // two scratch buffers: the Gram kernel of batch b + 1 runs while the
     // factor kernel of batch b reads the other one
     const size_t gram_elems = (size_t)nbatch * per_node / sizeof(double);
@@ -903,6 +916,7 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     a.psrf = c->d_psrf.as<float>();
     a.nnode = nn;
     a.err = c->d_err.as<int>();
+    a.etkf_ws = etkf_ws ? c->d_etkf.as<double>() : nullptr;
     // Node order and batches.  Buckets by the node's largest segment (the
     // work of its heaviest Gram item), [0, 256), [256, 1024), [1024, 4096)
     // and [4096, ...) local obs: the nodes are processed bucket by bucket,
@@ -1014,7 +1028,10 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
                                       (long long)(intptr_t)a.T, (long long)(intptr_t)a.w,
                                       (long long)(intptr_t)a.T64, (long long)(intptr_t)a.w64,
                                       (long long)(intptr_t)a.pnlobs, (long long)(intptr_t)a.pdfs,
-                                      (long long)(intptr_t)a.psrf, (long long)(intptr_t)a.err, nn, nbatch};
+                                      (long long)(intptr_t)a.psrf, (long long)(intptr_t)a.err,
+                                      (long long)(intptr_t)a.etkf_ws, nn, nbatch, a.m, a.md, a.mode, a.scheme,
+                                      a.ntypes, a.pertype, a.nseg, a.ms, a.nobs,
+                                      (long long)__builtin_bit_cast(long long, a.alpha)};
         for (const Batch& bt : batches) {
             sig.push_back(bt.node0);
             sig.push_back(bt.n);
@@ -1149,7 +1166,8 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     int err = 0;
     HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
     if (err != 0)
-        return fail(c, ENKFG_ERR_LINALG, "non-positive pivot in the factorisation of I + S^T S");
+        return fail(c, ENKFG_ERR_LINALG, "non-positive pivot in the factorisation of I + S^T S, or no "
+                                         "convergence of the ETKF square root");
 
     // calcstats (transforms.c:44-50) from the counts
     std::vector<long long> off((size_t)nn + 1);
@@ -1349,7 +1367,91 @@ UpdArgs upd_args(enkfg_ctx* c, const enkfg_grid* g, int nfield, int j1, int jsiz
     return a;
 }
 
+// drand48 (POSIX): X <- (0x5DEECE66D X + 0xB) mod 2^48, value X / 2^48
+double rand48_next(unsigned long long& X)
+{
+    X = (0x5DEECE66Dull * X + 0xBull) & ((1ull << 48) - 1);
+    return (double)X * (1.0 / 281474976710656.0);
+}
+
+// get_normalpair (common/utils.c:1477-1484) on the stream X
+void rand48_normalpair(unsigned long long& X, double* x)
+{
+    const double u = sqrt(-2.0 * log(1.0 - rand48_next(X)));
+    const double v = (M_PI * 2.0) * rand48_next(X);
+    // gcc -O2 turns the reference's cos(v), sin(v) into one sincos() call,
+    // whose results can differ from the separate calls in the last bit
+    double sv, cv;
+    sincos(v, &sv, &cv);
+    x[0] = u * cv;
+    x[1] = u * sv;
+}
+
+// addnoise (update/update.c:46-87) for the randomised fields of a batch of
+// nfield fields in E [field][ncell][m]: the noise vectors are drawn on the
+// host from the context's stream, in field order, then applied on the device
+int apply_randomise(enkfg_ctx* c, int nfield, const enkfg_field* fields, float* E, long long ncell)
+{
+    const int m = c->cfg.nmem;
+    std::vector<int> fid;
+    for (int f = 0; f < nfield; ++f)
+        if (std::isfinite(fields[f].deflation))
+            fid.push_back(f);
+    if (fid.empty())
+        return ENKFG_OK;
+    const int npair = (m + 1) / 2;
+    std::vector<double> r((size_t)fid.size() * 2 * npair);
+    for (size_t k = 0; k < fid.size(); ++k) {
+        double* rk = &r[k * 2 * npair];
+        for (int e = 0; e < npair; ++e)
+            rand48_normalpair(c->rand48, &rk[2 * e]);
+        double sum = 0.0;
+        for (int e = 0; e < m; ++e)
+            sum += rk[e];
+        sum /= (double)m;
+        for (int e = 0; e < m; ++e)
+            rk[e] -= sum;
+    }
+    HIPCHECK(c, c->d_noise.reserve(r.size() * sizeof(double)));
+    HIPCHECK(c, hipMemcpyAsync(c->d_noise.p, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    for (size_t k = 0; k < fid.size(); ++k) {
+        const enkfg_field& fl = fields[fid[k]];
+        const float mult = (float)(sqrt(1.0 - fl.deflation * fl.deflation) * fl.sigma);
+        HIPCHECK(c, launch_addnoise(c->stream, E + (size_t)fid[k] * ncell * m, ncell, m, (float)fl.deflation, mult,
+                                    c->d_noise.as<double>() + k * 2 * npair));
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return ENKFG_OK;
+}
+
 }  // namespace
+
+int enkfg_set_rand48(enkfg_ctx* c, unsigned long long state)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    c->rand48 = state & ((1ull << 48) - 1);
+    return ENKFG_OK;
+}
+
+int enkfg_get_rand48(const enkfg_ctx* c, unsigned long long* state)
+{
+    if (c == nullptr || state == nullptr)
+        return ENKFG_ERR_ARG;
+    *state = c->rand48;
+    return ENKFG_OK;
+}
+
+int enkfg_rand48_normals(unsigned long long* state, int npair, double* out)
+{
+    if (state == nullptr || npair < 0 || (npair > 0 && out == nullptr))
+        return ENKFG_ERR_ARG;
+    unsigned long long X = *state & ((1ull << 48) - 1);
+    for (int k = 0; k < npair; ++k)
+        rand48_normalpair(X, &out[2 * k]);
+    *state = X;
+    return ENKFG_OK;
+}
 
 int enkfg_update_fields(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enkfg_field* fields, int j1, int j2,
                         float** fb, float* infl, int flags)
@@ -1392,6 +1494,9 @@ int enkfg_update_fields(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enk
     HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
     HIPCHECK(c, hipEventSynchronize(c->ev[6]));
     c->timings[3] = elapsed(c->ev[5], c->ev[6]);
+    r = apply_randomise(c, nfield, fields, c->d_E.as<float>(), (long long)ncell);
+    if (r != ENKFG_OK)
+        return r;
     for (int f = 0; f < nfield; ++f) {
         HIPCHECK(c, launch_cells_to_members(c->stream, c->d_E.as<float>() + (size_t)f * ncell * m, c->d_tmp.as<float>(),
                                             m, (long long)ncell));
@@ -1400,6 +1505,112 @@ int enkfg_update_fields(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enk
     }
     if (infl != nullptr)
         HIPCHECK(c, hipMemcpy(infl, c->d_infl.p, (size_t)nfield * ncell * sizeof(float), hipMemcpyDeviceToHost));
+    return ENKFG_OK;
+}
+
+int enkfg_update_HE(enkfg_ctx* c, const enkfg_grid* g, int nobs, const double* fi, const double* fj,
+                    const short* status, const float* inflation, const double* inf_ratio, float* HE)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->transforms_valid)
+        return fail(c, ENKFG_ERR_STATE, "no transforms for the update of HE");
+    if (g == nullptr || g->mni <= 0 || g->mnj <= 0 || nobs < 0)
+        return fail(c, ENKFG_ERR_ARG, "invalid grid or obs count");
+    if (nobs > 0 && (fi == nullptr || fj == nullptr || status == nullptr || inflation == nullptr ||
+                     inf_ratio == nullptr || HE == nullptr))
+        return fail(c, ENKFG_ERR_ARG, "NULL obs array");
+    if (g->stride != c->stride || (g->mni - 1) / g->stride + 1 != c->ni_s || (g->mnj - 1) / g->stride + 1 != c->nj_s)
+        return fail(c, ENKFG_ERR_ARG, "grid %d x %d, stride %d does not match the transforms grid %d x %d, stride %d",
+                    g->mnj, g->mni, g->stride, c->nj_s, c->ni_s, c->stride);
+    if (nobs == 0)
+        return ENKFG_OK;
+    const int m = c->cfg.nmem;
+    const long long ncell = (long long)g->mni * g->mnj;
+    // the obs the reference updates (ensobs.c:1321-1428): status OK and the
+    // nearest row a row of the grid; the column rounding to mni is the last
+    // one.  Obs whose column rounds outside the grid, or with non-finite
+    // indices, are left unchanged (the reference would read past its row).
+    std::vector<int> cellof((size_t)nobs, -1);
+    std::vector<int> cnt((size_t)ncell + 1, 0);
+    for (int o = 0; o < nobs; ++o) {
+        if (status[o] != ENKFG_STATUS_OK || !std::isfinite(fi[o]) || !std::isfinite(fj[o]))
+            continue;
+        const double ja = fj[o] + 0.5, ia = fi[o] + 0.5;
+        if (ja >= (double)g->mnj || ja <= -1.0 || ia <= -1.0 || ia >= (double)g->mni + 1.0)
+            continue;
+        const int j = (int)ja;
+        int i = (int)ia;
+        if (i == g->mni)
+            i--;
+        if (j < 0 || j >= g->mnj || i < 0 || i >= g->mni)
+            continue;
+        cellof[o] = j * g->mni + i;
+        cnt[(size_t)cellof[o]]++;
+    }
+    std::vector<int> gcell, goff;
+    std::vector<long long> start((size_t)ncell, -1);
+    long long tot = 0;
+    for (long long k = 0; k < ncell; ++k)
+        if (cnt[(size_t)k] > 0) {
+            gcell.push_back((int)k);
+            goff.push_back((int)tot);
+            start[(size_t)k] = tot;
+            tot += cnt[(size_t)k];
+        }
+    goff.push_back((int)tot);
+    std::vector<int> order((size_t)std::max<long long>(tot, 1));
+    for (int o = 0; o < nobs; ++o)
+        if (cellof[o] >= 0)
+            order[(size_t)start[(size_t)cellof[o]]++] = o;
+    const int ng = (int)gcell.size();
+    HIPCHECK(c, c->d_he_order.reserve(order.size() * sizeof(int)));
+    HIPCHECK(c, c->d_he_gcell.reserve(std::max<size_t>(gcell.size(), 1) * sizeof(int)));
+    HIPCHECK(c, c->d_he_goff.reserve(goff.size() * sizeof(int)));
+    HIPCHECK(c, c->d_he_infl.reserve((size_t)nobs * sizeof(float)));
+    HIPCHECK(c, c->d_he_ratio.reserve((size_t)nobs * sizeof(double)));
+    HIPCHECK(c, c->d_E.reserve((size_t)nobs * m * sizeof(float)));
+    HIPCHECK(c, c->d_tmp.reserve((size_t)nobs * m * sizeof(float)));
+    HIPCHECK(c, hipMemcpyAsync(c->d_he_order.p, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
+                               c->stream));
+    if (ng > 0)
+        HIPCHECK(c, hipMemcpyAsync(c->d_he_gcell.p, gcell.data(), gcell.size() * sizeof(int), hipMemcpyHostToDevice,
+                                   c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->d_he_goff.p, goff.data(), goff.size() * sizeof(int), hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->d_he_infl.p, inflation, (size_t)nobs * sizeof(float), hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->d_he_ratio.p, inf_ratio, (size_t)nobs * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->d_tmp.p, HE, (size_t)nobs * m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, launch_members_to_cells(c->stream, c->d_tmp.as<float>(), c->d_E.as<float>(), m, nobs));
+    UpdArgs u{};
+    u.m = m;
+    u.md = c->cfg.nmem_dynamic;
+    u.mode = c->cfg.mode;
+    u.mni = g->mni;
+    u.mnj = g->mnj;
+    u.stride = g->stride;
+    u.periodic_i = g->periodic_i;
+    u.ni_s = c->ni_s;
+    u.nj_s = c->nj_s;
+    u.T = c->d_T.as<float>();
+    u.w = c->d_w.as<float>();
+    HeArgs h{};
+    h.ngroup = ng;
+    h.gcell = c->d_he_gcell.as<int>();
+    h.goff = c->d_he_goff.as<int>();
+    h.order = c->d_he_order.as<int>();
+    h.infl0 = c->d_he_infl.as<float>();
+    h.ratio = c->d_he_ratio.as<double>();
+    h.HE = c->d_E.as<float>();
+    HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+    HIPCHECK(c, launch_update_he(c->stream, u, h));
+    HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
+    HIPCHECK(c, launch_cells_to_members(c->stream, c->d_E.as<float>(), c->d_tmp.as<float>(), m, nobs));
+    HIPCHECK(c, hipMemcpyAsync(HE, c->d_tmp.p, (size_t)nobs * m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->timings[3] = elapsed(c->ev[5], c->ev[6]);
     return ENKFG_OK;
 }
 
@@ -1476,7 +1687,7 @@ int enkfg_ens_update(enkfg_ctx* c, int flags, float* dev_infl)
     HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
     HIPCHECK(c, hipEventSynchronize(c->ev[6]));
     c->timings[3] = elapsed(c->ev[5], c->ev[6]);
-    return ENKFG_OK;
+    return apply_randomise(c, c->att_nfield, c->att_fields.data(), c->attached_E, a.ncell);
 }
 
 int enkfg_adjoint(enkfg_ctx* c, const float* dev_dJ_dmean, double* dJ_dy, double* dJ_dR, double* dJ_dinfl)
@@ -1571,7 +1782,8 @@ int enkfg_adjoint(enkfg_ctx* c, const float* dev_dJ_dmean, double* dJ_dy, double
         int err = 0;
         HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
         if (err != 0)
-            return fail(c, ENKFG_ERR_LINALG, "non-positive pivot in the factorisation of I + S^T S");
+            return fail(c, ENKFG_ERR_LINALG, "non-positive pivot in the factorisation of I + S^T S, or no "
+                                         "convergence of the ETKF square root");
     }
     std::vector<double> gy((size_t)c->nobs), gR((size_t)c->nobs);
     if (c->nobs > 0) {

@@ -122,7 +122,17 @@ struct CalcArgs {
     // Gram item kernels (m <= 96): tile groups (wavefronts) per segment, 2 or
     // 3; any other value takes the kernel's default (3 FP64, 2 mixed)
     int tile_groups;
+    // ETKF with m > 96: the Denman-Beavers iterates Y, Z and Y^-1 of each batch
+    // node in device memory, [bnode][3][B * B][256] doubles (thread-major, so
+    // each thread's block elements are coalesced)
+    double* etkf_ws;
 };
+// doubles of etkf_ws per batch node
+__host__ __device__ inline size_t etkf_ws_per_node(int m)
+{
+    const size_t B = (size_t)(m + 15) / 16;
+    return 3 * B * B * 256;
+}
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a);
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a);   // gram_mixed.hip
 hipError_t launch_factor(hipStream_t s, const CalcArgs& a);
@@ -152,6 +162,19 @@ struct UpdArgs {
 };
 hipError_t launch_update_mfma(hipStream_t s, const UpdArgs& a, bool mixed);   // update_mfma.hip
 
+// analysed ensemble observations (obs_update.hip): the observations grouped
+// by grid cell, HE observation-major [nobs][m]
+struct HeArgs {
+    int ngroup;                    // cells holding observations
+    const int* gcell;              // [ngroup] cell j * mni + i
+    const int* goff;               // [ngroup + 1] offsets into order
+    const int* order;              // observation indices, grouped by cell
+    const float* infl0;            // [nobs] inflation of the obs type's variable
+    const double* ratio;           // [nobs] its inflation ratio (NaN = plain)
+    float* HE;                     // [nobs][m], updated in place
+};
+hipError_t launch_update_he(hipStream_t s, const UpdArgs& u, const HeArgs& h);
+
 // adjoint of the analysis mean (adjoint.hip)
 // g_nodes[node][m] += sum over cells of beta(cell, node) sum_f dJ/dmean[f][cell] A^f[f][cell][:]
 hipError_t launch_adj_cells(hipStream_t s, const UpdArgs& a, const float* dJ_dmean, double* g_nodes);
@@ -161,6 +184,9 @@ hipError_t launch_nonzero_rows(hipStream_t s, const double* g, int nrow, int wid
 hipError_t launch_adj_obs(hipStream_t s, const CalcArgs& a, int nlist, const int* perm, const short* type,
                           const float* estd, const double* sqrt_rfactor, double* dJ_dy, double* dJ_dR);
 hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a);
+// addnoise (update.c:46-87) on one field E [ncell][m]:
+// v <- (float)((double)(v * defl) + (double)mult * r[e])
+hipError_t launch_addnoise(hipStream_t s, float* E, long long ncell, int m, float defl, float mult, const double* r);
 hipError_t launch_hybrid_condition(hipStream_t s, const UpdArgs& a, double gamma);
 // [nmem][rows][cols] <-> [rows*cols][nmem]
 hipError_t launch_members_to_cells(hipStream_t s, const float* in, float* out, int nmem, long long ncell);

@@ -232,7 +232,31 @@ unsigned cell_blocks(long long ncell)
     return (unsigned)(8 * ((ncell + 7) / 8));
 }
 
+// addnoise (update/update.c:46-87): v <- v * d + mult * r_e, in the
+// reference's arithmetic: the product v * d in float, the sum in double
+// with mult * r_e (float times double), rounded to float on the store
+__global__ void __launch_bounds__(256) addnoise_kernel(float* __restrict__ E, long long n, int m, float defl,
+                                                       float mult, const double* __restrict__ r)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int e = (int)(i % m);
+        E[i] = (float)((double)(E[i] * defl) + (double)mult * r[e]);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_addnoise(hipStream_t s, float* E, long long ncell, int m, float defl, float mult, const double* r)
+{
+    const long long n = ncell * m;
+    if (n == 0)
+        return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536)
+        nb = 65536;
+    hipLaunchKernelGGL(addnoise_kernel, dim3((unsigned)nb), dim3(256), 0, s, E, n, m, defl, mult, r);
+    return hipGetLastError();
+}
 
 hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a)
 {

@@ -32,7 +32,8 @@ EXPORTED = [
     "enkfg_set_obs_status", "enkfg_set_S", "enkfg_get_S", "enkfg_calc_transforms", "enkfg_get_transforms",
     "enkfg_get_diag", "enkfg_get_stats", "enkfg_localise", "enkfg_get_local_counts", "enkfg_get_localobs",
     "enkfg_set_transforms", "enkfg_update_fields", "enkfg_update_bg", "enkfg_ens_attach", "enkfg_ens_update",
-    "enkfg_get_timings", "enkfg_get_tuning", "enkfg_adjoint",
+    "enkfg_get_timings", "enkfg_get_tuning", "enkfg_adjoint", "enkfg_set_rand48", "enkfg_get_rand48",
+    "enkfg_rand48_normals", "enkfg_update_HE",
 ]
 
 
@@ -70,7 +71,7 @@ class Grid(C.Structure):
 
 class Field(C.Structure):
     _fields_ = [("level", C.c_int), ("issurfacevar", C.c_int), ("inflation", C.c_float), ("inf_ratio", C.c_double),
-                ("applylog", C.c_int)]
+                ("applylog", C.c_int), ("deflation", C.c_double), ("sigma", C.c_double)]
 
 
 _lib = None
@@ -110,6 +111,10 @@ def lib():
         L.enkfg_get_timings.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
         L.enkfg_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
         L.enkfg_adjoint.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
+        L.enkfg_update_HE.argtypes = [vp, C.POINTER(Grid), C.c_int, vp, vp, vp, vp, vp, vp]
+        L.enkfg_set_rand48.argtypes = [vp, C.c_ulonglong]
+        L.enkfg_get_rand48.argtypes = [vp, C.POINTER(C.c_ulonglong)]
+        L.enkfg_rand48_normals.argtypes = [C.POINTER(C.c_ulonglong), C.c_int, vp]
         _lib = L
     return _lib
 
@@ -144,6 +149,8 @@ def make_fields(fields):
         arr[k].inflation = f.get("inflation", 1.0)
         arr[k].inf_ratio = f.get("inf_ratio", 1.0)
         arr[k].applylog = f.get("applylog", 0)
+        arr[k].deflation = f.get("deflation", float("nan"))
+        arr[k].sigma = f.get("sigma", 0.0)
     return arr
 
 
@@ -323,6 +330,21 @@ class Engine:
         self._check(lib().enkfg_update_fields(self.h, C.byref(g), len(fields), fa, j1, j2, bp, _p(infl), flags))
         return infl
 
+    def update_HE(self, mnj, mni, stride, periodic_i, fi, fj, status, inflation, inf_ratio, HE):
+        """das_updateHE's update of the ensemble observations: HE [nmem][nobs]
+        float32 (HE^f); returns HE^a (a new array)."""
+        g = Grid(mni, mnj, stride, periodic_i, 0, None)
+        out = np.ascontiguousarray(HE, np.float32).copy()
+        nobs = out.shape[1]
+        fi = np.ascontiguousarray(fi, np.float64)
+        fj = np.ascontiguousarray(fj, np.float64)
+        status = np.ascontiguousarray(status, np.int16)
+        inflation = np.ascontiguousarray(np.broadcast_to(inflation, (nobs,)), np.float32)
+        inf_ratio = np.ascontiguousarray(np.broadcast_to(inf_ratio, (nobs,)), np.float64)
+        self._check(lib().enkfg_update_HE(self.h, C.byref(g), nobs, _p(fi), _p(fj), _p(status), _p(inflation),
+                                          _p(inf_ratio), _p(out)))
+        return out
+
     def update_bg(self, nlevels, stride, periodic_i, surfk, fields, buffers, j1, j2):
         g, keep = self._grid(nlevels, stride, periodic_i, surfk)
         fa = make_fields(fields)
@@ -340,3 +362,23 @@ class Engine:
 
     def sync(self):
         self._check(lib().enkfg_sync(self.h))
+
+    # ---- RANDOMISE stream (drand48 state)
+    def set_rand48(self, state):
+        self._check(lib().enkfg_set_rand48(self.h, int(state)))
+
+    def rand48(self):
+        st = C.c_ulonglong()
+        self._check(lib().enkfg_get_rand48(self.h, C.byref(st)))
+        return st.value
+
+
+def rand48_normals(state, npair):
+    """The engine's normal pairs (host side, no device): returns (pairs as a
+    flat array of 2 npair values, the advanced state)."""
+    st = C.c_ulonglong(int(state))
+    out = np.zeros(2 * max(npair, 0), np.float64)
+    r = lib().enkfg_rand48_normals(C.byref(st), int(npair), _p(out))
+    if r != OK:
+        raise EnkfgError(r, "enkfg_rand48_normals failed")
+    return out, st.value

@@ -136,6 +136,9 @@ typedef struct {
     float inflation;     /* model_getvarinflation(): inflation0 */
     double inf_ratio;    /* NaN = PLAIN inflation */
     int applylog;        /* model_getvarislog() */
+    double deflation;    /* RANDOMISE <deflation> <sigma> (model.c:208-218,
+                            model_getvardeflation); NaN = not randomised */
+    double sigma;        /* model_getvarsigma() */
 } enkfg_field;
 
 /* update flags (common/global.h:92-104) */
@@ -203,6 +206,34 @@ int enkfg_set_transforms(enkfg_ctx* ctx, int nj_s, int ni_s, int stride, const f
  * fieldbuffer[f] is float [nmem][j2-j1+1][mni], updated in place; infl
  * (optional) receives [nfield][j2-j1+1][mni] inflation factors. */
 int enkfg_update_fields(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const enkfg_field* fields, int j1, int j2, float** fieldbuffer, float* infl, int flags);
+
+/* RANDOMISE (addnoise, update/update.c:46-87, applied at 554-563): after the
+ * update of a field batch, every field with a finite deflation d gets
+ *   v[e] <- v[e] * d + sqrt(1 - d^2) sigma (r_e - mean_e r)
+ * over all nmem members and all cells of the batch's rows, with r_e from
+ * (nmem + 1) / 2 normal pairs of drand48 (get_normalpair, common/utils.c:
+ * 1477-1484), drawn field by field in batch order.  The engine keeps its own
+ * drand48 stream per context: 48-bit state X, X <- (0x5DEECE66D X + 0xB) mod
+ * 2^48, drand48() = X / 2^48.  The initial state is 0, the state of glibc's
+ * drand48 when the process never seeds it -- the reference seeds srand()
+ * (common/utils.c:102-122), not srand48(), so its stream starts there too. */
+int enkfg_set_rand48(enkfg_ctx* ctx, unsigned long long state);
+int enkfg_get_rand48(const enkfg_ctx* ctx, unsigned long long* state);
+/* The engine's generator on the host, without a context or a device: npair
+ * normal pairs (get_normalpair) into out[2 * npair], advancing *state. */
+int enkfg_rand48_normals(unsigned long long* state, int npair, double* out);
+
+/* The analysed ensemble observations of a structured grid: update_HE_structured
+ * (calc/ensobs.c:1228-1553; update_Hx_structured, 1796-1983, in EnOI mode),
+ * the step of das_updateHE (ensobs.c:2116-2129) between sorting the obs and
+ * das_calcinnandspread.  HE is das->S in S_MODE_HE_f, [nmem][nobs], updated
+ * in place for the obs with status OK whose nearest cell
+ * ((int)(fi + 0.5), (int)(fj + 0.5); a column rounding to mni is the last
+ * one) lies in the grid; fi, fj are the obs' fractional indices (fij[0],
+ * fij[1]); inflation and inf_ratio per obs are model_getvarinflation() of
+ * the obs type's variable (NaN ratio = PLAIN).  Uses the transforms of the
+ * last calc or enkfg_set_transforms; grid->nlevels is not used. */
+int enkfg_update_HE(enkfg_ctx* ctx, const enkfg_grid* grid, int nobs, const double* fi, const double* fj, const short* status, const float* inflation, const double* inf_ratio, float* HE);
 
 /* das_updatebg (EnOI): fieldbuffer[f] is [nmem + 1][jsize][mni]; member nmem
  * is the background, updated in place. */

@@ -1284,3 +1284,216 @@ void or_sethybridensemble(int nmem, int nmem_d, double gamma, int nij, float* v,
         for (i = 0; i < nij; ++i)
             v[(size_t) e * nij + i] = v[(size_t) e * nij + i] * k_s + vmean[i];
 }
+
+/* ======================================================================
+ * RANDOMISE (update/update.c:46-87)
+ * ====================================================================== */
+
+/* drand48: X <- (a X + c) mod 2^48 with a = 0x5DEECE66D, c = 0xB (the
+ * POSIX drand48 family); the value is X / 2^48 */
+double or_drand48(unsigned long long* state)
+{
+    *state = (0x5DEECE66DULL * *state + 0xBULL) & 0xFFFFFFFFFFFFULL;
+    return (double) *state / 281474976710656.0;
+}
+
+/* get_normalpair (common/utils.c:1477-1484) */
+void or_get_normalpair(unsigned long long* state, double x[2])
+{
+    double u = sqrt(-2.0 * log(1.0 - or_drand48(state)));
+    double v = (M_PI * 2.0) * or_drand48(state);
+
+    x[0] = u * cos(v);
+    x[1] = u * sin(v);
+}
+
+/* addnoise: (nmem + 1) / 2 normal pairs, centred over the nmem members, then
+ * v = v * (float) deflation + (float) (sqrt(1 - deflation^2) sigma) * r_e */
+void or_addnoise(int nmem, double deflation, double sigma, unsigned long long* state, long long n, float** v)
+{
+    double* random = malloc((size_t) ((nmem + 1) / 2 * 2) * sizeof(double));
+    double sum = 0.0;
+    float mult;
+    int e;
+    long long i;
+
+    for (e = 0; e < (nmem + 1) / 2; ++e)
+        or_get_normalpair(state, &random[e * 2]);
+    for (e = 0; e < nmem; ++e)
+        sum += random[e];
+    sum /= (double) nmem;
+    for (e = 0; e < nmem; ++e)
+        random[e] -= sum;
+    mult = (float) (sqrt(1.0 - deflation * deflation) * sigma);
+    for (e = 0; e < nmem; ++e) {
+        float* vv = v[e];
+
+        for (i = 0; i < n; ++i)
+            vv[i] = vv[i] * (float) deflation + mult * random[e];
+    }
+    free(random);
+}
+
+/* ======================================================================
+ * analysed observations: update_HE_structured (calc/ensobs.c:1228-1553)
+ * ====================================================================== */
+
+/* T and w of model row j (update.c:195-272 / ensobs.c:1319-1405): Tj
+ * [mni][md m], wj [mni][m]; Tjj, wjj scratch [ni_s][.] */
+static void interp_row(const or_updatespec* sp, int j, float* Tjj, float* wjj, float* Tj, float* wj)
+{
+    int m = sp->nmem, md = sp->nmem_dynamic, ni = sp->ni_s, nj = sp->nj_s, stride = sp->stride;
+    int nT = md * m;
+    int jj = j / stride, stepj = j % stride;
+    size_t k;
+
+    if (Tj == NULL)
+        nT = 0;
+    if (stride == 1) {
+        if (Tj)
+            memcpy(Tj, &sp->T[(size_t) j * ni * nT], (size_t) ni * nT * sizeof(float));
+        memcpy(wj, &sp->w[(size_t) j * ni * m], (size_t) ni * m * sizeof(float));
+        return;
+    }
+    if (stepj == 0) {
+        if (Tj)
+            memcpy(Tjj, &sp->T[(size_t) jj * ni * nT], (size_t) ni * nT * sizeof(float));
+        memcpy(wjj, &sp->w[(size_t) jj * ni * m], (size_t) ni * m * sizeof(float));
+    } else {
+        float weight2 = (float) stepj / (float) stride;
+        float weight1 = 1.0f - weight2;
+        int jj2 = (jj < nj - 1) ? jj + 1 : jj;
+        const float* w1 = &sp->w[(size_t) jj * ni * m];
+        const float* w2 = &sp->w[(size_t) jj2 * ni * m];
+
+        if (Tj) {
+            const float* T1 = &sp->T[(size_t) jj * ni * nT];
+            const float* T2 = &sp->T[(size_t) jj2 * ni * nT];
+
+            for (k = 0; k < (size_t) ni * nT; ++k)
+                Tjj[k] = T1[k] * weight1 + T2[k] * weight2;
+        }
+        for (k = 0; k < (size_t) ni * m; ++k)
+            wjj[k] = w1[k] * weight1 + w2[k] * weight2;
+    }
+    interp_i(ni, sp->mni, stride, sp->periodic_i, nT, m, Tjj, wjj, Tj, wj);
+}
+
+int or_update_HE(const or_updatespec* sp, int nobs, const double* fi, const double* fj, const short* status, const float* inflation0, const double* inf_ratio, float* HE)
+{
+    int m = sp->nmem, md = sp->nmem_dynamic, mni = sp->mni, mnj = sp->mnj;
+    int enoi = (sp->mode == OR_MODE_ENOI);
+    int nT = enoi ? 1 : md * m;
+    float* Tj = malloc((size_t) mni * nT * sizeof(float));
+    float* wj = malloc((size_t) mni * m * sizeof(float));
+    float* Tjj = malloc((size_t) sp->ni_s * nT * sizeof(float));
+    float* wjj = malloc((size_t) sp->ni_s * m * sizeof(float));
+    float* HEo_f = malloc((size_t) m * sizeof(float));
+    float* HEo_a = malloc((size_t) m * sizeof(float));
+    int* rowof = malloc((size_t) (nobs > 0 ? nobs : 1) * sizeof(int));
+    int j, o, e, e2, r = OR_OK;
+
+    if (!Tj || !wj || !Tjj || !wjj || !HEo_f || !HEo_a || !rowof) {
+        r = OR_ERR_NOMEM;
+        goto done;
+    }
+    for (o = 0; o < nobs; ++o)
+        rowof[o] = (int) (fj[o] + 0.5);
+    for (j = 0; j < mnj; ++j) {
+        int have = 0;
+
+        for (o = 0; o < nobs && !have; ++o)
+            have = (rowof[o] == j && status[o] == OR_STATUS_OK);
+        if (!have)
+            continue;
+        interp_row(sp, j, Tjj, wjj, enoi ? NULL : Tj, wj);
+        for (o = 0; o < nobs; ++o) {
+            float inflation = NAN;
+            double v1_f = 0.0, v1_a = 0.0;
+            int i;
+            const float* Ti;
+            const float* wi;
+
+            if (rowof[o] != j || status[o] != OR_STATUS_OK)
+                continue;
+            i = (int) (fi[o] + 0.5);
+            if (i == mni)
+                i--;
+            Ti = &Tj[(size_t) i * nT];
+            wi = &wj[(size_t) i * m];
+            if (enoi) {
+                /* update_Hx_structured (ensobs.c:1933-1962): the mean only */
+                double Hx = 0.0, dHx = 0.0;
+
+                for (e = 0; e < m; ++e)
+                    Hx += HE[(size_t) e * nobs + o];
+                Hx /= (double) m;
+                for (e = 0; e < m; ++e)
+                    dHx += (HE[(size_t) e * nobs + o] - Hx) * wi[e];
+                for (e = 0; e < m; ++e)
+                    HE[(size_t) e * nobs + o] += dHx;
+                continue;
+            }
+            for (e = 0; e < m; ++e)
+                HEo_f[e] = HE[(size_t) e * nobs + o];
+            for (e = 0; e < md; ++e)
+                v1_f += HEo_f[e];
+            v1_f /= (double) md;
+            for (e = 0; e < m; ++e)
+                HEo_f[e] -= v1_f;
+            /* HA^a = HA^f T: sgemv('T', m, md, T) */
+            for (e = 0; e < md; ++e) {
+                float sum = 0.0f;
+
+                for (e2 = 0; e2 < m; ++e2)
+                    sum += Ti[(size_t) e * m + e2] * HEo_f[e2];
+                HEo_a[e] = sum;
+            }
+            for (e = 0; e < m; ++e)
+                v1_a += HEo_f[e] * wi[e];
+            v1_a += v1_f;
+            for (e = 0; e < md; ++e)
+                HEo_a[e] += v1_a;
+            for (e = 0; e < m; ++e)
+                HEo_f[e] += v1_f;
+            if (!isnan(inf_ratio[o])) {
+                double v2_f = 0.0, v2_a = 0.0, var_a, var_f;
+
+                for (e = 0; e < md; ++e) {
+                    double ve = (double) HEo_f[e];
+
+                    v2_f += ve * ve;
+                }
+                var_f = v2_f / (double) md - v1_f * v1_f;
+                for (e = 0; e < md; ++e) {
+                    double ve = (double) HEo_a[e];
+
+                    v2_a += ve * ve;
+                }
+                var_a = v2_a / (double) md - v1_a * v1_a;
+                if (var_a > 0.0) {
+                    inflation = (float) (sqrt(var_f / var_a) * inf_ratio[o] + 1.0 - inf_ratio[o]);
+                    if (inflation >= inflation0[o])
+                        inflation = inflation0[o];
+                }
+            } else
+                inflation = inflation0[o];
+            if (fabsf(inflation - 1.0f) > 1.0e-6f)
+                for (e = 0; e < md; ++e)
+                    HEo_a[e] = (HEo_a[e] - (float) v1_a) * inflation + v1_a;
+            for (e = 0; e < md; ++e)
+                HE[(size_t) e * nobs + o] = HEo_a[e];
+            for (e = md; e < m; ++e)
+                HE[(size_t) e * nobs + o] += v1_a - v1_f;
+        }
+    }
+  done:
+    free(Tj);
+    free(wj);
+    free(Tjj);
+    free(wjj);
+    free(HEo_f);
+    free(HEo_a);
+    free(rowof);
+    return r;
+}

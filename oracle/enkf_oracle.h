@@ -166,6 +166,21 @@ int or_update_fields(const or_updatespec* spec, int nfield, const or_field* fiel
 /* EnOI background: fieldbuffer[f] is [nmem + 1][jsize][mni], member nmem is
  * the background updated in place. */
 int or_update_bg(const or_updatespec* spec, int nfield, const or_field* fields, int j1, int j2, float** fieldbuffer);
+/* update_HE_structured (calc/ensobs.c:1228-1553): the analysed ensemble
+ * observations.  HE [nmem][nobs] holds HE^f on entry and HE^a on exit for
+ * the obs with status OK whose nearest row (int)(fj + 0.5) is a row of the
+ * grid; fi, fj are the obs' fractional grid indices; inflation0 and
+ * inf_ratio are those of the obs type's model variable (NaN ratio = PLAIN).
+ * sp->T, sp->w, stride, periodic_i, mni, mnj as for or_update_fields.  In
+ * EnOI mode (sp->T unused) it is update_Hx_structured (ensobs.c:1796-1983):
+ * every member moves by the interpolated w applied to the anomalies. */
+int or_update_HE(const or_updatespec* sp, int nobs, const double* fi, const double* fj, const short* status, const float* inflation0, const double* inf_ratio, float* HE);
+/* RANDOMISE: drand48 on an explicit 48-bit state (POSIX), get_normalpair
+ * (common/utils.c:1477-1484) and addnoise (update/update.c:46-87) applied to
+ * v[e][0 .. n-1], e < nmem */
+double or_drand48(unsigned long long* state);
+void or_get_normalpair(unsigned long long* state, double x[2]);
+void or_addnoise(int nmem, double deflation, double sigma, unsigned long long* state, long long n, float** v);
 /* hybrid conditioning of one layer; v is [nmem][nij], vmean scratch [nij] */
 void or_sethybridensemble(int nmem, int nmem_dynamic, double gamma, int nij, float* v, double* vmean);
 

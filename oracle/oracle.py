@@ -106,6 +106,12 @@ def lib():
         L.or_update_bg.argtypes = [C.POINTER(UpdateSpec), C.c_int, C.POINTER(Field), C.c_int, C.c_int,
                                    C.POINTER(_fp)]
         L.or_sethybridensemble.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _fp, _dp]
+        L.or_update_HE.argtypes = [C.POINTER(UpdateSpec), C.c_int, _dp, _dp, _sp, _fp, _dp, _fp]
+        L.or_drand48.restype = C.c_double
+        L.or_drand48.argtypes = [C.POINTER(C.c_ulonglong)]
+        L.or_get_normalpair.argtypes = [C.POINTER(C.c_ulonglong), _dp]
+        L.or_addnoise.argtypes = [C.c_int, C.c_double, C.c_double, C.POINTER(C.c_ulonglong), C.c_longlong,
+                                  C.POINTER(_fp)]
         _lib = L
     return _lib
 
@@ -121,10 +127,13 @@ def ref():
         if not ref_available():
             raise RuntimeError("oracle/_ref not built (needs the reference tree and MKL): make -C oracle ref")
         # one thread, sequential MKL: threaded MKL sharing a process with
-        # torch's OpenMP runtime gives wrong results
+        # torch's OpenMP runtime gives wrong products (the non-SPD I + S^T S
+        # that once made dpotrf fail with info = 7 in the CPU baseline)
         os.environ["MKL_THREADING_LAYER"] = "SEQUENTIAL"
         os.environ["MKL_NUM_THREADS"] = "1"
         R = C.CDLL(REF_PATH)
+        R.ref_last_error.restype = C.c_char_p
+        R.ref_last_error.argtypes = []
         R.ref_local_analysis.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                          _dp, _dp, _dp]
         R.ref_kd_build.restype = C.c_void_p
@@ -301,8 +310,10 @@ def local_analysis(mode, scheme, m, mout, S, s, use_ref=False):
     if use_ref:
         dfs = C.c_double()
         srf = C.c_double()
-        ref().ref_local_analysis(mode, scheme, m, mout, p, _ptr(Scm, _dp), _ptr(s, _dp), _ptr(T, _dp),
-                                 _ptr(w, _dp), C.byref(dfs), C.byref(srf), _ptr(G, _dp))
+        r = ref().ref_local_analysis(mode, scheme, m, mout, p, _ptr(Scm, _dp), _ptr(s, _dp), _ptr(T, _dp),
+                                     _ptr(w, _dp), C.byref(dfs), C.byref(srf), _ptr(G, _dp))
+        if r != 0:
+            raise RuntimeError("reference local analysis failed: %s" % ref().ref_last_error().decode())
         return T, w, dfs.value, srf.value
     L = lib()
     if mode == MODE_ENOI:
@@ -343,9 +354,17 @@ class RefTrees:
 
 
 def calc_transforms(mode, scheme, nmem, nmem_dynamic, alpha, obs, S, s, lon, lat, geographic=1,
-                    visible=None, node0=0, nnode_do=-1, use_ref=False, want_T=True):
+                    visible=None, node0=0, nnode_do=-1, use_ref=False, want_T=True, threads=1):
     """Runs the das_calctransforms loop.  S: [nmem][nobs] float32 (already
-    standardised), s: [nobs] float64, lon/lat: node coordinates [nnode]."""
+    standardised), s: [nobs] float64, lon/lat: node coordinates [nnode].
+
+    threads > 1 splits the nodes [node0, node0 + nnode_do) into contiguous
+    runs, one per thread (the C loops release the GIL; the oracle is
+    reentrant; with use_ref, each thread gets its own reference trees, since
+    kd_findnodeswithinrange keeps its results in the tree).  Outputs are per
+    node, so the runs write disjoint parts; the stats are summed."""
+    import threading
+
     lon = np.ascontiguousarray(lon, np.float64)
     lat = np.ascontiguousarray(lat, np.float64)
     nnode = len(lon)
@@ -361,31 +380,66 @@ def calc_transforms(mode, scheme, nmem, nmem_dynamic, alpha, obs, S, s, lon, lat
     pnlobs = np.zeros((nt + 1, nnode), np.int32)
     pdfs = np.zeros((nt + 1, nnode), np.float32)
     psrf = np.zeros((nt + 1, nnode), np.float32)
-    st = CalcStats()
     spec = CalcSpec(mode, scheme, nmem, md, alpha)
     vis = None if visible is None else np.ascontiguousarray(visible, np.uint8)
+    if nnode_do < 0:
+        nnode_do = nnode - node0
+    nth = max(1, min(int(threads), nnode_do))
+    bounds = [node0 + (nnode_do * k) // nth for k in range(nth + 1)]
+    results = [None] * nth
+    trees = [None] * nth
     if use_ref:
-        own = False
-        if not isinstance(use_ref, RefTrees):
-            use_ref = RefTrees(obs)
-            own = True
-        r = ref().ref_calc_transforms(C.byref(spec), C.byref(obs.o), use_ref.h, _ptr(S, _fp),
-                                      _ptr(s, _dp), nnode, _ptr(lon, _dp), _ptr(lat, _dp), int(geographic),
-                                      _ptr(vis, _up), _ptr(T, _fp), _ptr(w, _fp), _ptr(T64, _dp),
-                                      _ptr(w64, _dp), _ptr(pnlobs, _ip), _ptr(pdfs, _fp), _ptr(psrf, _fp),
-                                      C.byref(st), node0, nnode_do)
-        if own:
-            use_ref.close()
+        if isinstance(use_ref, RefTrees) and nth == 1:
+            trees[0] = (use_ref, False)
+        else:
+            def mk(k):
+                trees[k] = (RefTrees(obs), True)
+            ths = [threading.Thread(target=mk, args=(k,)) for k in range(nth)]
+            for t in ths:
+                t.start()
+            for t in ths:
+                t.join()
+
+    def work(k):
+        st = CalcStats()
+        a, b = bounds[k], bounds[k + 1]
+        if use_ref:
+            r = ref().ref_calc_transforms(C.byref(spec), C.byref(obs.o), trees[k][0].h, _ptr(S, _fp),
+                                          _ptr(s, _dp), nnode, _ptr(lon, _dp), _ptr(lat, _dp), int(geographic),
+                                          _ptr(vis, _up), _ptr(T, _fp), _ptr(w, _fp), _ptr(T64, _dp),
+                                          _ptr(w64, _dp), _ptr(pnlobs, _ip), _ptr(pdfs, _fp), _ptr(psrf, _fp),
+                                          C.byref(st), a, b - a)
+            msg = ref().ref_last_error().decode() if r != 0 else ""
+        else:
+            r = lib().or_calc_transforms(C.byref(spec), C.byref(obs.o), _ptr(S, _fp), _ptr(s, _dp), nnode,
+                                         _ptr(lon, _dp), _ptr(lat, _dp), int(geographic), _ptr(vis, _up),
+                                         _ptr(T, _fp), _ptr(w, _fp), _ptr(T64, _dp), _ptr(w64, _dp),
+                                         _ptr(pnlobs, _ip), _ptr(pdfs, _fp), _ptr(psrf, _fp), C.byref(st),
+                                         a, b - a)
+            msg = ""
+        results[k] = (r, msg, st)
+
+    if nth == 1:
+        work(0)
     else:
-        r = lib().or_calc_transforms(C.byref(spec), C.byref(obs.o), _ptr(S, _fp), _ptr(s, _dp), nnode,
-                                     _ptr(lon, _dp), _ptr(lat, _dp), int(geographic), _ptr(vis, _up),
-                                     _ptr(T, _fp), _ptr(w, _fp), _ptr(T64, _dp), _ptr(w64, _dp),
-                                     _ptr(pnlobs, _ip), _ptr(pdfs, _fp), _ptr(psrf, _fp), C.byref(st),
-                                     node0, nnode_do)
-    if r != 0:
-        raise RuntimeError("calc_transforms failed: %d" % r)
-    stats = dict(nlobs_sum=st.nlobs_sum, nlobs_max=st.nlobs_max, n_inv_obs=st.n_inv_obs,
-                 n_inv_ens=st.n_inv_ens, ncell=st.ncell)
+        ths = [threading.Thread(target=work, args=(k,)) for k in range(nth)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+    for tr in trees:
+        if tr is not None and tr[1]:
+            tr[0].close()
+    for r, msg, _ in results:
+        if r != 0:
+            raise RuntimeError("calc_transforms failed: %d %s" % (r, msg))
+    stats = dict(nlobs_sum=0, nlobs_max=0, n_inv_obs=0, n_inv_ens=0, ncell=0)
+    for _, _, st in results:
+        stats["nlobs_sum"] += st.nlobs_sum
+        stats["nlobs_max"] = max(stats["nlobs_max"], st.nlobs_max)
+        stats["n_inv_obs"] += st.n_inv_obs
+        stats["n_inv_ens"] += st.n_inv_ens
+        stats["ncell"] += st.ncell
     return dict(T=T, w=w, T64=T64, w64=w64, pnlobs=pnlobs, pdfs=pdfs, psrf=psrf, stats=stats)
 
 
@@ -439,3 +493,47 @@ def sethybridensemble(nmem, nmem_dynamic, gamma, v):
     nij = v.shape[1]
     vmean = np.zeros(nij, np.float64)
     lib().or_sethybridensemble(nmem, nmem_dynamic, float(gamma), nij, _ptr(v, _fp), _ptr(vmean, _dp))
+
+
+def normalpairs(state, npair):
+    """get_normalpair npair times from the drand48 state; returns (values,
+    new state)"""
+    st = C.c_ulonglong(int(state))
+    out = np.zeros(2 * npair)
+    for k in range(npair):
+        lib().or_get_normalpair(C.byref(st), _ptr(out[2 * k:], _dp))
+    return out, st.value
+
+
+def addnoise(nmem, deflation, sigma, state, v):
+    """addnoise on v: float32 [nmem][...] updated in place; returns the new
+    drand48 state"""
+    st = C.c_ulonglong(int(state))
+    n = int(np.prod(v.shape[1:]))
+    rows = (_fp * nmem)(*[v[e].ctypes.data_as(_fp) for e in range(nmem)])
+    lib().or_addnoise(nmem, float(deflation), float(sigma), C.byref(st), n, rows)
+    return st.value
+
+
+def update_HE(mode, nmem, nmem_dynamic, stride, periodic_i, mnj, mni, T, w, fi, fj, status, inflation0, inf_ratio,
+              HE):
+    """update_HE_structured: returns HE^a ([nmem][nobs] float32) from HE^f;
+    T [nj_s][ni_s][md][m], w [nj_s][ni_s][m]; per-obs inflation0, inf_ratio"""
+    T = np.ascontiguousarray(T, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    nj_s, ni_s = w.shape[0], w.shape[1]
+    nl = np.ones((mnj, mni), np.int32)
+    sp = UpdateSpec(mode, nmem, nmem_dynamic, mni, mnj, stride, periodic_i, 0, _ptr(nl, _ip), 0, ni_s, nj_s,
+                    _ptr(T, _fp), _ptr(w, _fp))
+    out = np.ascontiguousarray(HE, np.float32).copy()
+    nobs = out.shape[1]
+    fi = np.ascontiguousarray(fi, np.float64)
+    fj = np.ascontiguousarray(fj, np.float64)
+    status = np.ascontiguousarray(status, np.int16)
+    inflation0 = np.ascontiguousarray(inflation0, np.float32)
+    inf_ratio = np.ascontiguousarray(inf_ratio, np.float64)
+    r = lib().or_update_HE(C.byref(sp), nobs, _ptr(fi, _dp), _ptr(fj, _dp), _ptr(status, _sp), _ptr(inflation0, _fp),
+                           _ptr(inf_ratio, _dp), _ptr(out, _fp))
+    if r != 0:
+        raise RuntimeError("or_update_HE: %d" % r)
+    return out

@@ -29,6 +29,7 @@
 #include <math.h>
 #include <limits.h>
 #include <stdint.h>
+#include <setjmp.h>
 #include "kdtree.h"
 #include "enkf_oracle.h"
 
@@ -41,16 +42,36 @@ void calc_wT_etkf(int m, int mout, int p, double* s, double** S, double** Sa, do
 
 long int seed_rand48 = 0;
 
+/* enkf_quit() (common/utils.c:59-99) ends the reference process.  Here it
+ * returns control to the harness entry point that is running on this thread:
+ * each entry point arms `quit_env` with setjmp, and enkf_quit longjmps back
+ * to it, so the caller (a test or the CPU baseline) gets an error status and
+ * the message from ref_last_error() instead of an aborted process.  Storage
+ * allocated inside the interrupted call is not reclaimed (an error path).
+ * Without an armed entry point (a direct call of a reference routine) the
+ * message is printed and the process aborts, as the reference does. */
+static __thread jmp_buf quit_env;
+static __thread int quit_armed = 0;
+static __thread char quit_msg[512];
+
+const char* ref_last_error(void)
+{
+    return quit_msg;
+}
+
 void enkf_quit(char* format, ...)
 {
     va_list args;
 
-    fflush(stdout);
-    fprintf(stderr, "\n  ERROR: reference harness: ");
     va_start(args, format);
-    vfprintf(stderr, format, args);
+    vsnprintf(quit_msg, sizeof quit_msg, format, args);
     va_end(args);
-    fprintf(stderr, "\n");
+    if (quit_armed) {
+        quit_armed = 0;
+        longjmp(quit_env, 1);
+    }
+    fflush(stdout);
+    fprintf(stderr, "\n  ERROR: reference harness: %s\n", quit_msg);
     abort();
 }
 
@@ -88,7 +109,7 @@ void* alloc2d(size_t nj, size_t ni, size_t unitsize)
  * S: p x m column-major (S[e * p + o]), s: [p].  Outputs T (m x mout
  * column-major, i.e. C [mout][m]), w [m], dfs, srf (double, before the
  * float store of the reference), G (m x p column-major) optional. */
-int ref_local_analysis(int mode, int scheme, int m, int mout, int p, const double* S, const double* s, double* T, double* w, double* dfs, double* srf, double* Gout)
+static int local_analysis_body(int mode, int scheme, int m, int mout, int p, const double* S, const double* s, double* T, double* w, double* dfs, double* srf, double* Gout)
 {
     double** Sloc = alloc2d(m, p, sizeof(double));
     double** G = alloc2d(p, m, sizeof(double));
@@ -122,6 +143,20 @@ int ref_local_analysis(int mode, int scheme, int m, int mout, int p, const doubl
     free(Tm);
     free(Mstore);
     return 0;
+}
+
+/* Entry point: returns 0, or -1 when a reference routine called enkf_quit
+ * (message in ref_last_error()). */
+int ref_local_analysis(int mode, int scheme, int m, int mout, int p, const double* S, const double* s, double* T, double* w, double* dfs, double* srf, double* Gout)
+{
+    int r;
+
+    if (setjmp(quit_env))
+        return -1;
+    quit_armed = 1;
+    r = local_analysis_body(mode, scheme, m, mout, p, S, s, T, w, dfs, srf, Gout);
+    quit_armed = 0;
+    return r;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -236,7 +271,7 @@ ref_trees* ref_trees_create(const or_obs* obs, const short* status_at_build)
  * kernels (search, algebra).  Same arguments and outputs as
  * or_calc_transforms (oracle/enkf_oracle.h), the trees from
  * ref_trees_create(). */
-int ref_calc_transforms(const or_calcspec* spec, const or_obs* obs, const ref_trees* rt, const float* S, const double* s, int nnode, const double* lon, const double* lat, int geographic, const unsigned char* visible, float* Tout, float* wout, double* T64, double* w64, int* pnlobs, float* pdfs, float* psrf, or_calcstats* stats, int node0, int nnode_do)
+static int calc_transforms_body(const or_calcspec* spec, const or_obs* obs, const ref_trees* rt, const float* S, const double* s, int nnode, const double* lon, const double* lat, int geographic, const unsigned char* visible, float* Tout, float* wout, double* T64, double* w64, int* pnlobs, float* pdfs, float* psrf, or_calcstats* stats, int node0, int nnode_do)
 {
     int m = spec->nmem, md = spec->nmem_dynamic, nt = obs->ntypes;
     int dotT = (spec->mode == OR_MODE_ENKF || spec->mode == OR_MODE_HYBRID);
@@ -438,4 +473,18 @@ int ref_calc_transforms(const or_calcspec* spec, const or_obs* obs, const ref_tr
     free(T);
     free(Mstore);
     return 0;
+}
+
+/* Entry point: returns 0, or -1 when a reference routine called enkf_quit
+ * (message in ref_last_error()). */
+int ref_calc_transforms(const or_calcspec* spec, const or_obs* obs, const ref_trees* rt, const float* S, const double* s, int nnode, const double* lon, const double* lat, int geographic, const unsigned char* visible, float* Tout, float* wout, double* T64, double* w64, int* pnlobs, float* pdfs, float* psrf, or_calcstats* stats, int node0, int nnode_do)
+{
+    int r;
+
+    if (setjmp(quit_env))
+        return -1;
+    quit_armed = 1;
+    r = calc_transforms_body(spec, obs, rt, S, s, nnode, lon, lat, geographic, visible, Tout, wout, T64, w64, pnlobs, pdfs, psrf, stats, node0, nnode_do);
+    quit_armed = 0;
+    return r;
 }

@@ -7,6 +7,8 @@ import os
 import re
 import subprocess
 
+import numpy as np
+
 import pytest
 
 import enkfg
@@ -105,3 +107,15 @@ def test_no_gpu_fails_loudly():
     with pytest.raises(enkfg.EnkfgError) as ei:
         enkfg.Engine(24)
     assert ei.value.code == enkfg.ERR_HIP
+
+
+def test_engine_noise_generator_matches_oracle():
+    """The engine's RANDOMISE stream (host side, callable without a device)
+    draws the oracle's get_normalpair values bit for bit and advances the
+    state as the reference's drand48 would."""
+    import oracle as O
+    for state in (0, 77, (1 << 48) - 5):
+        got, st = enkfg.rand48_normals(state, 37)
+        want, st_o = O.normalpairs(state, 37)
+        assert np.array_equal(got, want)
+        assert st == st_o

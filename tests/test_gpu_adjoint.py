@@ -182,3 +182,139 @@ def test_adjoint_requires_the_calc_and_the_ensemble(case):
         eng.adjoint(dJ.data_ptr(), case.nobs)
     assert ei.value.code == enkfg.ERR_STATE
     eng.close()
+
+
+def test_adjoint_c5_m96_against_autograd_and_finite_differences():
+    """The adjoint at the C5 size: 32 layers x 600 x 600, m = 96, 1M surface
+    obs, LOCRAD 200 km, stride 3 (40,000 local analyses).  J is the squared
+    deviation of the analysis mean from a target over a 100 x 100 box of the
+    top 10 layers of temp and salt (20 fields).  The GPU forward is the whole
+    FP64 calc; the reference is the same forward in float64 torch over the
+    1,225 stride nodes whose transforms reach the box, differentiated by
+    autograd, and central differences (h = 1e-6, relative for the variances)
+    of that forward on the most sensitive observations."""
+    import time
+    import synth_gpu
+    print("case adjoint C5: 32x600x600, m=96, 1M obs", flush=True)
+    X = synth_gpu.make_tile(nj=600, ni=600, nk=32, nmem=96, n_sst=1_000_000, n_sla=0, n_prof=0, corr_km=150.0,
+                            locrad=200.0, obs_std=(0.3, 0.04, 0.05), seed=5)
+    m, s_ = X.nmem, X.stride
+    nobs, ncell = X.nobs, X.ncell
+    eng = enkfg.Engine(m, mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF, keep_fp64=True)
+    eng.set_obstypes(X.obstypes)
+    eng.set_obs(X.obs_type, X.obs_status, X.obs_lon, X.obs_lat, X.obs_estd)
+    eng.set_S(X.HA, X.innov, standardised=False)
+    eng.calc_transforms(X.node_lon, X.node_lat, stride=s_)
+    _, _, _, w64 = eng.transforms(fp64=True)
+    eng.ens_attach(X.nlevels, s_, 0, 0, X.fields, X.E.ptr)
+    # the box and the nodes whose interpolation weights reach it
+    j0, i0, nb = 250, 250, 100
+    fsel = list(range(10)) + list(range(X.nk, X.nk + 10))
+    rows = np.arange(j0, j0 + nb)
+    cols = np.arange(i0, i0 + nb)
+    cells = (rows[:, None] * X.ni + cols[None, :]).ravel()
+    wet = np.stack([X.nlevels.ravel()[cells] > X.fields[f]["level"] for f in fsel])   # [f][cell]
+    Af = np.stack([np.concatenate([X.E.read((f * ncell + j * X.ni + i0) * m, nb * m).reshape(nb, m) for j in rows])
+                   for f in fsel]).astype(np.float64)                         # [f][cell][member]
+    xf = Af.mean(2)
+    Af -= xf[:, :, None]
+    jjs = sorted(set(rows // s_) | set(np.minimum(rows // s_ + 1, X.nj_s - 1)))
+    iis = sorted(set(cols // s_) | set(np.minimum(cols // s_ + 1, X.ni_s - 1)))
+    nodes = np.array([jj * X.ni_s + ii for jj in jjs for ii in iis])
+    pos = {int(n): k for k, n in enumerate(nodes)}
+    Bm = np.zeros((len(cells), len(nodes)))
+    one = np.float32(1.0)
+    for c, (j, i) in enumerate((j, i) for j in rows for i in cols):
+        jj, sj, ii, si = j // s_, j % s_, i // s_, i % s_
+        jj2, ii2 = min(jj + 1, X.nj_s - 1), min(ii + 1, X.ni_s - 1)
+        wj2 = np.float32(sj) / np.float32(s_)
+        wi2 = np.float32(si) / np.float32(s_)
+        bj = (1.0, 0.0) if sj == 0 else (float(one - wj2), float(wj2))
+        bi = (1.0, 0.0) if si == 0 else (float(one - wi2), float(wi2))
+        for nj_, bjv in ((jj, bj[0]), (jj2, bj[1])):
+            for ni_, biv in ((ii, bi[0]), (ii2, bi[1])):
+                Bm[c, pos[nj_ * X.ni_s + ni_]] += bjv * biv
+    Bt = torch.tensor(Bm)
+    Aft = torch.tensor(Af)
+    xft = torch.tensor(xf)
+    wett = torch.tensor(wet.astype(np.float64))
+    jg, ig = np.meshgrid(rows, cols, indexing="ij")
+    patt = (np.cos(2 * np.pi * ig / 50.0) * np.sin(2 * np.pi * jg / 70.0)).ravel()
+    xt = torch.tensor(xf + np.array([0.3 if f < X.nk else 0.03 for f in fsel])[:, None] * patt[None, :])
+
+    def J_of_W(W):
+        xa = xft + (Aft * (Bt @ W)[None]).sum(2)
+        return (((xa - xt) ** 2) * wett).sum(), xa
+
+    # GPU: forward mean from w64, backward via enkfg_adjoint
+    J_gpu, xa_gpu = J_of_W(torch.tensor(w64[nodes]))
+    dJ = np.zeros((X.nfield, ncell), np.float32)
+    g = (2.0 * (xa_gpu - xt) * wett).numpy().astype(np.float32)
+    for k, f in enumerate(fsel):
+        dJ[f, cells] = g[k]
+    ddJ = synth_gpu.DeviceArray.from_host(dJ)
+    t = time.perf_counter()
+    gy, gR, gi = eng.adjoint(ddJ.ptr, nobs)
+    t_adj = time.perf_counter() - t
+    eng.close()
+    assert gi == 0.0
+    # reference: float64 forward over the listed nodes, autograd
+    ob = O.OracleObs(X.obstypes, X.obs_type, X.obs_lon, X.obs_lat, X.obs_estd)
+    lists = [ob.findlocal(X.node_lon.ravel()[n], X.node_lat.ravel()[n]) for n in nodes]
+    HA = torch.tensor(X.HA.astype(np.float64))
+    sq = math.sqrt(m - 1)
+
+    def node_w(k, innov, R):
+        ids, lc = lists[k]
+        if len(ids) == 0:
+            return torch.zeros(m, dtype=torch.float64)
+        it = torch.tensor(ids, dtype=torch.long)
+        d = torch.sqrt(R[it]) * sq
+        lct = torch.tensor(lc)
+        S = HA[:, it].T / d[:, None] * lct[:, None]
+        sv = innov[it] / d * lct
+        M = torch.eye(m, dtype=torch.float64) + S.T @ S
+        return torch.linalg.solve(M, S.T @ sv)
+
+    R0 = torch.tensor(X.obs_estd.astype(np.float64) ** 2)
+    inn0 = torch.tensor(X.innov)
+    iv = inn0.clone().requires_grad_(True)
+    Rv = R0.clone().requires_grad_(True)
+    t = time.perf_counter()
+    W = torch.stack([node_w(k, iv, Rv) for k in range(len(nodes))])
+    Jr, _ = J_of_W(W)
+    Jr.backward()
+    t_ref = time.perf_counter() - t
+    gy_ref, gR_ref = iv.grad.numpy(), Rv.grad.numpy()
+    ey = float(np.abs(gy - gy_ref).max() / np.abs(gy_ref).max())
+    eR = float(np.abs(gR - gR_ref).max() / np.abs(gR_ref).max())
+    print("  %d nodes reach the box, %d obs with dJ/dy != 0; J %.6g (GPU forward %.6g)" % (
+        len(nodes), int((gy_ref != 0).sum()), Jr.item(), J_gpu.item()), flush=True)
+    print("  enkfg_adjoint %.2f s (torch float64 reference %.1f s); error vs autograd: y %.2g  R %.2g"
+          % (t_adj, t_ref, ey, eR), flush=True)
+    assert abs(J_gpu.item() - Jr.item()) <= 1e-5 * abs(Jr.item())
+    assert (gy_ref != 0).sum() > 10000
+    assert ey <= 1e-4 and eR <= 1e-4
+    # central differences on the most sensitive obs: only the nodes whose
+    # list holds the observation change
+    with torch.no_grad():
+        W0 = W.detach().clone()
+        holds = {}
+        for name, base, gg in (("y", inn0, gy), ("R", R0, gR)):
+            worst = 0.0
+            for o in np.argsort(-np.abs(gg))[:4]:
+                o = int(o)
+                ks = [k for k in range(len(nodes)) if o in holds.setdefault(k, set(lists[k][0].tolist()))]
+                h = 1e-6 if name == "y" else 1e-6 * float(base[o])
+                Js = []
+                for sgn in (1.0, -1.0):
+                    x = base.clone()
+                    x[o] += sgn * h
+                    Wp = W0.clone()
+                    for k in ks:
+                        Wp[k] = node_w(k, x, R0) if name == "y" else node_w(k, inn0, x)
+                    Js.append(J_of_W(Wp)[0].item())
+                fd = (Js[0] - Js[1]) / (2 * h)
+                worst = max(worst, abs(fd - float(gg[o])) / float(np.abs(gg).max()))
+            print("  finite differences %s: max error %.2g" % (name, worst), flush=True)
+            assert worst <= 1e-4

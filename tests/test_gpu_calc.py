@@ -243,6 +243,8 @@ CASES = [
     (enkfg.MODE_ENOI, enkfg.SCHEME_DENKF, 40, 40, 1.0, 0),
     (enkfg.MODE_HYBRID, enkfg.SCHEME_DENKF, 48, 32, 1.0, 20),
     (enkfg.MODE_HYBRID, enkfg.SCHEME_DENKF, 144, 96, 1.0, 20),    # 96 dynamic + 48 static members
+    (enkfg.MODE_HYBRID, enkfg.SCHEME_ETKF, 144, 96, 1.0, 20),     # ETKF, m > 96: iterates in device memory
+    (enkfg.MODE_ENKF, enkfg.SCHEME_ETKF, 160, 160, 0.8, 10),      # the largest supported m
 ]
 
 

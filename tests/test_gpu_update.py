@@ -148,3 +148,89 @@ def test_update_bg_enoi():
             k, len(d), d[:3].tolist(), [float(a[tuple(x)]) for x in d[:3]], [float(b[tuple(x)]) for x in d[:3]])
         assert len(d) == 0, "EnOI background update is float-sequential and must match bit for bit; " + msg
     E.close()
+
+
+@pytest.mark.parametrize("precision", [enkfg.PREC_FP64, enkfg.PREC_MIXED])
+def test_update_randomise(precision):
+    """RANDOMISE fields (addnoise, update.c:46-87, applied after the update
+    at 554-563): the engine's drand48 stream from a set state against the
+    oracle's update followed by addnoise on the same state, field by field in
+    batch order, for the host-buffer call and for the device-resident
+    ensemble; the streams end in the same state."""
+    print("case update randomise precision=%d" % precision, flush=True)
+    m = 24
+    P = synth.make_problem(seed=41, nmem=m, nobs=1000, locrad=60.0, nj=25, ni=31, stride=3, land_frac=0.1)
+    T, w = transforms(P, 1, 1, m)
+    bufs, fl = fields_of(P, m)
+    for k in (0, 3, len(fl) - 1):           # three randomised fields, one of them 2D
+        fl[k] = dict(fl[k], deflation=0.9 - 0.1 * (k % 2), sigma=0.2 + 0.01 * k)
+    state0 = 0x5EED1234
+    ref = [b.copy() for b in bufs]
+    O.update_fields(1, m, m, 3, 0, 0, P.nlevels, T, w, fl, ref, 0, P.nj - 1)
+    st = state0
+    for k, f in enumerate(fl):
+        if "deflation" in f:
+            st = O.addnoise(m, f["deflation"], f["sigma"], st, ref[k])
+    tol = 1e-5 if precision == enkfg.PREC_FP64 else 2e-5
+    E = enkfg.Engine(m, mode=enkfg.MODE_ENKF, precision=precision)
+    E.set_transforms(T, w, 3)
+    assert E.rand48() == 0, "the stream starts where an unseeded drand48 starts"
+    E.set_rand48(state0)
+    gpu = [b.copy() for b in bufs]
+    E.update_fields(P.nlevels, 3, 0, 0, fl, gpu, 0, P.nj - 1)
+    assert E.rand48() == st
+    for k, (a, b) in enumerate(zip(gpu, ref)):
+        scale = max(np.abs(b).max(), 1.0)
+        assert np.abs(a.astype(np.float64) - b).max() <= tol * scale, "field %d" % k
+    # device-resident ensemble: the same fields in [field][cell][member]
+    import synth_gpu
+    Ecell = np.stack([b.reshape(m, -1).T for b in bufs]).astype(np.float32)
+    dE = synth_gpu.DeviceArray.from_host(Ecell)
+    E.set_rand48(state0)
+    E.ens_attach(P.nlevels, 3, 0, 0, fl, dE.ptr)
+    E.ens_update()
+    assert E.rand48() == st
+    out = dE.to_host()
+    for k, b in enumerate(ref):
+        scale = max(np.abs(b).max(), 1.0)
+        assert np.abs(out[k].T.reshape(b.shape).astype(np.float64) - b).max() <= tol * scale, "field %d" % k
+    E.close()
+
+
+@pytest.mark.parametrize("mode,m,md,stride,periodic", [(enkfg.MODE_ENKF, 24, 24, 3, 0), (enkfg.MODE_ENKF, 96, 96, 3, 1),
+                                                      (enkfg.MODE_HYBRID, 40, 28, 2, 0),
+                                                      (enkfg.MODE_ENOI, 30, 30, 3, 0)])
+def test_update_HE_parity(mode, m, md, stride, periodic):
+    """The analysed ensemble observations (update_HE_structured /
+    update_Hx_structured) against the oracle: obs anywhere in the grid,
+    several per cell (profiles), capped and plain inflation, obs not OK or
+    rounding past the last row left unchanged; within 1e-5 of max |HE|."""
+    print("case update_HE mode=%d m=%d md=%d stride=%d periodic=%d" % (mode, m, md, stride, periodic), flush=True)
+    P = synth.make_problem(seed=50 + m, nmem=m, nobs=1500, locrad=70.0, nj=31, ni=40, stride=stride)
+    scheme = enkfg.SCHEME_DENKF
+    T, w = transforms(P, mode, scheme, md)
+    rng = np.random.default_rng(m + stride)
+    nobs = 3000
+    fi = rng.uniform(-0.49, P.ni - 0.51, nobs)
+    fj = rng.uniform(-0.49, P.nj - 0.51, nobs)
+    fi[:200] = np.repeat(fi[:10], 20)            # 20 obs per cell
+    fj[:200] = np.repeat(fj[:10], 20)
+    fi[200:210] = P.ni - 0.5                     # rounds to mni: the last column
+    fj[210:220] = P.nj - 0.4                     # rounds past the last row: unchanged
+    status = np.zeros(nobs, np.int16)
+    status[300:340] = 2
+    HE = (10.0 + rng.standard_normal((m, nobs))).astype(np.float32)
+    infl = rng.uniform(1.0, 1.2, nobs).astype(np.float32)
+    ratio = np.where(rng.random(nobs) < 0.5, 1.0, np.nan)
+    Tt = np.zeros((P.nj_s, P.ni_s, 1, 1), np.float32) if T is None else T
+    ref = O.update_HE(mode, m, md, stride, periodic, P.nj, P.ni, Tt, w, fi, fj, status, infl, ratio, HE)
+    E = enkfg.Engine(m, nmem_dynamic=md, mode=mode)
+    E.set_transforms(T, w, stride)
+    got = E.update_HE(P.nj, P.ni, stride, periodic, fi, fj, status, infl, ratio, HE)
+    E.close()
+    same = np.all(ref == HE, axis=0)
+    assert same[210:220].all() and same[300:340].all()
+    assert np.array_equal(got[:, same], HE[:, same])
+    err = float(np.abs(got.astype(np.float64) - ref).max() / np.abs(ref).max())
+    print("  max rel error %.2e, %d obs updated" % (err, int((~same).sum())), flush=True)
+    assert err <= 1e-5

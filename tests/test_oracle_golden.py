@@ -93,3 +93,37 @@ def test_standardise_fixture_inputs():
     div = g["obs_estd"].astype(np.float64) * np.sqrt(g["rfactor"][g["obs_type"]]) * np.sqrt(23.0)
     assert np.array_equal(S, (g["HA"].astype(np.float64) / div[None, :]).astype(np.float32))
     assert np.array_equal(s, g["innov"] / div)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (needs the reference tree and MKL)")
+def test_reference_harness_reports_lapack_failure():
+    """calcs.c calls enkf_quit when dpotrf fails (calcs.c:109-118).  The
+    harness turns that into an error status with the reference's message,
+    and the process carries on: the next analysis on the same thread works."""
+    rng = np.random.default_rng(3)
+    S = rng.standard_normal((40, 12)) * 0.3
+    bad = S.copy()
+    bad[5, 3] = np.nan
+    with pytest.raises(RuntimeError, match="dpotrf"):
+        O.local_analysis(O.MODE_ENKF, O.SCHEME_DENKF, 12, 12, bad, np.ones(40), use_ref=True)
+    T, w, _, _ = O.local_analysis(O.MODE_ENKF, O.SCHEME_DENKF, 12, 12, S, np.ones(40), use_ref=True)
+    To, wo, _, _ = O.local_analysis(O.MODE_ENKF, O.SCHEME_DENKF, 12, 12, S, np.ones(40))
+    assert relmax(T, To) < 1e-12 and relmax(w, wo) < 1e-12
+
+
+@pytest.mark.parametrize("use_ref", [False, True])
+def test_threaded_calc_loop_equals_serial(use_ref):
+    """The calc loop split over threads (disjoint node runs, per-thread
+    reference trees) writes what the serial loop writes, bit for bit."""
+    if use_ref and not O.ref_available():
+        pytest.skip("oracle/_ref not built")
+    import synth
+    P = synth.make_problem(seed=8, nj=24, ni=30, nmem=16, nobs=1500, locrad=90.0, profiles=20)
+    ob = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd)
+    S, s = O.standardise(P.nmem, ob, P.HA, P.innov)
+    args = (O.MODE_ENKF, O.SCHEME_ETKF, P.nmem, P.nmem, 1.0, ob, S, s, P.node_lon.ravel(), P.node_lat.ravel())
+    a = O.calc_transforms(*args, use_ref=use_ref)
+    b = O.calc_transforms(*args, use_ref=use_ref, threads=5)
+    for k in ("T64", "w64", "pnlobs", "pdfs", "psrf"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["stats"] == b["stats"]

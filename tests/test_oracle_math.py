@@ -160,6 +160,43 @@ def test_findlocal_nlobsmax_brute_force(nmax):
         assert ties > 0, "the case must contain equal distances"
 
 
+def test_findlocal_global_node_and_visibility_brute_force():
+    """The two branches of obs_findlocal (observations.c:1837-1927) that do
+    not search a tree, derived from the obs arrays alone: a node with NaN
+    coordinates gets, type by type in ascending type order, every observation
+    of the type that was good when the trees were built (obs_getglobal,
+    observations.c:1773-1817; ascending index, taper 1, no search-time status
+    filter); a type that is not visible from the grid's domain, or is
+    stats-only, contributes nothing to any node (observations.c:1877-1897)."""
+    P = synth.make_problem(seed=6, nj=10, ni=12, nmem=12, nobs=900, locrad=150.0, profiles=30)
+    types = [dict(o) for o in P.obstypes] + [dict(locrad=[150.0], rfactor=1.0, statsonly=1)]
+    otype = P.obs_type.copy()
+    otype[::17] = 2                       # a stats-only type
+    build = P.obs_status.copy()
+    build[::9] = 5                        # bad at tree build: never listed
+    status = build.copy()
+    status[1::7] = 4                      # bad after the build: searched lists drop them, global lists keep them
+    ob = O.OracleObs(types, otype, P.obs_lon, P.obs_lat, P.obs_estd, status=status, status_at_build=build)
+    full = O.OracleObs(types, otype, P.obs_lon, P.obs_lat, P.obs_estd, status=status, status_at_build=build)
+    for vis in (None, np.array([1, 1, 1], np.uint8), np.array([0, 1, 1], np.uint8), np.array([1, 0, 0], np.uint8)):
+        visible = np.ones(3, bool) if vis is None else vis.astype(bool)
+        ids, lc = ob.findlocal(float("nan"), 10.0, visible=vis)
+        want = np.concatenate([np.flatnonzero((otype == t) & (build == 0))
+                               for t in range(2) if visible[t]] + [np.zeros(0, np.int64)])
+        assert np.array_equal(ids, want.astype(np.int32))
+        assert np.all(lc == 1.0)
+        assert (status[ids] != 0).any(), "the global list must keep obs bad at search time"
+        # a geographic node: the visible types' lists, in type order, are the
+        # all-visible lists restricted to those types
+        for lon, lat in zip(P.node_lon.ravel()[::11], P.node_lat.ravel()[::11]):
+            ids_v, lc_v = ob.findlocal(lon, lat, visible=vis)
+            ids_a, lc_a = full.findlocal(lon, lat)
+            keep = visible[otype[ids_a]]
+            assert np.array_equal(ids_v, ids_a[keep]) and np.array_equal(lc_v, lc_a[keep])
+            assert not np.isin(otype[ids_a], [2]).any(), "stats-only types are never searched"
+            assert np.all(status[ids_a] == 0)
+
+
 # ---------------------------------------------------------------- per-column algebra
 def spd_inv_sqrt(M):
     lam, U = np.linalg.eigh(M)
@@ -373,3 +410,157 @@ def test_sethybridensemble_closed_form():
     assert np.allclose(out[:md], (v[:md] - mean) * kd + mean, rtol=1e-6, atol=1e-6)
     assert np.allclose(out[md:], v[md:] * ks + mean, rtol=1e-6, atol=1e-6)
     assert np.allclose(out[:md].astype(np.float64).mean(0), mean, atol=1e-6)
+
+
+# ---------------------------------------------------------------- RANDOMISE
+def test_drand48_matches_glibc_and_default_state():
+    """or_drand48 is the POSIX drand48 recurrence: it matches glibc's
+    erand48 on the same 48-bit state draw for draw, and a process that never
+    seeds drand48 (the reference seeds srand(), common/utils.c:102-122)
+    starts from state 0."""
+    import ctypes as C
+    import subprocess
+    import sys
+    libc = C.CDLL("libc.so.6")
+    libc.erand48.restype = C.c_double
+    libc.erand48.argtypes = [C.POINTER(C.c_ushort)]
+    for seed in (0, 1, 0x1234ABCD330E, (1 << 48) - 1):
+        xs = (C.c_ushort * 3)(seed & 0xFFFF, (seed >> 16) & 0xFFFF, (seed >> 32) & 0xFFFF)
+        st = C.c_ulonglong(seed)
+        for _ in range(200):
+            assert O.lib().or_drand48(C.byref(st)) == libc.erand48(xs)
+    code = ("import ctypes; L = ctypes.CDLL('libc.so.6'); L.drand48.restype = ctypes.c_double; "
+            "print(repr(L.drand48())); print(repr(L.drand48()))")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout.split()
+    st = C.c_ulonglong(0)
+    assert [float(x) for x in out] == [O.lib().or_drand48(C.byref(st)), O.lib().or_drand48(C.byref(st))]
+
+
+def test_normalpair_and_addnoise_closed_forms():
+    """get_normalpair is Box-Muller on two draws; addnoise adds the centred
+    draws of the first nmem normals, scaled by sqrt(1 - d^2) sigma, to
+    v * d, with the reference's float / double rounding."""
+    import ctypes as C
+    st = C.c_ulonglong(12345)
+    d1 = O.lib().or_drand48(C.byref(st))
+    d2 = O.lib().or_drand48(C.byref(st))
+    x, st2 = O.normalpairs(12345, 1)
+    u = math.sqrt(-2.0 * math.log(1.0 - d1))
+    assert abs(x[0] - u * math.cos(2 * math.pi * d2)) <= 4e-16 * u
+    assert abs(x[1] - u * math.sin(2 * math.pi * d2)) <= 4e-16 * u
+    assert st2 == st.value
+    for nmem in (7, 8):
+        rng = np.random.default_rng(nmem)
+        v = rng.standard_normal((nmem, 5, 6)).astype(np.float32)
+        d, sigma = 0.8, 0.3
+        r, _ = O.normalpairs(99, (nmem + 1) // 2)
+        r = r[:nmem] - r[:nmem].sum() / nmem
+        mult = np.float32(math.sqrt(1.0 - d * d) * sigma)
+        want = ((v * np.float32(d)).astype(np.float64) + np.float64(mult) * r[:, None, None]).astype(np.float32)
+        got = v.copy()
+        st_after = O.addnoise(nmem, d, sigma, 99, got)
+        assert np.array_equal(got, want)
+        assert st_after == O.normalpairs(99, (nmem + 1) // 2)[1]
+        # zero-mean noise: the member mean moves by d only
+        assert np.allclose(got.astype(np.float64).mean(0), d * v.astype(np.float64).mean(0), atol=1e-6)
+
+
+# ---------------------------------------------------------------- analysed observations
+@pytest.mark.parametrize("mode,md", [(1, 24)])
+def test_update_HE_is_the_field_update_at_the_obs_cell(mode, md):
+    """update_HE_structured (ensobs.c:1228-1553) applies the update of
+    update.c to the ensemble observation vector at the obs' nearest cell
+    ((int)(f + 0.5), the last column clamped): for obs whose HE^f is the
+    field at that cell, HE^a is the updated field there (capped and plain
+    inflation), and obs that are not OK or whose row rounds past the grid are
+    left unchanged."""
+    m = 24
+    P = synth.make_problem(seed=21 + md, nj=20, ni=26, nk=1, nmem=m, nobs=500, locrad=80.0, stride=3)
+    ob = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd)
+    S, s = O.standardise(m, ob, P.HA, P.innov)
+    r = O.calc_transforms(mode, 1, m, md, 1.0, ob, S, s, P.node_lon.ravel(), P.node_lat.ravel())
+    T = r["T"].reshape(P.nj_s, P.ni_s, md, m)
+    w = r["w"].reshape(P.nj_s, P.ni_s, m)
+    rng = np.random.default_rng(md)
+    nobs = 400
+    fi = rng.uniform(-0.49, P.ni - 0.51, nobs)
+    fj = rng.uniform(-0.49, P.nj - 0.51, nobs)
+    fi[:5] = P.ni - 0.5           # rounds to mni: clamped to the last column
+    fj[5:8] = P.nj - 0.4          # rounds past the last row: not processed
+    status = np.zeros(nobs, np.int16)
+    status[10:15] = 3
+    field = P.ens["temp"][:, 0]                      # [m][nj][ni]
+    ii = np.minimum((fi + 0.5).astype(int), P.ni - 1)
+    jj = np.minimum((fj + 0.5).astype(int), P.nj - 1)
+    HE = np.ascontiguousarray(field[:, jj, ii])
+    for ratio in (1.0, float("nan")):
+        infl0 = np.full(nobs, 1.08, np.float32)
+        rat = np.full(nobs, ratio)
+        HEa = O.update_HE(mode, m, md, 3, 0, P.nj, P.ni, T, w, fi, fj, status, infl0, rat, HE)
+        buf = np.ascontiguousarray(field.copy())
+        nl = np.ones((P.nj, P.ni), np.int32)
+        O.update_fields(mode, m, md, 3, 0, 0, nl, T, w, [dict(level=0, inflation=1.08, inf_ratio=ratio)], [buf], 0,
+                        P.nj - 1)
+        want = buf[:, jj, ii]
+        ok = (status == 0) & ((fj + 0.5).astype(int) < P.nj)
+        assert np.allclose(HEa[:, ok], want[:, ok], rtol=0, atol=2e-6 * np.abs(want).max())
+        assert np.array_equal(HEa[:, ~ok], HE[:, ~ok])
+        assert not np.array_equal(HEa[:, ok], HE[:, ok])
+
+
+def test_update_HE_hybrid_at_stride_nodes_brute_force():
+    """Hybrid mode at obs on stride-node cells (T there is the node's T):
+    dynamic members HE^a = (HA^f T + x^a) with x^a = x^f + HA^f w and plain
+    inflation about x^a, no k_d rescaling (unlike update.c:413-427); static
+    members shift by x^a - x^f (ensobs.c:1524-1533)."""
+    m, md = 24, 18
+    P = synth.make_problem(seed=5, nj=19, ni=25, nk=1, nmem=m, nobs=500, locrad=80.0, stride=3)
+    ob = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd)
+    S, s = O.standardise(m, ob, P.HA, P.innov)
+    r = O.calc_transforms(3, 1, m, md, 1.0, ob, S, s, P.node_lon.ravel(), P.node_lat.ravel())
+    T = r["T"].reshape(P.nj_s, P.ni_s, md, m).astype(np.float64)
+    w = r["w"].reshape(P.nj_s, P.ni_s, m).astype(np.float64)
+    rng = np.random.default_rng(2)
+    nobs = 60
+    jn = rng.integers(0, P.nj_s, nobs)
+    inn = rng.integers(0, P.ni_s, nobs)
+    fi = inn * 3 + rng.uniform(-0.4, 0.4, nobs)
+    fj = jn * 3 + rng.uniform(-0.4, 0.4, nobs)
+    HE = (15.0 + rng.standard_normal((m, nobs))).astype(np.float32)
+    infl = 1.1
+    HEa = O.update_HE(3, m, md, 3, 0, P.nj, P.ni, T.astype(np.float32), w.astype(np.float32), fi, fj,
+                      np.zeros(nobs, np.int16), np.full(nobs, infl, np.float32), np.full(nobs, np.nan), HE)
+    X = HE.astype(np.float64)
+    for o in range(nobs):
+        Tn, wn = T[jn[o], inn[o]], w[jn[o], inn[o]]
+        xf = X[:md, o].mean()
+        A = X[:, o] - xf
+        xa = xf + A @ wn
+        dyn = (Tn @ A) * infl + xa
+        assert np.allclose(HEa[:md, o], dyn, rtol=0, atol=3e-5 * np.abs(dyn).max())
+        assert np.allclose(HEa[md:, o], X[md:, o] + xa - xf, rtol=0, atol=3e-5 * np.abs(dyn).max())
+
+
+def test_update_Hx_enoi_at_stride_nodes_brute_force():
+    """EnOI (update_Hx_structured, ensobs.c:1933-1962): every member of the
+    ensemble observation moves by (HE - mean(HE)) . w, with w of the obs'
+    nearest cell (here stride-node cells, where w is the node's w)."""
+    m = 20
+    P = synth.make_problem(seed=9, nj=19, ni=25, nk=1, nmem=m, nobs=400, locrad=80.0, stride=3)
+    ob = O.OracleObs(P.obstypes, P.obs_type, P.obs_lon, P.obs_lat, P.obs_estd)
+    S, s = O.standardise(m, ob, P.HA, P.innov)
+    r = O.calc_transforms(2, 1, m, m, 1.0, ob, S, s, P.node_lon.ravel(), P.node_lat.ravel())
+    w = r["w"].reshape(P.nj_s, P.ni_s, m)
+    rng = np.random.default_rng(4)
+    nobs = 50
+    jn = rng.integers(0, P.nj_s, nobs)
+    inn = rng.integers(0, P.ni_s, nobs)
+    fi = inn * 3 + rng.uniform(-0.4, 0.4, nobs)
+    fj = jn * 3 + rng.uniform(-0.4, 0.4, nobs)
+    HE = (3.0 + rng.standard_normal((m, nobs))).astype(np.float32)
+    HEa = O.update_HE(2, m, m, 3, 0, P.nj, P.ni, np.zeros((P.nj_s, P.ni_s, 1, 1), np.float32), w, fi, fj,
+                      np.zeros(nobs, np.int16), np.ones(nobs, np.float32), np.full(nobs, 1.0), HE)
+    X = HE.astype(np.float64)
+    d = ((X - X.mean(0)) * w[jn, inn].T.astype(np.float64)).sum(0)
+    assert np.allclose(HEa, X + d[None, :], rtol=0, atol=1e-5)
+    assert np.abs(d).max() > 1e-3
