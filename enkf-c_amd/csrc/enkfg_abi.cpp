@@ -7,11 +7,20 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "enkfg.h"
 #include "enkfg_internal.h"
+
+// Scratch of enkfg_update_fields_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  Host code never uses its name as an
+// address (on the host it is a stand-in); the device address of each GPU's
+// copy comes from hipGetSymbolAddress, taken once per device at context
+// creation with that device current (scratch_address below).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 using namespace enkfg;
 
@@ -146,6 +155,8 @@ struct enkfg_ctx {
     DevBuf d_noise;
     // analysed observations: cell groups and per-obs inflation parameters
     DevBuf d_he_order, d_he_gcell, d_he_goff, d_he_infl, d_he_ratio;
+    // this device's copy of the no-allocation scratch array (device address)
+    unsigned char* scratch = nullptr;
     // adjoint
     DevBuf d_adjg, d_adjflag, d_adjlist, d_adjz, d_adjw, d_gy, d_gR;
     float* attached_E = nullptr;
@@ -457,6 +468,27 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
     return ENKFG_OK;
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's scratch address (device `dev` must be current)
+hipError_t scratch_address(int dev, unsigned char** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (dev < 0 || dev >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -502,6 +534,10 @@ int enkfg_create(enkfg_ctx** out, const enkfg_config* cfg)
     if (c->d_err.reserve(sizeof(int)) != hipSuccess) {
         delete c;
         return ENKFG_ERR_NOMEM;
+    }
+    if (scratch_address(cfg->device, &c->scratch) != hipSuccess) {
+        delete c;
+        return ENKFG_ERR_HIP;
     }
     *out = c;
     return ENKFG_OK;
@@ -1505,6 +1541,112 @@ int enkfg_update_fields(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enk
     }
     if (infl != nullptr)
         HIPCHECK(c, hipMemcpy(infl, c->d_infl.p, (size_t)nfield * ncell * sizeof(float), hipMemcpyDeviceToHost));
+    return ENKFG_OK;
+}
+
+int enkfg_update_fields_noalloc(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enkfg_field* fields, int j1,
+                                int j2, float** fb, float* infl, int flags)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (c->cfg.mode == ENKFG_MODE_ENOI)
+        return fail(c, ENKFG_ERR_ARG, "das_updatefields needs EnKF or hybrid mode");
+    int r = check_grid(c, g);
+    if (r != ENKFG_OK)
+        return r;
+    if (nfield <= 0 || fields == nullptr || fb == nullptr || j1 < 0 || j2 < j1 || j2 >= g->mnj || j1 % g->stride)
+        return fail(c, ENKFG_ERR_ARG, "invalid field batch");
+    for (int f = 0; f < nfield; ++f)
+        if (fb[f] == nullptr)
+            return fail(c, ENKFG_ERR_ARG, "field buffer %d is NULL", f);
+    if (c->scratch == nullptr)
+        return fail(c, ENKFG_ERR_STATE, "no scratch address for this device");
+    const int m = c->cfg.nmem;
+    const int jsize = j2 - j1 + 1;
+    const size_t ncell = (size_t)jsize * g->mni;
+    // scratch layout: one field's parameters, the batch's rows of nlevels,
+    // one field's noise vector, two staging copies of one field (members
+    // major, cells major) and its inflation output; 256-byte aligned
+    size_t off = 0;
+    auto take = [&off](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_lev = take(sizeof(int)), o_surf = take(sizeof(int)), o_log = take(sizeof(int));
+    const size_t o_inf = take(sizeof(float)), o_rat = take(sizeof(double));
+    const size_t o_nlev = take(ncell * sizeof(int));
+    const size_t o_noise = take((size_t)m * sizeof(double));
+    const size_t o_A = take(ncell * m * sizeof(float)), o_B = take(ncell * m * sizeof(float));
+    const size_t o_if = take(ncell * sizeof(float));
+    if (off > kNoAllocScratchBytes)
+        return fail(c, ENKFG_ERR_ARG,
+                    "rows %d..%d of a %d-column grid with %d members need %zu bytes of scratch; the limit of "
+                    "enkfg_update_fields_noalloc is %zu bytes (split the rows into smaller tiles)",
+                    j1, j2, g->mni, m, off, kNoAllocScratchBytes);
+    unsigned char* base = c->scratch;
+    int* d_nlev = reinterpret_cast<int*>(base + o_nlev);
+    float* A = reinterpret_cast<float*>(base + o_A);
+    float* B = reinterpret_cast<float*>(base + o_B);
+    double* d_noise = reinterpret_cast<double*>(base + o_noise);
+    HIPCHECK(c, hipMemcpyAsync(d_nlev, g->nlevels + (size_t)j1 * g->mni, ncell * sizeof(int), hipMemcpyHostToDevice,
+                               c->stream));
+    UpdArgs a = upd_args(c, g, 1, j1, jsize, flags);
+    // the kernels index nlevels by grid row: the staged rows start at j1
+    a.nlevels = d_nlev - (ptrdiff_t)j1 * g->mni;
+    a.flevel = reinterpret_cast<const int*>(base + o_lev);
+    a.fsurf = reinterpret_cast<const int*>(base + o_surf);
+    a.fapplylog = reinterpret_cast<const int*>(base + o_log);
+    a.finfl = reinterpret_cast<const float*>(base + o_inf);
+    a.finfratio = reinterpret_cast<const double*>(base + o_rat);
+    a.E = B;
+    a.infl = infl != nullptr ? reinterpret_cast<float*>(base + o_if) : nullptr;
+    const bool condition = (flags & ENKFG_UPD_HYBRID_CONDITION) && c->cfg.mode == ENKFG_MODE_HYBRID;
+    double tupd = 0.0;
+    for (int f = 0; f < nfield; ++f) {
+        const enkfg_field& fl = fields[f];
+        const int lev = fl.level, surf = fl.issurfacevar, lg = fl.applylog;
+        const float inf = fl.inflation;
+        const double rat = fl.inf_ratio;
+        HIPCHECK(c, hipMemcpyAsync(base + o_lev, &lev, sizeof lev, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(base + o_surf, &surf, sizeof surf, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(base + o_log, &lg, sizeof lg, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(base + o_inf, &inf, sizeof inf, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(base + o_rat, &rat, sizeof rat, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(A, fb[f], ncell * m * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, launch_members_to_cells(c->stream, A, B, m, (long long)ncell));
+        if (condition)
+            HIPCHECK(c, launch_hybrid_condition(c->stream, a, c->cfg.gamma));
+        HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+        HIPCHECK(c, launch_update_mfma(c->stream, a, c->cfg.precision == ENKFG_PREC_MIXED));
+        HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
+        double r48[160];
+        if (std::isfinite(fl.deflation)) {
+            // addnoise for this field, drawn in field order as the batch call draws them
+            const int npair = (m + 1) / 2;
+            for (int e = 0; e < npair; ++e)
+                rand48_normalpair(c->rand48, &r48[2 * e]);
+            double sum = 0.0;
+            for (int e = 0; e < m; ++e)
+                sum += r48[e];
+            sum /= (double)m;
+            for (int e = 0; e < m; ++e)
+                r48[e] -= sum;
+            const float mult = (float)(sqrt(1.0 - fl.deflation * fl.deflation) * fl.sigma);
+            HIPCHECK(c, hipMemcpyAsync(d_noise, r48, (size_t)m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIPCHECK(c, launch_addnoise(c->stream, B, (long long)ncell, m, (float)fl.deflation, mult, d_noise));
+        }
+        HIPCHECK(c, launch_cells_to_members(c->stream, B, A, m, (long long)ncell));
+        HIPCHECK(c, hipMemcpyAsync(fb[f], A, ncell * m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        if (infl != nullptr)
+            HIPCHECK(c, hipMemcpyAsync(infl + (size_t)f * ncell, base + o_if, ncell * sizeof(float),
+                                       hipMemcpyDeviceToHost, c->stream));
+        // the stack values above must outlive their copies, and the next
+        // field reuses the staging areas
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        tupd += elapsed(c->ev[5], c->ev[6]);
+    }
+    c->timings[3] = tupd;
     return ENKFG_OK;
 }
 

@@ -33,7 +33,7 @@ EXPORTED = [
     "enkfg_get_diag", "enkfg_get_stats", "enkfg_localise", "enkfg_get_local_counts", "enkfg_get_localobs",
     "enkfg_set_transforms", "enkfg_update_fields", "enkfg_update_bg", "enkfg_ens_attach", "enkfg_ens_update",
     "enkfg_get_timings", "enkfg_get_tuning", "enkfg_adjoint", "enkfg_set_rand48", "enkfg_get_rand48",
-    "enkfg_rand48_normals", "enkfg_update_HE",
+    "enkfg_rand48_normals", "enkfg_update_HE", "enkfg_update_fields_noalloc",
 ]
 
 
@@ -112,6 +112,8 @@ def lib():
         L.enkfg_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
         L.enkfg_adjoint.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
         L.enkfg_update_HE.argtypes = [vp, C.POINTER(Grid), C.c_int, vp, vp, vp, vp, vp, vp]
+        L.enkfg_update_fields_noalloc.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Field), C.c_int, C.c_int,
+                                                  C.POINTER(vp), vp, C.c_int]
         L.enkfg_set_rand48.argtypes = [vp, C.c_ulonglong]
         L.enkfg_get_rand48.argtypes = [vp, C.POINTER(C.c_ulonglong)]
         L.enkfg_rand48_normals.argtypes = [C.POINTER(C.c_ulonglong), C.c_int, vp]
@@ -318,8 +320,9 @@ class Engine:
         return g, nlevels
 
     def update_fields(self, nlevels, stride, periodic_i, surfk, fields, buffers, j1, j2, outputinc=False,
-                      hybrid_condition=False, want_infl=False):
-        """buffers: float32 arrays [nmem][j2-j1+1][mni], updated in place."""
+                      hybrid_condition=False, want_infl=False, noalloc=False):
+        """buffers: float32 arrays [nmem][j2-j1+1][mni], updated in place;
+        noalloc: through enkfg_update_fields_noalloc (no device allocation)."""
         g, keep = self._grid(nlevels, stride, periodic_i, surfk)
         fa = make_fields(fields)
         for b in buffers:
@@ -327,7 +330,8 @@ class Engine:
         bp = (C.c_void_p * len(buffers))(*[b.ctypes.data for b in buffers])
         infl = np.zeros((len(fields), j2 - j1 + 1, g.mni), np.float32) if want_infl else None
         flags = (UPD_OUTPUTINC if outputinc else 0) | (UPD_HYBRID_CONDITION if hybrid_condition else 0)
-        self._check(lib().enkfg_update_fields(self.h, C.byref(g), len(fields), fa, j1, j2, bp, _p(infl), flags))
+        fn = lib().enkfg_update_fields_noalloc if noalloc else lib().enkfg_update_fields
+        self._check(fn(self.h, C.byref(g), len(fields), fa, j1, j2, bp, _p(infl), flags))
         return infl
 
     def update_HE(self, mnj, mni, stride, periodic_i, fi, fj, status, inflation, inf_ratio, HE):

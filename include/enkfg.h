@@ -223,6 +223,19 @@ int enkfg_get_rand48(const enkfg_ctx* ctx, unsigned long long* state);
  * normal pairs (get_normalpair) into out[2 * npair], advancing *state. */
 int enkfg_rand48_normals(unsigned long long* state, int npair, double* out);
 
+/* enkfg_update_fields for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, results and errors as
+ * enkfg_update_fields, but it calls no hipMalloc / hipFree: its staging
+ * lives in a fixed 64 MiB device array per GPU, held until the process ends,
+ * and the fields are processed one at a time through it.  A batch needs
+ * about 8 (m + 1) jsize mni bytes (jsize = j2 - j1 + 1); larger batches are
+ * refused with ENKFG_ERR_ARG (split the rows).  The transforms (from the
+ * calc or enkfg_set_transforms) must already be on the device.  Every call
+ * on one GPU shares that array: calls that overlap in time on one GPU (from
+ * several threads or contexts) race. */
+int enkfg_update_fields_noalloc(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const enkfg_field* fields, int j1, int j2, float** fieldbuffer, float* infl, int flags);
+
 /* The analysed ensemble observations of a structured grid: update_HE_structured
  * (calc/ensobs.c:1228-1553; update_Hx_structured, 1796-1983, in EnOI mode),
  * the step of das_updateHE (ensobs.c:2116-2129) between sorting the obs and
