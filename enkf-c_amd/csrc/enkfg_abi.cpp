@@ -96,7 +96,7 @@ struct enkfg_ctx {
     hipStream_t stream2 = nullptr;   // factor kernels, overlapping the next batch's Gram
     hipEvent_t ev[10] = {};
     std::vector<hipEvent_t> bev;     // 4 per calc batch
-    double timings[6] = {0, 0, 0, 0, 0, 0};
+    double timings[7] = {0, 0, 0, 0, 0, 0, 0};
 
     // obstypes and obs
     std::vector<enkfg_obstype> types;
@@ -157,6 +157,8 @@ struct enkfg_ctx {
     DevBuf d_he_order, d_he_gcell, d_he_goff, d_he_infl, d_he_ratio;
     // this device's copy of the no-allocation scratch array (device address)
     unsigned char* scratch = nullptr;
+    // ensemble observations: H operator tables, obs records, spread
+    DevBuf d_hg_tab, d_hg_obs, d_hg_std;
     // adjoint
     DevBuf d_adjg, d_adjflag, d_adjlist, d_adjz, d_adjw, d_gy, d_gR;
     float* attached_E = nullptr;
@@ -751,6 +753,38 @@ int enkfg_set_obs_status(enkfg_ctx* c, const short* status)
     return ENKFG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// S observation-major [nobs][m] in d_S and s [nobs] in d_s, in observation
+// order: standardise (unless done), then into device row order with rows
+// padded to ms floats, plus the zero row.  The result is copied back rather
+// than swapped, so that d_S keeps its address and a captured calc graph that
+// reads it stays valid across calls.
+int finish_S(enkfg_ctx* c, bool standardised)
+{
+    const int m = c->cfg.nmem;
+    const int ms = (m + 3) & ~3;
+    const size_t nrow = (size_t)ms * (c->nobs + 1);
+    if (!standardised)
+        HIPCHECK(c, launch_standardise(c->stream, m, c->nobs, c->d_type.as<short>(), c->d_estd.as<float>(),
+                                       c->d_sqrtrf.as<double>(), c->d_S.as<float>(), c->d_s.as<double>()));
+    HIPCHECK(c, launch_gather_rows(c->stream, c->d_S.as<float>(), c->d_perm.as<int>(), c->d_tmp.as<float>(), c->nobs,
+                                   m, ms));
+    HIPCHECK(c, hipMemcpyAsync(c->d_S.p, c->d_tmp.p, nrow * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, launch_gather_rows(c->stream, c->d_s.as<double>(), c->d_perm.as<int>(), c->d_tmp2.as<double>(),
+                                   c->nobs, 1, 1));
+    HIPCHECK(c, hipMemcpyAsync(c->d_s.p, c->d_tmp2.p, (c->nobs + 1) * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return ENKFG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int enkfg_set_S(enkfg_ctx* c, const float* S, const double* s, int standardised)
 {
     if (check_ctx(c))
@@ -775,21 +809,11 @@ int enkfg_set_S(enkfg_ctx* c, const float* S, const double* s, int standardised)
         HIPCHECK(c, hipMemcpyAsync(c->d_s.p, s, c->nobs * sizeof(double), hipMemcpyHostToDevice, c->stream));
         // [m][nobs] -> [nobs][m]
         HIPCHECK(c, launch_members_to_cells(c->stream, c->d_tmp.as<float>(), c->d_S.as<float>(), m, c->nobs));
-        if (!standardised)
-            HIPCHECK(c, launch_standardise(c->stream, m, c->nobs, c->d_type.as<short>(), c->d_estd.as<float>(),
-                                           c->d_sqrtrf.as<double>(), c->d_S.as<float>(), c->d_s.as<double>()));
-        // into device row order, rows padded to ms floats, plus the zero row
-        HIPCHECK(c, launch_gather_rows(c->stream, c->d_S.as<float>(), c->d_perm.as<int>(), c->d_tmp.as<float>(),
-                                       c->nobs, m, ms));
-        // copied back rather than swapped, so that d_S keeps its address and
-        // a captured calc graph that reads it stays valid across set_S calls
-        HIPCHECK(c, hipMemcpyAsync(c->d_S.p, c->d_tmp.p, nrow * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHECK(c, launch_gather_rows(c->stream, c->d_s.as<double>(), c->d_perm.as<int>(), c->d_tmp2.as<double>(),
-                                       c->nobs, 1, 1));
-        HIPCHECK(c, hipMemcpyAsync(c->d_s.p, c->d_tmp2.p, (c->nobs + 1) * sizeof(double), hipMemcpyDeviceToDevice,
-                                   c->stream));
-        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        const int r = finish_S(c, standardised != 0);
+        if (r != ENKFG_OK)
+            return r;
     }
+    (void)nrow;
     c->S_set = true;
     return ENKFG_OK;
 }
@@ -820,6 +844,145 @@ int enkfg_get_S(enkfg_ctx* c, float* S, double* s)
         HIPCHECK(c, hipMemcpyAsync(s, c->d_tmp.p, c->nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return ENKFG_OK;
+}
+
+int enkfg_calc_HE(enkfg_ctx* c, const enkfg_grid* g, int ntypes, const enkfg_hop* hops, const double* fi,
+                  const double* fj, const float* fk, const double* value, float* HA, double* s, double* std_out)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->obs_set)
+        return fail(c, ENKFG_ERR_STATE, "observations not set");
+    if (c->attached_E == nullptr)
+        return fail(c, ENKFG_ERR_STATE, "no ensemble attached (enkfg_ens_attach)");
+    if (c->cfg.mode == ENKFG_MODE_ENOI)
+        return fail(c, ENKFG_ERR_UNSUPPORTED, "HE in EnOI mode needs the background; not supported");
+    const int nt = (int)c->types.size();
+    if (g == nullptr || g->nlevels == nullptr || g->mni != c->att_grid.mni || g->mnj != c->att_grid.mnj)
+        return fail(c, ENKFG_ERR_ARG, "the grid must be the attached ensemble's");
+    if (ntypes != nt || hops == nullptr)
+        return fail(c, ENKFG_ERR_ARG, "one H operator per obs type (%d) is needed", nt);
+    const int nobs = c->nobs;
+    if (nobs > 0 && (fi == nullptr || fj == nullptr || fk == nullptr || value == nullptr))
+        return fail(c, ENKFG_ERR_ARG, "NULL obs array");
+    // H operator tables
+    std::vector<int> hk(nt), nk(nt), f0(nt), fl;
+    for (int t = 0; t < nt; ++t) {
+        const enkfg_hop& h = hops[t];
+        if (h.hfunction != ENKFG_H_SURF && h.hfunction != ENKFG_H_SUBSURF)
+            return fail(c, ENKFG_ERR_UNSUPPORTED, "obs type %d: H function %d is not supported", t, h.hfunction);
+        if (h.nk < 1 || h.field == nullptr)
+            return fail(c, ENKFG_ERR_ARG, "obs type %d: no fields", t);
+        if (g->surfk != 0 && g->surfk != h.nk - 1 && h.nk > 1)
+            return fail(c, ENKFG_ERR_ARG, "obs type %d: surface layer %d is neither 0 nor nk - 1", t, g->surfk);
+        hk[t] = h.hfunction;
+        nk[t] = h.nk;
+        f0[t] = (int)fl.size();
+        for (int k = 0; k < h.nk; ++k) {
+            if (h.field[k] < 0 || h.field[k] >= c->att_nfield)
+                return fail(c, ENKFG_ERR_ARG, "obs type %d layer %d: field %d is not attached", t, k, h.field[k]);
+            fl.push_back(h.field[k]);
+        }
+    }
+    const int m = c->cfg.nmem;
+    const size_t ncell = (size_t)g->mni * g->mnj;
+    if (nobs == 0) {
+        c->S_set = true;
+        return ENKFG_OK;
+    }
+    HIPCHECK(c, c->d_hg_tab.reserve((3 * (size_t)nt + fl.size()) * sizeof(int)));
+    std::vector<int> tab;
+    tab.insert(tab.end(), hk.begin(), hk.end());
+    tab.insert(tab.end(), nk.begin(), nk.end());
+    tab.insert(tab.end(), f0.begin(), f0.end());
+    tab.insert(tab.end(), fl.begin(), fl.end());
+    HIPCHECK(c, hipMemcpyAsync(c->d_hg_tab.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, c->d_nlev.reserve(ncell * sizeof(int)));
+    HIPCHECK(c, hipMemcpyAsync(c->d_nlev.p, g->nlevels, ncell * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, c->d_hg_obs.reserve((size_t)nobs * (3 * sizeof(double) + sizeof(float) + 2 * sizeof(short))));
+    unsigned char* ob = c->d_hg_obs.as<unsigned char>();
+    double* d_fi = reinterpret_cast<double*>(ob);
+    double* d_fj = d_fi + nobs;
+    double* d_val = d_fj + nobs;
+    float* d_fk = reinterpret_cast<float*>(d_val + nobs);
+    short* d_st = reinterpret_cast<short*>(d_fk + nobs);
+    short* d_ty = d_st + nobs;
+    HIPCHECK(c, hipMemcpyAsync(d_fi, fi, (size_t)nobs * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_fj, fj, (size_t)nobs * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_val, value, (size_t)nobs * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_fk, fk, (size_t)nobs * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_st, c->ostatus.data(), (size_t)nobs * sizeof(short), hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_ty, c->otype.data(), (size_t)nobs * sizeof(short), hipMemcpyHostToDevice,
+                               c->stream));
+    const int ms = (m + 3) & ~3;
+    const size_t nrow = (size_t)ms * (nobs + 1);
+    HIPCHECK(c, c->d_S.reserve(std::max((size_t)m * nobs, nrow) * sizeof(float)));
+    HIPCHECK(c, c->d_tmp.reserve(std::max((size_t)m * nobs, nrow) * sizeof(float)));
+    HIPCHECK(c, c->d_s.reserve((nobs + 1) * sizeof(double)));
+    HIPCHECK(c, c->d_tmp2.reserve((nobs + 1) * sizeof(double)));
+    HIPCHECK(c, c->d_hg_std.reserve((size_t)nobs * sizeof(double)));
+    int zero = 0;
+    HIPCHECK(c, hipMemcpyAsync(c->d_err.p, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    HeGetArgs a{};
+    a.m = m;
+    a.md = c->cfg.nmem_dynamic;
+    a.mode = c->cfg.mode;
+    a.gamma = c->cfg.gamma;
+    a.ni = g->mni;
+    a.nj = g->mnj;
+    a.ktop = g->surfk;
+    a.periodic_i = g->periodic_i;
+    a.ncell = (long long)ncell;
+    a.nlevels = c->d_nlev.as<int>();
+    a.E = c->attached_E;
+    a.nobs = nobs;
+    a.type = d_ty;
+    a.status = d_st;
+    a.fi = d_fi;
+    a.fj = d_fj;
+    a.fk = d_fk;
+    a.value = d_val;
+    const int* dt = c->d_hg_tab.as<int>();
+    a.hkind = dt;
+    a.nk = dt + nt;
+    a.field0 = dt + 2 * nt;
+    a.fields = dt + 3 * nt;
+    a.HA = c->d_S.as<float>();          // [nobs][m], observation-major
+    a.s = c->d_s.as<double>();
+    a.std = c->d_hg_std.as<double>();
+    a.err = c->d_err.as<int>();
+    HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));
+    HIPCHECK(c, launch_he(c->stream, a));
+    HIPCHECK(c, hipEventRecord(c->ev[8], c->stream));
+    int err = 0;
+    HIPCHECK(c, hipMemcpyAsync(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->timings[6] = elapsed(c->ev[7], c->ev[8]);
+    if (err != 0) {
+        c->S_set = false;
+        return fail(c, ENKFG_ERR_NONFINITE, err == 1 ? "a forecast observation is not finite or exceeds MAXOBSVAL "
+                                                        "(model2obs.c:111-119)"
+                                                      : "y - Hx_f is not finite or exceeds MAXOBSVAL (ensobs.c:607-615)");
+    }
+    // the caller's copies: HA [nmem][nobs], s and std as das_calcinnandspread leaves them
+    if (HA != nullptr) {
+        HIPCHECK(c, launch_cells_to_members(c->stream, c->d_S.as<float>(), c->d_tmp.as<float>(), m, nobs));
+        HIPCHECK(c, hipMemcpyAsync(HA, c->d_tmp.p, (size_t)m * nobs * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->stream));
+    }
+    if (s != nullptr)
+        HIPCHECK(c, hipMemcpyAsync(s, c->d_s.p, (size_t)nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (std_out != nullptr)
+        HIPCHECK(c, hipMemcpyAsync(std_out, c->d_hg_std.p, (size_t)nobs * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+    // the engine's S and s, as enkfg_set_S(HA, s, standardised = 0) leaves them
+    HIPCHECK(c, launch_he_clear_bad(c->stream, d_st, nobs, m, c->d_S.as<float>(), c->d_s.as<double>()));
+    int r = finish_S(c, false);
+    if (r != ENKFG_OK)
+        return r;
+    c->S_set = true;
     return ENKFG_OK;
 }
 
@@ -1281,7 +1444,7 @@ int enkfg_get_timings(enkfg_ctx* c, double* ms, int n)
 {
     if (check_ctx(c) || ms == nullptr)
         return ENKFG_ERR_ARG;
-    for (int k = 0; k < n && k < 6; ++k)
+    for (int k = 0; k < n && k < 7; ++k)
         ms[k] = c->timings[k];
     return ENKFG_OK;
 }
@@ -1795,9 +1958,11 @@ int enkfg_ens_attach(enkfg_ctx* c, const enkfg_grid* g, int nfield, const enkfg_
 {
     if (check_ctx(c))
         return ENKFG_ERR_ARG;
-    int r = check_grid(c, g);
-    if (r != ENKFG_OK)
-        return r;
+    // the ensemble may be attached before any transforms exist (enkfg_calc_HE
+    // reads it first); the update checks the grid against the transforms
+    if (g == nullptr || g->mni <= 0 || g->mnj <= 0 || g->nlevels == nullptr || g->stride <= 0)
+        return fail(c, ENKFG_ERR_ARG, "invalid grid");
+    int r = ENKFG_OK;
     if (nfield <= 0 || fields == nullptr || dev_E == nullptr)
         return fail(c, ENKFG_ERR_ARG, "invalid ensemble");
     r = upload_fields(c, g, nfield, fields);
@@ -1819,6 +1984,13 @@ int enkfg_ens_update(enkfg_ctx* c, int flags, float* dev_infl)
         return fail(c, ENKFG_ERR_STATE, "no ensemble attached");
     if (c->cfg.mode == ENKFG_MODE_ENOI)
         return fail(c, ENKFG_ERR_ARG, "das_updatefields needs EnKF or hybrid mode");
+    {
+        enkfg_grid g = c->att_grid;
+        g.nlevels = reinterpret_cast<const int*>(c->d_nlev.p);   // checked for presence only
+        const int r = check_grid(c, &g);
+        if (r != ENKFG_OK)
+            return r;
+    }
     UpdArgs a = upd_args(c, &c->att_grid, c->att_nfield, 0, c->att_grid.mnj, flags);
     a.E = c->attached_E;
     a.infl = dev_infl;

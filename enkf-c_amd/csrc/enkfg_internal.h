@@ -175,6 +175,34 @@ struct HeArgs {
 };
 hipError_t launch_update_he(hipStream_t s, const UpdArgs& u, const HeArgs& h);
 
+// the ensemble observations from the device-resident ensemble (he_kernels.hip)
+struct HeGetArgs {
+    int m, md, mode;
+    double gamma;
+    int ni, nj, ktop, periodic_i;  // model grid, surface layer id
+    long long ncell;
+    const int* nlevels;            // [nj][ni]
+    const float* E;                // [field][cell][m]
+    int nobs;
+    const short* type;             // [nobs], observation order
+    const short* status;
+    const double* fi;
+    const double* fj;
+    const float* fk;
+    const double* value;
+    const int* hkind;              // [ntypes] ENKFG_H_SURF / ENKFG_H_SUBSURF
+    const int* nk;                 // [ntypes] layers of the type's variable
+    const int* field0;             // [ntypes] first entry of the type in fields
+    const int* fields;             // field index of each layer
+    float* HA;                     // [nobs][m] out: HE minus the member mean
+    double* s;                     // [nobs] out: innovations
+    double* std;                   // [nobs] out: spread
+    int* err;                      // 1: bad forecast, 2: bad innovation
+};
+hipError_t launch_he(hipStream_t s, const HeGetArgs& a);
+// rows of HA and entries of s of the obs that are not OK set to 0
+hipError_t launch_he_clear_bad(hipStream_t s, const short* status, int nobs, int m, float* HA, double* sv);
+
 // adjoint of the analysis mean (adjoint.hip)
 // g_nodes[node][m] += sum over cells of beta(cell, node) sum_f dJ/dmean[f][cell] A^f[f][cell][:]
 hipError_t launch_adj_cells(hipStream_t s, const UpdArgs& a, const float* dJ_dmean, double* g_nodes);

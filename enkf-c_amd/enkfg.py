@@ -33,7 +33,7 @@ EXPORTED = [
     "enkfg_get_diag", "enkfg_get_stats", "enkfg_localise", "enkfg_get_local_counts", "enkfg_get_localobs",
     "enkfg_set_transforms", "enkfg_update_fields", "enkfg_update_bg", "enkfg_ens_attach", "enkfg_ens_update",
     "enkfg_get_timings", "enkfg_get_tuning", "enkfg_adjoint", "enkfg_set_rand48", "enkfg_get_rand48",
-    "enkfg_rand48_normals", "enkfg_update_HE", "enkfg_update_fields_noalloc",
+    "enkfg_rand48_normals", "enkfg_update_HE", "enkfg_update_fields_noalloc", "enkfg_calc_HE",
 ]
 
 
@@ -67,6 +67,13 @@ class Stats(C.Structure):
 class Grid(C.Structure):
     _fields_ = [("mni", C.c_int), ("mnj", C.c_int), ("stride", C.c_int), ("periodic_i", C.c_int), ("surfk", C.c_int),
                 ("nlevels", C.POINTER(C.c_int))]
+
+
+class Hop(C.Structure):
+    _fields_ = [("hfunction", C.c_int), ("nk", C.c_int), ("field", C.POINTER(C.c_int))]
+
+
+H_SURF, H_SUBSURF = 0, 1
 
 
 class Field(C.Structure):
@@ -112,6 +119,7 @@ def lib():
         L.enkfg_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
         L.enkfg_adjoint.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
         L.enkfg_update_HE.argtypes = [vp, C.POINTER(Grid), C.c_int, vp, vp, vp, vp, vp, vp]
+        L.enkfg_calc_HE.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Hop), vp, vp, vp, vp, vp, vp, vp]
         L.enkfg_update_fields_noalloc.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Field), C.c_int, C.c_int,
                                                   C.POINTER(vp), vp, C.c_int]
         L.enkfg_set_rand48.argtypes = [vp, C.c_ulonglong]
@@ -215,6 +223,31 @@ class Engine:
         s = np.ascontiguousarray(s, np.float64)
         self._check(lib().enkfg_set_S(self.h, _p(S), _p(s), int(standardised)))
 
+    def calc_HE(self, nlevels, surfk, periodic_i, hops, fi, fj, fk, value, want=True):
+        """das_getHE + das_calcinnandspread from the attached ensemble; the
+        engine's S and s are set from the result.  hops: per obs type
+        (hfunction, [field index of layer k]).  Returns (HA, s, std) host
+        copies when want."""
+        nlevels = np.ascontiguousarray(nlevels, np.int32)
+        g = Grid(nlevels.shape[1], nlevels.shape[0], 1, periodic_i, surfk,
+                 nlevels.ctypes.data_as(C.POINTER(C.c_int)))
+        fl = [np.ascontiguousarray(f, np.int32) for _, f in hops]
+        arr = (Hop * max(len(hops), 1))()
+        for k, (h, f) in enumerate(hops):
+            arr[k].hfunction = h
+            arr[k].nk = len(f)
+            arr[k].field = fl[k].ctypes.data_as(C.POINTER(C.c_int))
+        fi = np.ascontiguousarray(fi, np.float64)
+        fj = np.ascontiguousarray(fj, np.float64)
+        fk = np.ascontiguousarray(fk, np.float32)
+        value = np.ascontiguousarray(value, np.float64)
+        HA = np.zeros((self.nmem, self.nobs), np.float32) if want else None
+        s = np.zeros(self.nobs) if want else None
+        sd = np.zeros(self.nobs) if want else None
+        self._check(lib().enkfg_calc_HE(self.h, C.byref(g), len(hops), arr, _p(fi), _p(fj), _p(fk), _p(value),
+                                        _p(HA), _p(s), _p(sd)))
+        return HA, s, sd
+
     def get_S(self):
         S = np.zeros((self.nmem, self.nobs), np.float32)
         s = np.zeros(self.nobs, np.float64)
@@ -279,6 +312,12 @@ class Engine:
         t = (C.c_double * 4)()
         self._check(lib().enkfg_get_timings(self.h, t, 4))
         return list(t)
+
+    def he_ms(self):
+        """kernel time of the last calc_HE"""
+        t = (C.c_double * 7)()
+        self._check(lib().enkfg_get_timings(self.h, t, 7))
+        return t[6]
 
     def adjoint(self, dev_dJ_dmean, nobs):
         """Gradients of J with respect to the observation values and the

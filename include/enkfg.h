@@ -161,6 +161,19 @@ int enkfg_set_obstypes(enkfg_ctx* ctx, int ntypes, const enkfg_obstype* types);
  * (lon, lat).  lon/lat are the float obs coordinates of observations.h:67-68. */
 int enkfg_set_obs(enkfg_ctx* ctx, int nobs, const short* type, const short* status, const float* lon, const float* lat, const float* estd);
 
+/* H operators of the obs types (calc/allhs.c:30-44) that enkfg_calc_HE
+ * evaluates on the device */
+#define ENKFG_H_SURF 0       /* H_surf_standard (model2obs.c:127-181): 2D
+                                interpolation of the variable's surface layer */
+#define ENKFG_H_SUBSURF 1    /* H_subsurf_standard (model2obs.c:256-355): 3D
+                                interpolation between the layers */
+typedef struct {
+    int hfunction;           /* ENKFG_H_* */
+    int nk;                  /* layers of the obs type's model variable (1: 2D) */
+    const int* field;        /* [nk] index, in the attached ensemble, of the
+                                field holding layer k of the variable */
+} enkfg_hop;
+
 /* Search-time obs status (observations.c:1906): obs marked bad after the trees
  * were built (e.g. by obs_markbadbatches) are skipped by the local search. */
 int enkfg_set_obs_status(enkfg_ctx* ctx, const short* status);
@@ -169,6 +182,23 @@ int enkfg_set_obs_status(enkfg_ctx* ctx, const short* status);
  * standardised == 0 they are HA^f and the innovations, and the engine applies
  * das_standardise (calc/ensobs.c:849-898) on the device. */
 int enkfg_set_S(enkfg_ctx* ctx, const float* S, const double* s, int standardised);
+
+/* das_getHE (calc/ensobs.c:46-508) from the device-resident ensemble
+ * (enkfg_ens_attach), with the hybrid scaling (ensobs.c:434-486) and
+ * das_calcinnandspread (ensobs.c:541-620): for every obs of the last
+ * enkfg_set_obs with status OK, HE = H(E) for the H operator of its type
+ * (structured grids, footprint 0; fi, fj, fk are fij[0], fij[1], fk of the
+ * observation record), then the innovations y - mean(HE) and the anomalies.
+ * The result becomes the engine's S and s exactly as enkfg_set_S(HA, s, 0)
+ * would make them, so S never leaves the device on its way to the calc.
+ * grid is the attached ensemble's (nlevels for the land mask, surfk = the
+ * surface layer id, 0 or nk - 1).  Optional host copies: HA [nmem][nobs]
+ * (das->S in S_MODE_HA_f), s [nobs] (das->s_f) and std [nobs] (das->std_f),
+ * with the reference's values for obs that are not OK (NaN).  A forecast or
+ * innovation that is not finite or exceeds MAXOBSVAL returns
+ * ENKFG_ERR_NONFINITE where the reference calls enkf_quit.  EnOI (the
+ * background forecast) is not supported. */
+int enkfg_calc_HE(enkfg_ctx* ctx, const enkfg_grid* grid, int ntypes, const enkfg_hop* hops, const double* fi, const double* fj, const float* fk, const double* value, float* HA, double* s, double* std);
 
 /* The standardised S and s_f (the reference standardises in place; callers
  * such as das_updateHE read them back). */
@@ -284,7 +314,8 @@ int enkfg_adjoint(enkfg_ctx* ctx, const float* dev_dJ_dmean, double* dJ_dy, doub
  * the number of Gram (and factor) launches of the last calc and [5] the
  * device time of the whole calc (the factor kernels of one batch run
  * concurrently with the Gram kernel of the next, so [1] + [2] may exceed
- * the time they take together). */
+ * the time they take together); [6] is the kernel time of the last
+ * enkfg_calc_HE. */
 int enkfg_get_timings(enkfg_ctx* ctx, double* ms, int n);
 
 /* Calc batching and the autotuner's choices (no counterpart in the

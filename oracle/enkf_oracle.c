@@ -44,11 +44,11 @@
 #include <float.h>
 #include "enkf_oracle.h"
 
-#define OR_REARTH 6371.0                 /* common/global.h:119 */
-#define OR_DEG2RAD (M_PI / 180.0)        /* common/global.h:120 */
+#define OR_REARTH 6371.0                 /* common/global.h:121 */
+#define OR_DEG2RAD (M_PI / 180.0)        /* common/global.h:122 */
 #define OR_DFS_MIN 1.0e-6                /* calc/transforms.c:32 */
 #define OR_EPSF 1.0e-6f                  /* update/update.c:41 */
-#define OR_MAXOBSVAL 999999.0            /* common/global.h:114 */
+#define OR_MAXOBSVAL 999999.0            /* common/global.h:119 */
 
 /* ======================================================================
  * geometry and taper
@@ -1496,4 +1496,234 @@ int or_update_HE(const or_updatespec* sp, int nobs, const double* fi, const doub
     free(HEo_a);
     free(rowof);
     return r;
+}
+
+/* ======================================================================
+ * ensemble observations: das_getHE (calc/ensobs.c:46-508) with the standard
+ * H functions (calc/model2obs.c:127-365), das_calcinnandspread (541-690)
+ * ====================================================================== */
+
+#define OR_EPS_FLOAT 1.0e-7   /* common/utils.c:51 */
+
+/* interpolate2d_structured, common/utils.c:1140-1213 */
+float or_interp2d(const double* fij, int ni, int nj, const float* v, const int* mask, int periodic_i)
+{
+    int i1, i2, j1, j2;
+    double wi1, wi2, wj1, wj2;
+    double sum, w, ww;
+    double fi = fij[0];
+    double fj = fij[1];
+
+    if (fi >= (double) ni)
+        fi = (double) ni - OR_EPS_FLOAT;
+    if (fj >= (double) nj)
+        fj = (double) nj - OR_EPS_FLOAT;
+    i1 = (int) floor(fi);
+    wi1 = ceil(fi) - fi;
+    i2 = (int) ceil(fi);
+    wi2 = fi - floor(fi);
+    j1 = (int) floor(fj);
+    wj1 = ceil(fj) - fj;
+    j2 = (int) ceil(fj);
+    wj2 = fj - floor(fj);
+    if (i1 == i2)
+        wi1 = 1.0;
+    if (j1 == j2)
+        wj1 = 1.0;
+    if (i1 == -1)
+        i1 = (periodic_i) ? ni - 1 : i2;
+    if (i2 == ni)
+        i2 = (periodic_i) ? 0 : i1;
+    if (j1 == -1)
+        j1 = j2;
+    if (j2 == nj)
+        j2 = j1;
+    sum = 0.0;
+    w = 0.0;
+    if (mask[(size_t) j1 * ni + i1]) {
+        ww = wj1 * wi1;
+        sum += v[(size_t) j1 * ni + i1] * ww;
+        w += ww;
+    }
+    if (mask[(size_t) j1 * ni + i2]) {
+        ww = wj1 * wi2;
+        sum += v[(size_t) j1 * ni + i2] * ww;
+        w += ww;
+    }
+    if (mask[(size_t) j2 * ni + i1]) {
+        ww = wj2 * wi1;
+        sum += v[(size_t) j2 * ni + i1] * ww;
+        w += ww;
+    }
+    if (mask[(size_t) j2 * ni + i2]) {
+        ww = wj2 * wi2;
+        sum += v[(size_t) j2 * ni + i2] * ww;
+        w += ww;
+    }
+    sum /= w;
+    return (float) sum;
+}
+
+/* interpolate3d_structured, common/utils.c:1252-1367; v[k] is layer k */
+float or_interp3d(const double* fij, double fk, int ni, int nj, int nk, int ktop, const float* const* v, const int* nlevels, int periodic_i)
+{
+    int i1, i2, j1, j2, k1, k2, k1top, k2top;
+    double wi1, wi2, wj1, wj2, wk1, wk2;
+    double sum, w, ww;
+    double fi = fij[0];
+    double fj = fij[1];
+
+    if (fi > (double) ni)
+        fi = (double) ni - OR_EPS_FLOAT;
+    if (fj >= (double) nj)
+        fj = (double) nj - OR_EPS_FLOAT;
+    i1 = (int) floor(fi);
+    wi1 = ceil(fi) - fi;
+    i2 = (int) ceil(fi);
+    wi2 = fi - floor(fi);
+    j1 = (int) floor(fj);
+    wj1 = ceil(fj) - fj;
+    j2 = (int) ceil(fj);
+    wj2 = fj - floor(fj);
+    if (fk < 0.0)
+        fk = 0.0;
+    if (fk > (double) (nk - 1))
+        fk = (double) (nk - 1);
+    k1 = (int) floor(fk);
+    k1top = abs(ktop - k1);
+    wk1 = ceil(fk) - fk;
+    k2 = (int) ceil(fk);
+    k2top = abs(ktop - k2);
+    wk2 = fk - floor(fk);
+    if (wk1 == 0.0 && wk2 == 0.0)
+        wk1 = 1.0;
+    if (i1 == i2)
+        wi1 = 1.0;
+    if (j1 == j2)
+        wj1 = 1.0;
+    if (k1 == k2)
+        wk1 = 1.0;
+    if (i1 == -1)
+        i1 = (periodic_i) ? ni - 1 : i2;
+    if (i2 == ni)
+        i2 = (periodic_i) ? 0 : i1;
+    if (j1 == -1)
+        j1 = j2;
+    if (j2 == nj)
+        j2 = j1;
+    sum = 0.0;
+    w = 0.0;
+#define OR_C3(J, I, WJ, WI, K, KT, WK) \
+    if (nlevels[(size_t) (J) * ni + (I)] > (KT)) { \
+        ww = (WJ) * (WI) * (WK); \
+        sum += v[K][(size_t) (J) * ni + (I)] * ww; \
+        w += ww; \
+    }
+    OR_C3(j1, i1, wj1, wi1, k1, k1top, wk1)
+    OR_C3(j1, i2, wj1, wi2, k1, k1top, wk1)
+    OR_C3(j2, i1, wj2, wi1, k1, k1top, wk1)
+    OR_C3(j2, i2, wj2, wi2, k1, k1top, wk1)
+    OR_C3(j1, i1, wj1, wi1, k2, k2top, wk2)
+    OR_C3(j1, i2, wj1, wi2, k2, k2top, wk2)
+    OR_C3(j2, i1, wj2, wi1, k2, k2top, wk2)
+    OR_C3(j2, i2, wj2, wi2, k2, k2top, wk2)
+#undef OR_C3
+    sum = sum / w;
+    return (float) sum;
+}
+
+/* das_getHE (ensobs.c:46-508) for structured grids with the standard H
+ * functions (H_surf_standard, H_subsurf_standard; model2obs.c:127-365,
+ * footprint 0), the hybrid scaling (ensobs.c:434-486) and
+ * das_calcinnandspread (ensobs.c:541-620).  E[f] is field f of the ensemble,
+ * [nmem][nj][ni]; obs type t evaluates hkind[t] (0 surface, 1 subsurface)
+ * over the fields fields[t][0 .. nk[t]-1] (layer k; the surface field is
+ * fields[t][0]).  Only obs with status OK are evaluated (obs_find_bytype);
+ * the others keep NaN.  Outputs: HA [nmem][nobs] (das->S after the call),
+ * s [nobs] (s_f) and std [nobs] (std_f).  Returns OR_ERR_NONFINITE where the
+ * reference calls enkf_quit (a forecast or innovation that is not finite
+ * or exceeds MAXOBSVAL). */
+int or_getHE(int mode, int nmem, int nmem_dynamic, double gamma, int ni, int nj, int ktop, int periodic_i, const int* nlevels, const float* const* E, int nobs, const short* type, const short* status, const double* fi, const double* fj, const float* fk, const double* value, const int* hkind, const int* nk, const int* const* fields, float* HA, double* s, double* std)
+{
+    size_t nij = (size_t) ni * nj;
+    int e, o, k;
+    const float* v[256];
+
+    for (e = 0; e < nmem; ++e)
+        for (o = 0; o < nobs; ++o)
+            HA[(size_t) e * nobs + o] = NAN;
+    for (o = 0; o < nobs; ++o) {
+        int t = type[o];
+        double fij[2] = { fi[o], fj[o] };
+
+        if (status[o] != OR_STATUS_OK)
+            continue;
+        for (e = 0; e < nmem; ++e) {
+            float x;
+
+            if (hkind[t] == 0)
+                x = or_interp2d(fij, ni, nj, E[fields[t][0]] + (size_t) e * nij, nlevels, periodic_i);
+            else {
+                for (k = 0; k < nk[t]; ++k)
+                    v[k] = E[fields[t][k]] + (size_t) e * nij;
+                x = or_interp3d(fij, (double) fk[o], ni, nj, nk[t], ktop, v, nlevels, periodic_i);
+            }
+            if (!isfinite(x) || fabs(x) > OR_MAXOBSVAL)
+                return OR_ERR_NONFINITE;
+            HA[(size_t) e * nobs + o] = x;
+        }
+    }
+    if (mode == OR_MODE_HYBRID) {
+        int md = nmem_dynamic, ms = nmem - nmem_dynamic;
+        double k_d = (md > 1) ? sqrt((double) (nmem - 1) / (double) (md - 1)) : 0.0;
+        double k_s = sqrt(gamma * (double) (nmem - 1) / (double) (ms - 1));
+
+        for (o = 0; o < nobs; ++o) {
+            double mean_d = 0.0, mean_s = 0.0;
+
+            for (e = 0; e < md; ++e)
+                mean_d += HA[(size_t) e * nobs + o];
+            mean_d /= (double) md;
+            for (e = 0; e < md; ++e) {
+                float* x = &HA[(size_t) e * nobs + o];
+
+                *x = (*x - mean_d) * k_d + mean_d;
+            }
+            for (e = md; e < nmem; ++e)
+                mean_s += HA[(size_t) e * nobs + o];
+            mean_s /= (double) ms;
+            for (e = md; e < nmem; ++e) {
+                float* x = &HA[(size_t) e * nobs + o];
+
+                *x = (*x - mean_s) * k_s + mean_d;
+            }
+        }
+    }
+    for (o = 0; o < nobs; ++o) {
+        s[o] = 0.0;
+        std[o] = 0.0;
+    }
+    for (e = 0; e < nmem; ++e)
+        for (o = 0; o < nobs; ++o)
+            s[o] += (double) HA[(size_t) e * nobs + o];
+    for (o = 0; o < nobs; ++o)
+        s[o] /= (double) nmem;
+    for (e = 0; e < nmem; ++e)
+        for (o = 0; o < nobs; ++o)
+            HA[(size_t) e * nobs + o] -= (float) s[o];
+    for (e = 0; e < nmem; ++e)
+        for (o = 0; o < nobs; ++o) {
+            float x = HA[(size_t) e * nobs + o];
+
+            std[o] += (double) (x * x);
+        }
+    for (o = 0; o < nobs; ++o) {
+        if (status[o] != OR_STATUS_OK)
+            continue;
+        std[o] = sqrt(std[o] / (double) (nmem - 1));
+        s[o] = value[o] - s[o];
+        if (!isfinite(s[o]) || fabs(s[o]) > OR_MAXOBSVAL)
+            return OR_ERR_NONFINITE;
+    }
+    return OR_OK;
 }

@@ -39,6 +39,7 @@ extern "C" {
 #define OR_ERR_LINALG -2
 #define OR_ERR_NOMEM -3
 #define OR_ERR_CAP -4
+#define OR_ERR_NONFINITE -5
 
 /* ---- geometry and taper (common/utils.c:1425-1458, obstypes.c:800-822) */
 void or_ll2xyz(const double ll[2], double xyz[3]);
@@ -175,6 +176,15 @@ int or_update_bg(const or_updatespec* spec, int nfield, const or_field* fields, 
  * EnOI mode (sp->T unused) it is update_Hx_structured (ensobs.c:1796-1983):
  * every member moves by the interpolated w applied to the anomalies. */
 int or_update_HE(const or_updatespec* sp, int nobs, const double* fi, const double* fj, const short* status, const float* inflation0, const double* inf_ratio, float* HE);
+/* ---- ensemble observations (calc/ensobs.c:46-690, common/utils.c:1140-1367) */
+/* interpolate2d_structured: v [nj][ni], mask [nj][ni] (non-zero = wet) */
+float or_interp2d(const double* fij, int ni, int nj, const float* v, const int* mask, int periodic_i);
+/* interpolate3d_structured: v[k] the layer k [nj][ni]; ktop the surface layer
+ * index (0 or nk - 1); wet where nlevels > |ktop - k| */
+float or_interp3d(const double* fij, double fk, int ni, int nj, int nk, int ktop, const float* const* v, const int* nlevels, int periodic_i);
+/* das_getHE with the standard H functions, the hybrid scaling and
+ * das_calcinnandspread (see enkf_oracle.c) */
+int or_getHE(int mode, int nmem, int nmem_dynamic, double gamma, int ni, int nj, int ktop, int periodic_i, const int* nlevels, const float* const* E, int nobs, const short* type, const short* status, const double* fi, const double* fj, const float* fk, const double* value, const int* hkind, const int* nk, const int* const* fields, float* HA, double* s, double* std);
 /* RANDOMISE: drand48 on an explicit 48-bit state (POSIX), get_normalpair
  * (common/utils.c:1477-1484) and addnoise (update/update.c:46-87) applied to
  * v[e][0 .. n-1], e < nmem */

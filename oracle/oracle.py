@@ -107,6 +107,13 @@ def lib():
                                    C.POINTER(_fp)]
         L.or_sethybridensemble.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _fp, _dp]
         L.or_update_HE.argtypes = [C.POINTER(UpdateSpec), C.c_int, _dp, _dp, _sp, _fp, _dp, _fp]
+        L.or_interp2d.restype = C.c_float
+        L.or_interp2d.argtypes = [_dp, C.c_int, C.c_int, _fp, _ip, C.c_int]
+        L.or_interp3d.restype = C.c_float
+        L.or_interp3d.argtypes = [_dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_fp), _ip, C.c_int]
+        L.or_getHE.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, _ip,
+                               C.POINTER(_fp), C.c_int, _sp, _sp, _dp, _dp, _fp, _dp, _ip, _ip, C.POINTER(_ip), _fp,
+                               _dp, _dp]
         L.or_drand48.restype = C.c_double
         L.or_drand48.argtypes = [C.POINTER(C.c_ulonglong)]
         L.or_get_normalpair.argtypes = [C.POINTER(C.c_ulonglong), _dp]
@@ -537,3 +544,52 @@ def update_HE(mode, nmem, nmem_dynamic, stride, periodic_i, mnj, mni, T, w, fi, 
     if r != 0:
         raise RuntimeError("or_update_HE: %d" % r)
     return out
+
+def getHE(mode, nmem, nmem_dynamic, gamma, nlevels, ktop, periodic_i, fields, otypes_h, type_, status, fi, fj, fk,
+          value):
+    """das_getHE with the standard H functions + hybrid scaling +
+    das_calcinnandspread.  fields: list of float32 [nmem][nj][ni] (one per
+    model field); otypes_h: per obs type (hkind, [field index of layer k]).
+    Returns (HA [nmem][nobs] float32, s [nobs], std [nobs])."""
+    nlevels = np.ascontiguousarray(nlevels, np.int32)
+    nj, ni = nlevels.shape
+    fields = [np.ascontiguousarray(f, np.float32) for f in fields]
+    E = (_fp * len(fields))(*[_ptr(f, _fp) for f in fields])
+    nt = len(otypes_h)
+    hk = np.array([h for h, _ in otypes_h], np.int32)
+    nk = np.array([len(fl) for _, fl in otypes_h], np.int32)
+    flarr = [np.ascontiguousarray(fl, np.int32) for _, fl in otypes_h]
+    FL = (_ip * nt)(*[_ptr(a, _ip) for a in flarr])
+    nobs = len(type_)
+    type_ = np.ascontiguousarray(type_, np.int16)
+    status = np.ascontiguousarray(status, np.int16)
+    fi = np.ascontiguousarray(fi, np.float64)
+    fj = np.ascontiguousarray(fj, np.float64)
+    fk = np.ascontiguousarray(fk, np.float32)
+    value = np.ascontiguousarray(value, np.float64)
+    HA = np.zeros((nmem, nobs), np.float32)
+    s = np.zeros(nobs)
+    sd = np.zeros(nobs)
+    r = lib().or_getHE(mode, nmem, nmem_dynamic, float(gamma), ni, nj, ktop, periodic_i, _ptr(nlevels, _ip), E, nobs,
+                       _ptr(type_, _sp), _ptr(status, _sp), _ptr(fi, _dp), _ptr(fj, _dp), _ptr(fk, _fp),
+                       _ptr(value, _dp), _ptr(hk, _ip), _ptr(nk, _ip), FL, _ptr(HA, _fp), _ptr(s, _dp), _ptr(sd, _dp))
+    if r != 0:
+        raise RuntimeError("or_getHE: %d" % r)
+    return HA, s, sd
+
+
+def interp2d(fij, v, mask, periodic_i=0):
+    nj, ni = v.shape
+    fij = np.ascontiguousarray(fij, np.float64)
+    v = np.ascontiguousarray(v, np.float32)
+    mask = np.ascontiguousarray(mask, np.int32)
+    return lib().or_interp2d(_ptr(fij, _dp), ni, nj, _ptr(v, _fp), _ptr(mask, _ip), periodic_i)
+
+
+def interp3d(fij, fk, v, nlevels, ktop=0, periodic_i=0):
+    nk, nj, ni = v.shape
+    fij = np.ascontiguousarray(fij, np.float64)
+    layers = [np.ascontiguousarray(v[k], np.float32) for k in range(nk)]
+    L = (_fp * nk)(*[_ptr(a, _fp) for a in layers])
+    nlevels = np.ascontiguousarray(nlevels, np.int32)
+    return lib().or_interp3d(_ptr(fij, _dp), float(fk), ni, nj, nk, ktop, L, _ptr(nlevels, _ip), periodic_i)

@@ -564,3 +564,97 @@ def test_update_Hx_enoi_at_stride_nodes_brute_force():
     d = ((X - X.mean(0)) * w[jn, inn].T.astype(np.float64)).sum(0)
     assert np.allclose(HEa, X + d[None, :], rtol=0, atol=1e-5)
     assert np.abs(d).max() > 1e-3
+
+
+# ---------------------------------------------------------------- ensemble observations
+def test_interp2d_linear_masked_edges_periodic():
+    """interpolate2d_structured (utils.c:1140-1213): a field linear in (i, j)
+    is reproduced where the four corners are wet; a dry corner drops out and
+    the others are renormalised; columns past the last wrap on periodic grids
+    and clamp otherwise; indices at or past the end are pulled back by
+    EPS_FLOAT."""
+    nj, ni = 7, 9
+    jj, ii = np.meshgrid(np.arange(nj), np.arange(ni), indexing="ij")
+    v = (2.0 + 0.5 * ii - 0.25 * jj).astype(np.float32)
+    wet = np.ones((nj, ni), np.int32)
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        fi, fj = rng.uniform(0, ni - 1), rng.uniform(0, nj - 1)
+        assert abs(O.interp2d([fi, fj], v, wet) - (2.0 + 0.5 * fi - 0.25 * fj)) < 1e-5
+    # a dry corner: weights of the wet three, renormalised
+    mask = wet.copy()
+    mask[3, 4] = 0
+    fi, fj = 4.3, 2.6
+    wts = {(2, 4): 0.4 * 0.7, (2, 5): 0.4 * 0.3, (3, 5): 0.6 * 0.3}
+    want = sum(float(v[j, i]) * w for (j, i), w in wts.items()) / sum(wts.values())
+    assert abs(O.interp2d([fi, fj], v, mask) - want) < 1e-6
+    # past the last column: periodic wrap to column 0, else the last column
+    u = np.zeros((nj, ni), np.float32)
+    u[:, 0] = 1.0
+    assert abs(O.interp2d([ni - 0.75, 3.0], u, wet, periodic_i=1) - 0.25) < 1e-7
+    assert O.interp2d([ni - 0.75, 3.0], u, wet, periodic_i=0) == 0.0
+    # an index at the end is pulled back by 1e-7
+    assert abs(O.interp2d([ni - 1.0, nj * 1.0], v, wet) - O.interp2d([ni - 1.0, nj - 1e-7], v, wet)) == 0.0
+
+
+def test_interp3d_levels_and_depth_mask():
+    """interpolate3d_structured (utils.c:1252-1367): linear in k between layer
+    centres, clamped to the first and last layers, and layers deeper than a
+    column's level count drop out (|ktop - k| >= nlevels)."""
+    nk, nj, ni = 5, 4, 4
+    v = np.stack([np.full((nj, ni), 10.0 + 3.0 * k, np.float32) for k in range(nk)])
+    nl = np.full((nj, ni), nk, np.int32)
+    assert abs(O.interp3d([1.2, 1.7], 2.25, v, nl) - (10.0 + 3.0 * 2.25)) < 1e-5
+    assert O.interp3d([1.2, 1.7], -0.4, v, nl) == 10.0
+    assert O.interp3d([1.2, 1.7], nk - 0.6, v, nl) == 10.0 + 3.0 * (nk - 1)
+    # ktop = nk - 1: layer k is |nk - 1 - k| from the surface
+    assert abs(O.interp3d([1.0, 1.0], 1.5, v, nl, ktop=nk - 1) - (10.0 + 3.0 * 1.5)) < 1e-5
+    # the deeper layer is dry in every column: the shallower one alone
+    shallow = np.full((nj, ni), 3, np.int32)
+    assert O.interp3d([2.0, 2.0], 2.5, v, shallow) == v[2, 0, 0]
+
+
+@pytest.mark.parametrize("mode,md", [(1, 12), (3, 8)])
+def test_getHE_cell_centres_innovations_spread_hybrid(mode, md):
+    """das_getHE + das_calcinnandspread: at cell centres (integer indices) HE
+    is the field value; the innovation is y minus the member mean and HA the
+    anomaly from it (FP32), the spread the unbiased sample deviation; in
+    hybrid mode the dynamic members' anomalies are scaled by k_d and the
+    static ones by k_s about the dynamic mean (ensobs.c:434-486); obs not OK
+    stay NaN."""
+    m = 12
+    rng = np.random.default_rng(md)
+    nk, nj, ni = 3, 6, 7
+    fields = [rng.standard_normal((m, nj, ni)).astype(np.float32) + 20.0 + k for k in range(nk)]
+    fields.append(rng.standard_normal((m, nj, ni)).astype(np.float32))          # a 2D variable
+    nl = np.full((nj, ni), nk, np.int32)
+    nobs = 40
+    type_ = rng.integers(0, 2, nobs).astype(np.int16)
+    fi = rng.integers(0, ni, nobs).astype(np.float64)
+    fj = rng.integers(0, nj, nobs).astype(np.float64)
+    fk = rng.integers(0, nk, nobs).astype(np.float32)
+    status = np.zeros(nobs, np.int16)
+    status[5] = 2
+    value = rng.standard_normal(nobs) + 20.0
+    hops = [(1, [0, 1, 2]), (0, [3])]
+    gamma = 0.7
+    HA, s, sd = O.getHE(mode, m, md, gamma, nl, 0, 0, fields, hops, type_, status, fi, fj, fk, value)
+    HE = np.full((m, nobs), np.nan, np.float64)
+    for o in range(nobs):
+        if status[o]:
+            continue
+        f = 3 if type_[o] == 1 else int(fk[o])
+        HE[:, o] = fields[f][:, int(fj[o]), int(fi[o])]
+    if mode == 3:
+        ms = m - md
+        kd = math.sqrt((m - 1) / (md - 1))
+        ks = math.sqrt(gamma * (m - 1) / (ms - 1))
+        mu_d = HE[:md].mean(0)
+        mu_s = HE[md:].mean(0)
+        HE = np.concatenate([(HE[:md] - mu_d) * kd + mu_d, (HE[md:] - mu_s) * ks + mu_d])
+    mean = HE.mean(0)
+    ok = status == 0
+    assert np.allclose(HA[:, ok], (HE - mean)[:, ok], atol=2e-5)
+    assert np.allclose(s[ok], (value - mean)[ok], atol=1e-5)
+    assert np.allclose(sd[ok], (HE - mean)[:, ok].std(0, ddof=1), rtol=1e-5)
+    assert np.isnan(HA[:, ~ok]).all()
