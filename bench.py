@@ -108,10 +108,15 @@ def cpu_side(X, args, mode, scheme_id, md, engine_kw):
     S, s = O.standardise(X.nmem, ob, X.HA, X.innov)
     jj0 = X.nj_s // 2
     lon_all, lat_all = X.node_lon, X.node_lat
+    # one set of the reference's kd-trees per thread (its search keeps its
+    # results in the tree), built before any timing
+    t = time.perf_counter()
+    trees = O.ref_trees_parallel(ob, nth) if use_ref else False
+    log("CPU baseline: %d sets of kd-trees built in %.1f s" % (nth if use_ref else 0, time.perf_counter() - t))
     # calibrate on 8 nodes of the band's first row, one thread
     t = time.perf_counter()
     O.calc_transforms(mode, scheme_id, X.nmem, md, 1.0, ob, S, s, lon_all[jj0, :8], lat_all[jj0, :8],
-                      use_ref=use_ref)
+                      use_ref=trees[0] if use_ref else False)
     per = (time.perf_counter() - t) / 8
     # stride rows for about args.cpu_seconds of wall time on nth threads
     nrow_calc = int(max(2, min(X.nj_s - jj0, args.cpu_seconds * nth / max(per, 1e-6) / X.ni_s)))
@@ -121,8 +126,11 @@ def cpu_side(X, args, mode, scheme_id, md, engine_kw):
         % (kind, nth, nrow_calc, lon.size, 1e3 * per))
     t = time.perf_counter()
     ref = O.calc_transforms(mode, scheme_id, X.nmem, md, 1.0, ob, S, s, lon.ravel(), lat.ravel(),
-                            use_ref=use_ref, threads=nth)
+                            use_ref=trees, threads=nth)
     dt = time.perf_counter() - t
+    if use_ref:
+        for tr in trees:
+            tr.close()
     pbar = ref["stats"]["nlobs_sum"] / max(ref["stats"]["ncell"], 1)
     what = ("reference calc/calcs.c + common/kdtree.c (MKL sequential, %d threads; one set of kd-trees "
             "per thread, built outside the timing) driven by the restated das_calctransforms loop" % nth
@@ -260,7 +268,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall time of each CPU baseline sample")
     ap.add_argument("--precision", choices=["fp64", "both"], default="both",
                     help="both: the FP64 path and the mixed path")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run two plain device copies of GIB GiB (the byte counters' reference reading) and exit")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import synth_gpu
+        nb = int(args.copy_probe * (1 << 30)) // 16 * 16
+        dt = synth_gpu.copy_probe(nb)
+        log("copy probe: %d bytes read and %d written in %.3f ms (%.1f GB/s each way)" % (nb, nb, 1e3 * dt,
+                                                                                          nb / dt / 1e9))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         log("bench.py measures one GPU: the path would shard by stride-row bands balanced by local "

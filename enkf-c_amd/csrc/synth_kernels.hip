@@ -256,6 +256,14 @@ __global__ void uniform_kernel(double* out, long long n, uint32_t seed, uint32_t
     out[i] = u01d(r.x, r.y);
 }
 
+// a plain copy, 16 bytes per thread and step: the reference reading of the
+// hardware byte counters (n16 = bytes / 16)
+__global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, long long n16)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 int ret(hipError_t e)
 {
     return e == hipSuccess ? 0 : (int)e;
@@ -398,6 +406,21 @@ int es_tile_ha(float* HA, double* innov, int nmem, int nobs, long long ncell, co
     a.HA = HA;
     a.innov = innov;
     tile_ha_kernel<<<(nobs + 255) / 256, 256>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    return ret(e);
+}
+
+// copies `bytes` (a multiple of 16) from src to dst with copy_kernel
+int es_copy(void* dst, const void* src, size_t bytes)
+{
+    if (bytes % 16 != 0)
+        return (int)hipErrorInvalidValue;
+    const long long n16 = (long long)(bytes / 16);
+    if (n16 == 0)
+        return 0;
+    copy_kernel<<<2048, 256>>>(static_cast<const float4*>(src), static_cast<float4*>(dst), n16);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipDeviceSynchronize();

@@ -51,6 +51,7 @@ def lib():
         L.es_tile_ha.argtypes = [vp, vp, C.c_int, C.c_int, C.c_longlong, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                  C.c_uint]
         L.es_uniform.argtypes = [vp, C.c_longlong, C.c_uint, C.c_uint]
+        L.es_copy.argtypes = [vp, vp, C.c_size_t]
         _lib = L
     return _lib
 
@@ -320,3 +321,16 @@ def make_clustered_tile(nj=1200, ni=1200, nk=51, nmem=144, nmem_dynamic=96, medi
     # four obstypes are declared; type 0 (SST) has no obs here
     del G, G2
     return X
+
+
+def copy_probe(nbytes):
+    """one plain device copy of nbytes (a multiple of 16) with the generator's
+    copy kernel; returns its wall time in seconds (after a warm-up copy)"""
+    import time
+    a = DeviceArray((nbytes // 4,), np.float32)
+    b = DeviceArray((nbytes // 4,), np.float32)
+    a.zero()
+    _check(lib().es_copy(b.ptr, a.ptr, nbytes), "es_copy")
+    t = time.perf_counter()
+    _check(lib().es_copy(b.ptr, a.ptr, nbytes), "es_copy")
+    return time.perf_counter() - t

@@ -368,8 +368,9 @@ def calc_transforms(mode, scheme, nmem, nmem_dynamic, alpha, obs, S, s, lon, lat
     threads > 1 splits the nodes [node0, node0 + nnode_do) into contiguous
     runs, one per thread (the C loops release the GIL; the oracle is
     reentrant; with use_ref, each thread gets its own reference trees, since
-    kd_findnodeswithinrange keeps its results in the tree).  Outputs are per
-    node, so the runs write disjoint parts; the stats are summed."""
+    kd_findnodeswithinrange keeps its results in the tree; use_ref may be a
+    list of at least `threads` RefTrees to reuse).  Outputs are per node, so
+    the runs write disjoint parts; the stats are summed."""
     import threading
 
     lon = np.ascontiguousarray(lon, np.float64)
@@ -398,6 +399,9 @@ def calc_transforms(mode, scheme, nmem, nmem_dynamic, alpha, obs, S, s, lon, lat
     if use_ref:
         if isinstance(use_ref, RefTrees) and nth == 1:
             trees[0] = (use_ref, False)
+        elif isinstance(use_ref, (list, tuple)) and len(use_ref) >= nth:
+            for k in range(nth):
+                trees[k] = (use_ref[k], False)
         else:
             def mk(k):
                 trees[k] = (RefTrees(obs), True)
@@ -593,3 +597,18 @@ def interp3d(fij, fk, v, nlevels, ktop=0, periodic_i=0):
     L = (_fp * nk)(*[_ptr(a, _fp) for a in layers])
     nlevels = np.ascontiguousarray(nlevels, np.int32)
     return lib().or_interp3d(_ptr(fij, _dp), float(fk), ni, nj, nk, ktop, L, _ptr(nlevels, _ip), periodic_i)
+
+
+def ref_trees_parallel(obs, n):
+    """n independent sets of the reference's kd-trees, built on n threads"""
+    import threading
+    out = [None] * n
+
+    def mk(k):
+        out[k] = RefTrees(obs)
+    ths = [threading.Thread(target=mk, args=(k,)) for k in range(n)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    return out
