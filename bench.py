@@ -318,9 +318,12 @@ def main():
     sample = (X.node_lon[jb:jb + 8], X.node_lat[jb:jb + 8])
     pname = {enkfg.PREC_FP64: "fp64", enkfg.PREC_MIXED: "mixed"}
 
+    spread_ms = None
+
     def run(precision):
         """warmup + timed steps of one precision; returns rates, phases and the
         transforms of the comparison band"""
+        nonlocal spread_ms
         E = enkfg.Engine(m, precision=precision, pertype_diag=True, keep_fp64=False, **engine_kw)
         E.set_obstypes(X.obstypes)
         t0 = time.perf_counter()
@@ -333,6 +336,14 @@ def main():
             ph = E.timings()
             if w == 0:
                 E.ens_attach(X.nlevels, X.stride, 0, 0, X.fields, X.E.ptr)
+                if precision == enkfg.PREC_FP64 and spread_ms is None:
+                    # das_writespread of the forecast: every field, before any update
+                    dsp = synth_gpu.DeviceArray((nfield, X.nj, X.ni), np.float32)
+                    for _ in range(2):
+                        E.ens_spread(0, nfield, (X.nj, X.ni), dev_out=dsp.ptr)
+                    spread_ms = E.spread_ms()
+                    dsp.free()
+                    log("  [fp64] forecast spread of %d fields: kernel %.2f ms" % (nfield, spread_ms))
             if w < args.warmup:
                 E.ens_update(upd_flags)
                 ph[3] = E.timings()[3]
@@ -475,6 +486,12 @@ def main():
                roofline=roof,
                update_roofline=upd_roof,
                calc_buckets=V["tuning"])
+    if spread_ms:
+        B_sp = 4.0 * nfield * ncell * (m + 1)
+        res["spread"] = dict(kernel="spread_cells_kernel<true>", note="das_writespread of the forecast, all fields",
+                             ms=spread_ms, achieved_GBps=B_sp / (spread_ms * 1e-3) / 1e9,
+                             frac=B_sp / (spread_ms * 1e-3) / PEAK_HBM,
+                             algorithmic="4*(m+1) B per cell-field = %.3e B per launch" % B_sp)
     if graph_ms is not None:
         res["graph"] = dict(note="the same calc pass with the Gram/factor batches replayed as one captured "
                                  "HIP graph (ENKFG_FLAG_GRAPH)", calc_ms=graph_ms,

@@ -96,7 +96,7 @@ struct enkfg_ctx {
     hipStream_t stream2 = nullptr;   // factor kernels, overlapping the next batch's Gram
     hipEvent_t ev[10] = {};
     std::vector<hipEvent_t> bev;     // 4 per calc batch
-    double timings[7] = {0, 0, 0, 0, 0, 0, 0};
+    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
     // obstypes and obs
     std::vector<enkfg_obstype> types;
@@ -148,7 +148,7 @@ struct enkfg_ctx {
     enkfg_stats stats{};
 
     // update
-    DevBuf d_E, d_infl, d_nlev, d_flevel, d_fsurf, d_flog, d_finfl, d_fratio;
+    DevBuf d_E, d_infl, d_nlev, d_flevel, d_fsurf, d_flog, d_finfl, d_fratio, d_spread;
     // RANDOMISE: the context's drand48 stream (48-bit state) and the noise
     // vectors of the randomised fields of the last batch
     unsigned long long rand48 = 0;
@@ -561,7 +561,9 @@ void enkfg_destroy(enkfg_ctx* c)
                       &c->d_srow, &c->d_sd2, &c->d_T, &c->d_w, &c->d_T64, &c->d_w64, &c->d_pnlobs,
                       &c->d_pdfs, &c->d_psrf, &c->d_gram, &c->d_order, &c->d_sts, &c->d_err, &c->d_E, &c->d_infl, &c->d_nlev,
                       &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio, &c->d_adjg, &c->d_adjflag,
-                      &c->d_adjlist, &c->d_adjz, &c->d_adjw, &c->d_gy, &c->d_gR};
+                      &c->d_adjlist, &c->d_adjz, &c->d_adjw, &c->d_gy, &c->d_gR, &c->d_etkf, &c->d_noise,
+                      &c->d_he_order, &c->d_he_gcell, &c->d_he_goff, &c->d_he_infl, &c->d_he_ratio, &c->d_hg_tab,
+                      &c->d_hg_obs, &c->d_hg_std, &c->d_spread};
     for (DevBuf* b : bufs)
         b->release();
     for (auto& e : c->ev)
@@ -1444,7 +1446,7 @@ int enkfg_get_timings(enkfg_ctx* c, double* ms, int n)
 {
     if (check_ctx(c) || ms == nullptr)
         return ENKFG_ERR_ARG;
-    for (int k = 0; k < n && k < 7; ++k)
+    for (int k = 0; k < n && k < 8; ++k)
         ms[k] = c->timings[k];
     return ENKFG_OK;
 }
@@ -2002,6 +2004,57 @@ int enkfg_ens_update(enkfg_ctx* c, int flags, float* dev_infl)
     HIPCHECK(c, hipEventSynchronize(c->ev[6]));
     c->timings[3] = elapsed(c->ev[5], c->ev[6]);
     return apply_randomise(c, c->att_nfield, c->att_fields.data(), c->attached_E, a.ncell);
+}
+
+int enkfg_ens_spread(enkfg_ctx* c, int f0, int nfield, float* spread)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (c->attached_E == nullptr)
+        return fail(c, ENKFG_ERR_STATE, "no ensemble attached");
+    if (f0 < 0 || nfield <= 0 || f0 + nfield > c->att_nfield || spread == nullptr)
+        return fail(c, ENKFG_ERR_ARG, "invalid field range %d + %d of %d, or NULL output", f0, nfield, c->att_nfield);
+    const int m = c->cfg.nmem;
+    const long long ncell = (long long)c->att_grid.mni * c->att_grid.mnj;
+    const size_t bytes = (size_t)nfield * ncell * sizeof(float);
+    HIPCHECK(c, c->d_spread.reserve(bytes));
+    HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+    HIPCHECK(c, launch_spread(c->stream, c->attached_E + (size_t)f0 * ncell * m, nfield, ncell, m, true,
+                              c->d_spread.as<float>()));
+    HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
+    HIPCHECK(c, hipMemcpyAsync(spread, c->d_spread.p, bytes, hipMemcpyDefault, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->timings[7] = elapsed(c->ev[5], c->ev[6]);
+    return ENKFG_OK;
+}
+
+int enkfg_spread_fields(enkfg_ctx* c, int nfield, int nij, float* const* fieldbuffer, float* spread)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (nfield <= 0 || nij <= 0 || fieldbuffer == nullptr || spread == nullptr)
+        return fail(c, ENKFG_ERR_ARG, "invalid field buffers");
+    const int m = c->cfg.nmem;
+    const size_t per = (size_t)m * nij;
+    HIPCHECK(c, c->d_tmp.reserve(per * sizeof(float)));
+    HIPCHECK(c, c->d_spread.reserve((size_t)nfield * nij * sizeof(float)));
+    double t = 0.0;
+    for (int f = 0; f < nfield; ++f) {
+        if (fieldbuffer[f] == nullptr)
+            return fail(c, ENKFG_ERR_ARG, "field buffer %d is NULL", f);
+        HIPCHECK(c, hipMemcpyAsync(c->d_tmp.p, fieldbuffer[f], per * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+        HIPCHECK(c, launch_spread(c->stream, c->d_tmp.as<float>(), 1, nij, m, false,
+                                  c->d_spread.as<float>() + (size_t)f * nij));
+        HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
+        HIPCHECK(c, hipEventSynchronize(c->ev[6]));
+        t += elapsed(c->ev[5], c->ev[6]);
+    }
+    HIPCHECK(c, hipMemcpyAsync(spread, c->d_spread.p, (size_t)nfield * nij * sizeof(float), hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->timings[7] = t;
+    return ENKFG_OK;
 }
 
 int enkfg_adjoint(enkfg_ctx* c, const float* dev_dJ_dmean, double* dJ_dy, double* dJ_dR, double* dJ_dinfl)

@@ -215,6 +215,11 @@ hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a);
 // addnoise (update.c:46-87) on one field E [ncell][m]:
 // v <- (float)((double)(v * defl) + (double)mult * r[e])
 hipError_t launch_addnoise(hipStream_t s, float* E, long long ncell, int m, float defl, float mult, const double* r);
+
+// das_writespread's spread (diag_kernels.hip) of nfield fields: E is
+// [field][cell][m] when cells_major, else [field][m][cell]; out [field][cell]
+hipError_t launch_spread(hipStream_t s, const float* E, int nfield, long long ncell, int m, bool cells_major,
+                         float* out);
 hipError_t launch_hybrid_condition(hipStream_t s, const UpdArgs& a, double gamma);
 // [nmem][rows][cols] <-> [rows*cols][nmem]
 hipError_t launch_members_to_cells(hipStream_t s, const float* in, float* out, int nmem, long long ncell);

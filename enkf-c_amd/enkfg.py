@@ -34,6 +34,7 @@ EXPORTED = [
     "enkfg_set_transforms", "enkfg_update_fields", "enkfg_update_bg", "enkfg_ens_attach", "enkfg_ens_update",
     "enkfg_get_timings", "enkfg_get_tuning", "enkfg_adjoint", "enkfg_set_rand48", "enkfg_get_rand48",
     "enkfg_rand48_normals", "enkfg_update_HE", "enkfg_update_fields_noalloc", "enkfg_calc_HE",
+    "enkfg_ens_spread", "enkfg_spread_fields",
 ]
 
 
@@ -122,6 +123,8 @@ def lib():
         L.enkfg_calc_HE.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Hop), vp, vp, vp, vp, vp, vp, vp]
         L.enkfg_update_fields_noalloc.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Field), C.c_int, C.c_int,
                                                   C.POINTER(vp), vp, C.c_int]
+        L.enkfg_ens_spread.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.enkfg_spread_fields.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp), vp]
         L.enkfg_set_rand48.argtypes = [vp, C.c_ulonglong]
         L.enkfg_get_rand48.argtypes = [vp, C.POINTER(C.c_ulonglong)]
         L.enkfg_rand48_normals.argtypes = [C.POINTER(C.c_ulonglong), C.c_int, vp]
@@ -318,6 +321,34 @@ class Engine:
         t = (C.c_double * 7)()
         self._check(lib().enkfg_get_timings(self.h, t, 7))
         return t[6]
+
+    def spread_ms(self):
+        """kernel time of the last spread"""
+        t = (C.c_double * 8)()
+        self._check(lib().enkfg_get_timings(self.h, t, 8))
+        return t[7]
+
+    def ens_spread(self, f0, nfield, shape, dev_out=None):
+        """das_writespread of attached fields f0 .. f0+nfield-1: float32
+        [nfield][mnj][mni] (shape = (mnj, mni)); into dev_out (a device
+        pointer) instead when given."""
+        if dev_out is not None:
+            self._check(lib().enkfg_ens_spread(self.h, f0, nfield, C.c_void_p(dev_out)))
+            return None
+        out = np.empty((nfield,) + tuple(shape), np.float32)
+        self._check(lib().enkfg_ens_spread(self.h, f0, nfield, _p(out)))
+        return out
+
+    def spread_fields(self, buffers):
+        """das_writespread of host field buffers [nmem][...] (float32,
+        C-contiguous): float32 [nfield][...]"""
+        for b in buffers:
+            assert b.dtype == np.float32 and b.flags.c_contiguous and b.shape[0] == self.nmem
+        nij = int(np.prod(buffers[0].shape[1:]))
+        out = np.empty((len(buffers),) + buffers[0].shape[1:], np.float32)
+        bp = (C.c_void_p * len(buffers))(*[b.ctypes.data for b in buffers])
+        self._check(lib().enkfg_spread_fields(self.h, len(buffers), nij, bp, _p(out)))
+        return out
 
     def adjoint(self, dev_dJ_dmean, nobs):
         """Gradients of J with respect to the observation values and the

@@ -289,6 +289,19 @@ int enkfg_update_bg(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const en
 int enkfg_ens_attach(enkfg_ctx* ctx, const enkfg_grid* grid, int nfield, const enkfg_field* fields, float* dev_E);
 int enkfg_ens_update(enkfg_ctx* ctx, int flags, float* dev_infl);
 
+/* ---------------------------------------------------------------- diagnostics */
+/* das_writespread (diag/diags.c:79-168): the spread of each field over all
+ * nmem members, sqrt(max(0, sum v^2 / nmem - mean^2)) with the sums in
+ * double in member order and v^2 a float product, 0 where |mean| >
+ * MAXOBSVAL; the values spread.nc holds.  The reference writes the forecast
+ * spread: call before the update.
+ *   enkfg_ens_spread: fields f0 .. f0 + nfield - 1 of the attached ensemble;
+ *     spread [nfield][mnj][mni], host or device memory.
+ *   enkfg_spread_fields: host field buffers as the update takes them,
+ *     fieldbuffer[f] [nmem][nij] (nij = jsize * mni); spread [nfield][nij]. */
+int enkfg_ens_spread(enkfg_ctx* ctx, int f0, int nfield, float* spread);
+int enkfg_spread_fields(enkfg_ctx* ctx, int nfield, int nij, float* const* fieldbuffer, float* spread);
+
 /* ---------------------------------------------------------------- adjoint */
 /* Adjoint of the analysis mean.  No counterpart in the reference; it
  * differentiates the reference's own forward path: w = M^-1 S^T s per node
@@ -315,7 +328,8 @@ int enkfg_adjoint(enkfg_ctx* ctx, const float* dev_dJ_dmean, double* dJ_dy, doub
  * device time of the whole calc (the factor kernels of one batch run
  * concurrently with the Gram kernel of the next, so [1] + [2] may exceed
  * the time they take together); [6] is the kernel time of the last
- * enkfg_calc_HE. */
+ * enkfg_calc_HE and [7] of the last spread (enkfg_ens_spread,
+ * enkfg_spread_fields). */
 int enkfg_get_timings(enkfg_ctx* ctx, double* ms, int n);
 
 /* Calc batching and the autotuner's choices (no counterpart in the

@@ -1727,3 +1727,32 @@ int or_getHE(int mode, int nmem, int nmem_dynamic, double gamma, int ni, int nj,
     }
     return OR_OK;
 }
+
+/* das_writespread (diag/diags.c:79-168), the reduction at 101-123: sums over
+ * all nmem members in member order, in double; the square is a product of
+ * two floats (float arithmetic, then promoted); population variance clamped
+ * at 0; spread 0 where |mean| > MAXOBSVAL (global.h:119); written as float
+ * (ncw_put_var_double of the tile file converts). */
+void or_spread(int nmem, long long nij, const float* const* v, float* spread)
+{
+    long long i;
+    int e;
+
+    for (i = 0; i < nij; ++i) {
+        double v1 = 0.0, v2 = 0.0;
+
+        for (e = 0; e < nmem; ++e) {
+            float x = v[e][i];
+            float x2 = x * x;
+
+            v1 += x;
+            v2 += x2;
+        }
+        v1 /= (double) nmem;
+        v2 = v2 / (double) nmem - v1 * v1;
+        v2 = (v2 < 0.0) ? 0.0 : sqrt(v2);
+        if (fabs(v1) > OR_MAXOBSVAL)
+            v2 = 0.0;
+        spread[i] = (float) v2;
+    }
+}

@@ -191,6 +191,9 @@ int or_getHE(int mode, int nmem, int nmem_dynamic, double gamma, int ni, int nj,
 double or_drand48(unsigned long long* state);
 void or_get_normalpair(unsigned long long* state, double x[2]);
 void or_addnoise(int nmem, double deflation, double sigma, unsigned long long* state, long long n, float** v);
+/* das_writespread's arithmetic (diag/diags.c:101-123) for one field layer:
+ * v is [nmem][nij]; spread [nij] */
+void or_spread(int nmem, long long nij, const float* const* v, float* spread);
 /* hybrid conditioning of one layer; v is [nmem][nij], vmean scratch [nij] */
 void or_sethybridensemble(int nmem, int nmem_dynamic, double gamma, int nij, float* v, double* vmean);
 

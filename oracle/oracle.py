@@ -114,6 +114,7 @@ def lib():
         L.or_getHE.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, _ip,
                                C.POINTER(_fp), C.c_int, _sp, _sp, _dp, _dp, _fp, _dp, _ip, _ip, C.POINTER(_ip), _fp,
                                _dp, _dp]
+        L.or_spread.argtypes = [C.c_int, C.c_longlong, C.POINTER(_fp), _fp]
         L.or_drand48.restype = C.c_double
         L.or_drand48.argtypes = [C.POINTER(C.c_ulonglong)]
         L.or_get_normalpair.argtypes = [C.POINTER(C.c_ulonglong), _dp]
@@ -504,6 +505,17 @@ def sethybridensemble(nmem, nmem_dynamic, gamma, v):
     nij = v.shape[1]
     vmean = np.zeros(nij, np.float64)
     lib().or_sethybridensemble(nmem, nmem_dynamic, float(gamma), nij, _ptr(v, _fp), _ptr(vmean, _dp))
+
+
+def spread(v):
+    """das_writespread's spread of v: float32 [nmem][...] -> float32 [...]"""
+    v = np.ascontiguousarray(v, np.float32)
+    nmem = v.shape[0]
+    n = int(np.prod(v.shape[1:]))
+    out = np.empty(v.shape[1:], np.float32)
+    rows = (_fp * nmem)(*[v[e].ctypes.data_as(_fp) for e in range(nmem)])
+    lib().or_spread(nmem, n, rows, _ptr(out, _fp))
+    return out
 
 
 def normalpairs(state, npair):

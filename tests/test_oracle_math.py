@@ -658,3 +658,30 @@ def test_getHE_cell_centres_innovations_spread_hybrid(mode, md):
     assert np.allclose(s[ok], (value - mean)[ok], atol=1e-5)
     assert np.allclose(sd[ok], (HE - mean)[:, ok].std(0, ddof=1), rtol=1e-5)
     assert np.isnan(HA[:, ~ok]).all()
+
+
+def test_spread_population_std_float_square_and_limits():
+    """or_spread (diag/diags.c:101-123) against numpy's population standard
+    deviation; the members' squares are float products (a constant member
+    value whose float square is above its exact square gives a spread of
+    sqrt(float(c*c) - c^2), not 0); |mean| > MAXOBSVAL gives 0; a negative
+    variance from rounding is clamped to 0."""
+    rng = np.random.default_rng(11)
+    v = (rng.standard_normal((96, 40, 30)) * 0.7).astype(np.float32)
+    sp = O.spread(v)
+    ref = v.astype(np.float64).std(axis=0)          # ddof 0: / nmem, not / (nmem - 1)
+    assert np.abs(sp - ref).max() <= 2e-6 * ref.max()
+    assert np.abs(sp - v.astype(np.float64).std(axis=0, ddof=1)).max() > 1e-3 * ref.max()
+    # constant members: the float square decides (above the exact square: a
+    # positive spread; below: a negative variance, clamped to 0)
+    cs = [np.float32(1.0 + 0.01 * k) for k in range(1, 100)]
+    up = next(c for c in cs if float(np.float32(c * c)) > np.float64(c) ** 2)
+    down = next(c for c in cs if float(np.float32(c * c)) < np.float64(c) ** 2)
+    gap = float(np.float32(up * up)) - np.float64(up) ** 2
+    assert np.all(O.spread(np.full((24, 5), up, np.float32)) == np.float32(np.sqrt(gap)))
+    assert np.all(O.spread(np.full((24, 3), down, np.float32)) == 0.0)
+    # |mean| > MAXOBSVAL: 0, whatever the variance
+    z = rng.standard_normal((24, 7))
+    assert np.all(O.spread((z * 1e5 + 2e6).astype(np.float32)) == 0.0)
+    assert np.all(O.spread((z * 1e5 - 2e6).astype(np.float32)) == 0.0)
+    assert np.all(O.spread((z * 1e5 + 4e5).astype(np.float32)) > 1e4)
