@@ -1,21 +1,28 @@
 // update_mfma.hip -- das_updatefields (update/update.c:92-564) on the matrix
 // cores.
 //
-// One workgroup (4 waves) per grid cell; the cell's fields are processed in
-// groups of 64, wave w owning fields 16w .. 16w + 15 of the group.  Per cell:
+// One workgroup of 8 waves per grid cell.  The cell's fields are cut into
+// tiles of 16; wave w takes tile w (and w + 8, ... when there are more than
+// 128 fields).  Per cell:
 //
-//   1. T and w are interpolated from the stride grid into LDS exactly as
-//      update.c:195-272 does (FP32, j first then i, no contraction).
-//   2. Per group: the 64 field records E[f][cell][0..m) are loaded, and each
-//      field's statistics are formed by 4 lanes (a quarter of the members
-//      each): first bad member (missing values, update.c:337-371), FP64 mean
-//      of the dynamic members, FP32 anomalies, x_a = x_f + sum v_f w
-//      (FP32 products, FP64 sums), and the spread sum for the capped
-//      inflation.  The FP64 sums are combined in a fixed order; they equal
-//      the reference's sequential sums whenever those are exact (members
-//      within 2^29 of each other in magnitude), and differ by rounding only
-//      otherwise.
-//   3. V^a = V^f T_cell^T as 16x16 MFMA tiles: FP32 inputs with FP32
+//   1. Each wave starts loading its tile's records; meanwhile the eight waves
+//      interpolate T and w from the stride grid into LDS exactly as
+//      update.c:195-272 does (FP32, j first then i, no contraction), with the
+//      corner loads and arithmetic of the cell's case (node, j only, i only,
+//      both).
+//   2. In the wave's registers: lane (col, kq) = (lane % 16, lane / 16)
+//      holds field 16 t + col at members 16 s + 4 kq + c (s < KT, c < 4),
+//      loaded as float4.  Those registers are the A operand of the product as
+//      they stand: the product's summation runs over the members in this
+//      order, and T is stored in LDS in the same order.  The lane forms its
+//      share of the field's statistics there: missing values (update.c:
+//      337-371), FP64 mean of the dynamic members, FP32 anomalies,
+//      x_a = x_f + sum v_f w (FP32 products, FP64 sums) and the spread sum of
+//      the capped inflation; the four lanes of a field combine their partial
+//      sums in a fixed order.  The FP64 sums equal the reference's sequential
+//      sums whenever those are exact (members within 2^29 of each other in
+//      magnitude), and differ by rounding only otherwise.
+//   3. V^a = V^f T_cell^T as 16 x 16 MFMA tiles: FP32 inputs with FP32
 //      accumulation (v_mfma_f32_16x16x4_f32, the arithmetic class of the
 //      reference's sgemv), or, in mixed precision, FP16 hi/lo splits of both
 //      operands with FP32 accumulation (v_mfma_f32_16x16x32_f16, three
@@ -23,7 +30,8 @@
 //      power of two into FP16's normal range first.
 //   4. Capped inflation (update.c:432-458) from the accumulators (the spread
 //      sum is a 16-lane reduction), then the analysis / increment / log
-//      increment outputs (update.c:459-526).
+//      increment outputs (update.c:459-526); each store writes 16
+//      consecutive members of 4 fields.
 #include "enkfg_internal.h"
 #include "update_interp.h"
 
@@ -31,37 +39,52 @@ namespace enkfg {
 
 namespace {
 
-constexpr float kMaxVal = 999999.0f;   // MAXOBSVAL, common/global.h:114
+constexpr float kMaxVal = 999999.0f;   // MAXOBSVAL, common/global.h:119
 constexpr float kEps = 1.0e-6f;        // EPSF, update/update.c:41
-constexpr int FG = 64;                 // fields per group (16 per wave)
+constexpr int kWaves = 8;              // waves per workgroup, one field tile each
 
 typedef float flt4 __attribute__((__vector_size__(4 * sizeof(float))));
 typedef _Float16 half8 __attribute__((__vector_size__(8 * sizeof(_Float16))));
+typedef _Float16 half4 __attribute__((__vector_size__(4 * sizeof(_Float16))));
 
-template <int MT, bool MIX>
-struct UpdGeom {
-    static constexpr int MP = 16 * MT;                          // padded members
-    static constexpr int KP = MIX ? 32 * ((MP + 31) / 32) : MP; // padded K of the product
-    // row strides == 16 or 48 mod 64 floats (the 4 rows of an operand read
-    // fall in disjoint banks), tighter
-    // for the largest ensembles so that the workgroup fits in LDS
-    static constexpr int AS = (MT <= 9) ? FG + 16 : FG + 4;                          // at
-    static constexpr int TS = (MT <= 9) ? MP + ((16 - MP % 32) + 32) % 32 : MP + 4;  // tt
-    static constexpr int HS = KP + 8;                           // th/tl row stride (halves, 16-byte rows)
+// KT: member tiles of the summation (16 KT >= m); OT: output member tiles
+// (16 OT >= m_d)
+template <int KT, int OT, bool MIX>
+struct TileGeom {
+    static constexpr int NK = 16 * KT;                        // summation length, padded
+    static constexpr int NKH = 32 * ((KT + 1) / 2);           // ... for the K = 32 half-precision steps
+    static constexpr int NO = 16 * OT;                        // output members, padded
+    // FP32 row stride of tk == 16 or 48 (mod 64) floats: the four rows a
+    // B-operand read touches fall in disjoint banks
+    static constexpr int LD = NO + ((16 - NO % 32) + 32) % 32;
+    static constexpr int HS = NKH + 8;                        // half row stride of th/tl (16-byte rows)
 };
 
-template <int MT, bool MIX>
-struct UpdLds {
-    using G = UpdGeom<MT, MIX>;
-    float at[G::KP][G::AS];                        // at[e][f]: values, then anomalies (0 for e >= m)
-    float tt[MIX ? 1 : G::KP][MIX ? 1 : G::TS];    // tt[e2][e1 ^ ((e2 / 4) % 16)] = T_cell[e1][e2] (FP32 path)
-    alignas(16) _Float16 th[MIX ? G::MP : 1][MIX ? G::HS : 8];   // th[e1][e2] = hi(T_cell[e1][e2]) (mixed)
-    alignas(16) _Float16 tl[MIX ? G::MP : 1][MIX ? G::HS : 8];   // lo part
-    float wc[G::KP];
-    double v1f[FG], v1a[FG], v2f[FG];
-    float scale[FG];
-    int zero[FG];              // 0 update, 1 all members zero, 2 static members zero
+template <int KT, int OT, bool MIX>
+struct TileLds {
+    using G = TileGeom<KT, OT, MIX>;
+    // FP32: tk[kpos32(e2)][e1] = T_cell[e1][e2]
+    alignas(16) float tk[MIX ? 1 : G::NK][MIX ? 1 : G::LD];
+    // mixed: th/tl[e1][kpos16(e2)] = hi/lo(T_cell[e1][e2])
+    alignas(16) _Float16 th[MIX ? G::NO : 1][MIX ? G::HS : 8];
+    alignas(16) _Float16 tl[MIX ? G::NO : 1][MIX ? G::HS : 8];
+    float wc[MIX ? G::NKH : G::NK];                           // w_cell[e2], 0 for e2 >= m
 };
+
+// position of member e in the summation order of the FP32 product (4-step
+// MFMA): step s = 4 (e / 16) + e % 4 is taken by lane group kq = (e / 4) % 4
+__device__ inline int kpos32(int e)
+{
+    return 16 * (e >> 4) + 4 * (e & 3) + ((e >> 2) & 3);
+}
+
+// ... of the half-precision product (32-step MFMA): lane group kq = (e / 4) % 4
+// supplies 8 consecutive positions, members 32 t + 4 kq + c and
+// 32 t + 16 + 4 kq + c
+__device__ inline int kpos16(int e)
+{
+    return 32 * (e >> 5) + 8 * ((e >> 2) & 3) + 4 * ((e >> 4) & 1) + (e & 3);
+}
 
 __device__ inline long long cell_of_block(long long ncell)
 {
@@ -98,43 +121,188 @@ __device__ __attribute__((always_inline)) inline double row16_sum(double x)
     return x;
 }
 
-// the records of field group f0 into registers, coalesced: float q of field
-// f is lane-contiguous; fields past the end re-read the last one
-template <int MT, int NL>
-__device__ __attribute__((always_inline)) inline void load_group(const UpdArgs& a, long long cell, int f0,
-                                                                 float (&v)[NL])
+// the four lanes of a field (col, col + 16, col + 32, col + 48) combine
+// their partial sums, in lane order
+__device__ __attribute__((always_inline)) inline double sum4(double p, int col)
 {
-    constexpr int MP = 16 * MT;
+    const double s0 = __shfl(p, col), s1 = __shfl(p, col + 16);
+    const double s2 = __shfl(p, col + 32), s3 = __shfl(p, col + 48);
+    return ((s0 + s1) + s2) + s3;
+}
+
+// a field (col) is flagged when any of its four lanes is
+__device__ __attribute__((always_inline)) inline bool any4(bool b, int col)
+{
+    const unsigned long long m = __ballot(b);
+    return ((m >> col) | (m >> (col + 16)) | (m >> (col + 32)) | (m >> (col + 48))) & 1ull;
+}
+
+// records of tile t: lane (col, kq) reads field 16 t + col (fields past the
+// end re-read the last one) at members 16 s + 4 kq + c
+template <int KT, bool VEC>
+__device__ __attribute__((always_inline)) inline void load_tile(const UpdArgs& a, long long cell, int t, int col,
+                                                                int kq, float (&x)[KT][4])
+{
+    const int f = min(16 * t + col, a.nfield - 1);
+    const float* rec = a.E + ((long long)f * a.ncell + cell) * a.m;
 #pragma unroll
-    for (int u = 0; u < NL; ++u) {
-        const int q = (int)threadIdx.x + 256 * u;
-        const int f = q / MP, e = q - f * MP;
-        const int fg = min(f0 + min(f, FG - 1), a.nfield - 1);
-        v[u] = a.E[((long long)fg * a.ncell + cell) * a.m + min(e, a.m - 1)];
+    for (int s = 0; s < KT; ++s) {
+        const int e = 16 * s + 4 * kq;
+        if constexpr (VEC) {
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (16 * (s + 1) <= a.m || e < a.m)   // m % 4 == 0: the whole quad is in the record
+                v = *reinterpret_cast<const float4*>(rec + e);
+            x[s][0] = v.x;
+            x[s][1] = v.y;
+            x[s][2] = v.z;
+            x[s][3] = v.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                x[s][c] = (e + c < a.m) ? rec[e + c] : 0.0f;
+        }
     }
 }
 
-template <int MT, bool MIX>
-__device__ inline void store_t(UpdLds<MT, MIX>& sh, int e1, int e2, float t)
+template <int KT, int OT, bool MIX>
+__device__ inline void store_t(TileLds<KT, OT, MIX>& sh, int e1, int e2, float t)
 {
     if constexpr (MIX) {
         const _Float16 h = (_Float16)t;
-        sh.th[e1][e2] = h;
-        sh.tl[e1][e2] = (_Float16)(t - (float)h);
+        sh.th[e1][kpos16(e2)] = h;
+        sh.tl[e1][kpos16(e2)] = (_Float16)(t - (float)h);
     } else {
-        // column swizzle within 16-member blocks: the lanes of a store hold
-        // e2 four apart, all in one bank without it
-        sh.tt[e2][e1 ^ ((e2 >> 2) & 15)] = t;
+        sh.tk[kpos32(e2)][e1] = t;
     }
 }
 
-// two workgroups per CU (LDS permitting): at most 256 registers per lane
-template <int MT, bool MIX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) update_mfma_kernel(UpdArgs a)
+// four consecutive members e2 .. e2 + 3 (e2 % 4 == 0) of row e1
+template <int KT, int OT, bool MIX>
+__device__ inline void store_t4(TileLds<KT, OT, MIX>& sh, int e1, int e2, const float (&t)[4])
 {
-    using G = UpdGeom<MT, MIX>;
-    constexpr int MP = G::MP, KP = G::KP;
-    __shared__ UpdLds<MT, MIX> sh;
+    if constexpr (MIX) {
+        const int k = kpos16(e2);   // the four are consecutive positions
+        half4 h, l;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            h[c] = (_Float16)t[c];
+            l[c] = (_Float16)(t[c] - (float)h[c]);
+        }
+        *reinterpret_cast<half4*>(&sh.th[e1][k]) = h;
+        *reinterpret_cast<half4*>(&sh.tl[e1][k]) = l;
+    } else {
+        const int k = kpos32(e2);   // positions k, k + 4, k + 8, k + 12
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            sh.tk[k + 4 * c][e1] = t[c];
+    }
+}
+
+// one interpolated value in the cell's case: CASE bit 0 = interpolate in j,
+// bit 1 = in i (update.c:224-271; the operations of bilin() for that case)
+template <int CASE>
+__device__ __attribute__((always_inline)) inline float interp(const Bilin& b, float v00, float v10, float v01,
+                                                              float v11)
+{
+    if constexpr (CASE == 0)
+        return v00;
+    if constexpr (CASE == 1)
+        return v00 * b.wj1 + v10 * b.wj2;
+    if constexpr (CASE == 2)
+        return v00 * b.wi1 + v01 * b.wi2;
+    const float ta = v00 * b.wj1 + v10 * b.wj2;
+    const float tb = v01 * b.wj1 + v11 * b.wj2;
+    return ta * b.wi1 + tb * b.wi2;
+}
+
+// T_cell into LDS, loading only the corners the case uses
+template <int KT, int OT, bool MIX, bool VEC, int CASE>
+__device__ inline void form_t(const UpdArgs& a, const Bilin& b, TileLds<KT, OT, MIX>& sh, int tid)
+{
+    constexpr int NT = 64 * kWaves;
+    constexpr bool J = (CASE & 1) != 0, I = (CASE & 2) != 0;
+    const int m = a.m, md = a.md;
+    if constexpr (VEC) {
+        // quads of 4 consecutive e2: row e1 = q / (m / 4) by a multiply-high
+        // (exact for these small q)
+        constexpr int UB = 3;
+        const int d = m >> 2;
+        const unsigned magic = 0xffffffffu / (unsigned)d + 1u;
+        const int n = md * d;
+        for (int q0 = tid; q0 < n; q0 += NT * UB) {
+            float4 v00[UB], v10[UB], v01[UB], v11[UB];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const int q = 4 * min(q0 + NT * u, n - 1);   // in range; duplicates are not stored
+                v00[u] = *reinterpret_cast<const float4*>(a.T + b.o00 + q);
+                if (J)
+                    v10[u] = *reinterpret_cast<const float4*>(a.T + b.o10 + q);
+                if (I)
+                    v01[u] = *reinterpret_cast<const float4*>(a.T + b.o01 + q);
+                if (J && I)
+                    v11[u] = *reinterpret_cast<const float4*>(a.T + b.o11 + q);
+            }
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const int q = q0 + NT * u;
+                if (q < n) {
+                    const int e1 = (int)__umulhi((unsigned)q, magic);
+                    const int e2 = 4 * (q - e1 * d);
+                    const float t[4] = {interp<CASE>(b, v00[u].x, v10[u].x, v01[u].x, v11[u].x),
+                                        interp<CASE>(b, v00[u].y, v10[u].y, v01[u].y, v11[u].y),
+                                        interp<CASE>(b, v00[u].z, v10[u].z, v01[u].z, v11[u].z),
+                                        interp<CASE>(b, v00[u].w, v10[u].w, v01[u].w, v11[u].w)};
+                    store_t4<KT, OT, MIX>(sh, e1, e2, t);
+                }
+            }
+        }
+    } else {
+        constexpr int UB = 4;
+        const unsigned magic = 0xffffffffu / (unsigned)m + 1u;
+        const int n = md * m;
+        for (int q0 = tid; q0 < n; q0 += NT * UB) {
+            float v00[UB], v10[UB], v01[UB], v11[UB];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const int q = min(q0 + NT * u, n - 1);   // in range; duplicates are not stored
+                v00[u] = a.T[b.o00 + q];
+                v10[u] = J ? a.T[b.o10 + q] : 0.0f;
+                v01[u] = I ? a.T[b.o01 + q] : 0.0f;
+                v11[u] = (J && I) ? a.T[b.o11 + q] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const int q = q0 + NT * u;
+                if (q < n) {
+                    const int e1 = (int)__umulhi((unsigned)q, magic), e2 = q - e1 * m;
+                    store_t<KT, OT, MIX>(sh, e1, e2, interp<CASE>(b, v00[u], v10[u], v01[u], v11[u]));
+                }
+            }
+        }
+    }
+}
+
+// the log increment of an APPLYLOG variable (update.c:472-516), kept out of
+// line: it is rare, and inlined it would take registers from the common path
+__device__ __attribute__((noinline)) float log_increment(float va, float vf)
+{
+    return isnormal(vf) ? (float)(exp10((double)va) - exp10((double)vf)) : 0.0f;
+}
+
+// waves per SIMD the register budget allows (the LDS tiles of the larger
+// ensembles allow fewer workgroups anyway)
+template <int KT>
+constexpr int tile_waves()
+{
+    return KT <= 6 ? 4 : (KT <= 8 ? 3 : 2);
+}
+
+template <int KT, int OT, bool MIX, bool VEC>
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(tile_waves<KT>())))
+update_tile_kernel(UpdArgs a)
+{
+    using G = TileGeom<KT, OT, MIX>;
+    __shared__ TileLds<KT, OT, MIX> sh;
 
     const long long cell = cell_of_block(a.ncell);
     if (cell >= a.ncell)
@@ -142,299 +310,267 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) u
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
+    const int col = lane & 15, kq = lane >> 4;
     const int jl = (int)(cell / a.mni);
     const int i = (int)(cell % a.mni);
     const int j = a.j1 + jl;
     const int m = a.m, md = a.md;
     const bool hybrid = (a.mode == ENKFG_MODE_HYBRID);
     const double k_d = (md > 1) ? sqrt((double)(m - 1) / (double)(md - 1)) : 1.0;
+    // the hybrid rescaling v /= k_d (update.c:420-427) as a product with the
+    // reciprocal: the FP64 results differ by at most an ulp of FP64, so the
+    // FP32 values stored agree except at rounding ties
+    const double inv_kd = 1.0 / k_d;
     const int nl = a.nlevels[(long long)j * a.mni + i];
-    // the first field group's records are loaded while T_cell is formed; each
-    // later group's while the previous one is processed
-    constexpr int NL = (FG * MP + 255) / 256;
-    float vnext[NL];
-    load_group<MT, NL>(a, cell, 0, vnext);
+    const int ntiles = (a.nfield + 15) / 16;
+
+    // the wave's first tile is loaded while T_cell is formed
+    float x[KT][4];
+    if (w < ntiles)
+        load_tile<KT, VEC>(a, cell, w, col, kq, x);
 
     // ---- 1. T_cell and w_cell (zero outside md x m)
     {
-        const long long rec = (long long)md * m;
-        const Bilin b = make_bilin(a, j, i, rec);
-        if ((m & 3) == 0) {
-            // quads of 4 consecutive e2: row e1 = q / (m / 4) by a multiply-high
-            // (exact for these small q)
-            constexpr int UB = 5;   // md*m/4 = 2304 quads at m = 96: two rounds of loads
-            const int d = m >> 2;
-            const unsigned magic = 0xffffffffu / (unsigned)d + 1u;
-            const int n = md * d;
-            for (int q0 = tid; q0 < n; q0 += 256 * UB) {
-                float4 v00[UB], v10[UB], v01[UB], v11[UB];
-#pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    const int q = 4 * min(q0 + 256 * u, n - 1);   // in range; duplicates are not stored
-                    v00[u] = *reinterpret_cast<const float4*>(a.T + b.o00 + q);
-                    v10[u] = *reinterpret_cast<const float4*>(a.T + b.o10 + q);
-                    v01[u] = *reinterpret_cast<const float4*>(a.T + b.o01 + q);
-                    v11[u] = *reinterpret_cast<const float4*>(a.T + b.o11 + q);
-                }
-#pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    const int q = q0 + 256 * u;
-                    if (q < n) {
-                        const int e1 = (int)__umulhi((unsigned)q, magic);
-                        const int e2 = 4 * (q - e1 * d);
-                        store_t<MT, MIX>(sh, e1, e2, bilin(b, v00[u].x, v10[u].x, v01[u].x, v11[u].x));
-                        store_t<MT, MIX>(sh, e1, e2 + 1, bilin(b, v00[u].y, v10[u].y, v01[u].y, v11[u].y));
-                        store_t<MT, MIX>(sh, e1, e2 + 2, bilin(b, v00[u].z, v10[u].z, v01[u].z, v11[u].z));
-                        store_t<MT, MIX>(sh, e1, e2 + 3, bilin(b, v00[u].w, v10[u].w, v01[u].w, v11[u].w));
-                    }
-                }
-            }
-        } else {
-            constexpr int UB = 8;
-            const unsigned magic = 0xffffffffu / (unsigned)m + 1u;
-            const int n = md * m;
-            for (int q0 = tid; q0 < n; q0 += 256 * UB) {
-                float v00[UB], v10[UB], v01[UB], v11[UB];
-#pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    const int q = min(q0 + 256 * u, n - 1);   // in range; duplicates are not stored
-                    v00[u] = a.T[b.o00 + q];
-                    v10[u] = a.T[b.o10 + q];
-                    v01[u] = a.T[b.o01 + q];
-                    v11[u] = a.T[b.o11 + q];
-                }
-#pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    const int q = q0 + 256 * u;
-                    if (q < n) {
-                        const int e1 = (int)__umulhi((unsigned)q, magic), e2 = q - e1 * m;
-                        store_t<MT, MIX>(sh, e1, e2, bilin(b, v00[u], v10[u], v01[u], v11[u]));
-                    }
-                }
-            }
+        const Bilin b = make_bilin(a, j, i, (long long)md * m);
+        const int ca = (b.jcopy ? 0 : 1) | (b.icopy ? 0 : 2);
+        if (ca == 0)
+            form_t<KT, OT, MIX, VEC, 0>(a, b, sh, tid);
+        else if (ca == 1)
+            form_t<KT, OT, MIX, VEC, 1>(a, b, sh, tid);
+        else if (ca == 2)
+            form_t<KT, OT, MIX, VEC, 2>(a, b, sh, tid);
+        else
+            form_t<KT, OT, MIX, VEC, 3>(a, b, sh, tid);
+        // zero padding: members e2 in [m, NK(H)) of every output row, and output
+        // rows e1 in [md, NO) of every member
+        constexpr int NKX = MIX ? G::NKH : G::NK;
+        if (m < NKX) {
+            const int np = NKX - m;
+            for (int q = tid; q < G::NO * np; q += 64 * kWaves)
+                store_t<KT, OT, MIX>(sh, q / np, m + q % np, 0.0f);
         }
-        // zero padding of the operand: e1 in [md, MP), e2 in [m, KP)
-        for (int q = tid; q < MP * KP; q += 256) {
-            const int e1 = q / KP, e2 = q - e1 * KP;
-            if (e1 >= md || e2 >= m)
-                store_t<MT, MIX>(sh, e1, e2, 0.0f);
+        if (md < G::NO) {
+            const int nr = G::NO - md;
+            for (int q = tid; q < nr * m; q += 64 * kWaves)
+                store_t<KT, OT, MIX>(sh, md + q / m, q % m, 0.0f);
         }
         const Bilin bw = make_bilin(a, j, i, m);
-        if (tid < KP) {
-            const int e = min(tid, m - 1);
-            const float v = bilin(bw, a.w[bw.o00 + e], a.w[bw.o10 + e], a.w[bw.o01 + e], a.w[bw.o11 + e]);
-            sh.wc[tid] = (tid < m) ? v : 0.0f;
+        for (int e = tid; e < NKX; e += 64 * kWaves) {
+            const int ec = min(e, m - 1);
+            const float v = bilin(bw, a.w[bw.o00 + ec], a.w[bw.o10 + ec], a.w[bw.o01 + ec], a.w[bw.o11 + ec]);
+            sh.wc[e] = (e < m) ? v : 0.0f;
         }
     }
+    __syncthreads();
 
-    for (int f0 = 0; f0 < a.nfield; f0 += FG) {
-        // per-field parameters of this group, loaded ahead of the next
-        // group's records (so that waiting for them does not wait for those)
-        const int fst = min(f0 + 16 * w + (lane & 15), a.nfield - 1);   // statistics field of this lane
-        const int lev_s = a.flevel[fst], surf_s = a.fsurf[fst];
-        float infl0_r[4];
-        double ratio_r[4];
-        int log_r[4];
+    for (int t = w; t < ntiles; t += kWaves) {
+        if (t != w)
+            load_tile<KT, VEC>(a, cell, t, col, kq, x);
+
+        // ---- 2. statistics of field 16 t + col over the lane's members
+        const int fs = 16 * t + col;
+        const bool live = fs < a.nfield;
+        const int fsc = min(fs, a.nfield - 1);
+        // a member is below m / m_d when its block of 16 is (a uniform test),
+        // or by a per-lane test in the block that m / m_d cuts
+        bool badd = false, bads = false;
+        double p = 0.0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int fo = min(f0 + 16 * w + 4 * (lane >> 4) + r, a.nfield - 1);   // output fields of this lane
-            infl0_r[r] = a.finfl[fo];
-            ratio_r[r] = a.finfratio[fo];
-            log_r[r] = a.fapplylog[fo];
-        }
-        __syncthreads();   // T is in LDS; the previous group's reads are done
-        // ---- 2a. the group's records (loaded earlier) into LDS
+        for (int s = 0; s < KT; ++s)
 #pragma unroll
-        for (int u = 0; u < NL; ++u) {
-            const int q = tid + 256 * u;
-            const int f = q / MP, e = q - f * MP;
-            if (f < FG)
-                sh.at[e][f] = (e < m) ? vnext[u] : 0.0f;
-        }
-        for (int q = tid; q < (KP - MP) * FG; q += 256)
-            sh.at[MP + q / FG][q % FG] = 0.0f;
-        if (f0 + FG < a.nfield)
-            load_group<MT, NL>(a, cell, f0 + FG, vnext);
-        __syncthreads();
-        // ---- 2b. per-field statistics: lane (f, g) covers members
-        // e = g (mod 4); the 4 lanes of a field are lanes f, f + 16, f + 32,
-        // f + 48 of wave f / 16 (and read 4 consecutive rows of at[][], which
-        // fall in disjoint banks)
-        {
-            const int fl = 16 * w + (lane & 15);
-            const int g = lane >> 4;
-            const int fgl = f0 + fl;
-            const bool live = fgl < a.nfield;
-            // first bad member and the FP64 sum of the dynamic members in one
-            // pass (the sum is used only when no dynamic member is bad)
-            int eb = m;
-            double p = 0.0;
-            for (int e = g; e < m; e += 4) {
-                const float v = sh.at[e][fl];
-                if (eb == m && (!isfinite(v) || fabsf(v) > kMaxVal))
-                    eb = e;
-                if (e < md)
-                    p += (double)v;
+            for (int c = 0; c < 4; ++c) {
+                const int e = 16 * s + 4 * kq + c;
+                const bool inm = (16 * (s + 1) <= m) || e < m;
+                const bool inmd = (s < OT) && ((16 * (s + 1) <= md) || e < md);
+                const float v = x[s][c];
+                const bool bad = !(fabsf(v) <= kMaxVal);   // not finite, or beyond MAXOBSVAL
+                if (s < OT)
+                    badd = badd || (bad && inmd);
+                if (s >= OT - 1)
+                    bads = bads || (bad && inm && !inmd);
+                if (s < OT)
+                    p += inmd ? (double)v : 0.0;
             }
-            eb = min(eb, __shfl_xor(eb, 16));
-            eb = min(eb, __shfl_xor(eb, 32));
-            int z = 0;
-            if (!live || is_masked(nl, lev_s, surf_s, a.surfk))
-                z = 1;
-            else if (eb < md)
-                z = 1;
-            else if (eb < m)
-                z = 2;
-            // FP64 mean of the dynamic members (partials combined in lane order)
-            double s0 = __shfl(p, lane & 15), s1 = __shfl(p, (lane & 15) + 16);
-            double s2 = __shfl(p, (lane & 15) + 32), s3 = __shfl(p, (lane & 15) + 48);
-            const double v1_f = (((s0 + s1) + s2) + s3) / (double)md;
-            double pa = 0.0, pf = 0.0;
-            float amax = 0.0f;
-            for (int e = g; e < m; e += 4) {
-                const float raw = (z == 2 && e >= md) ? 0.0f : sh.at[e][fl];
-                const float vf = (z == 1) ? 0.0f : (float)((double)raw - v1_f);
-                sh.at[e][fl] = vf;
-                pa += (double)(vf * sh.wc[e]);
-                if (e < md) {
-                    const float vh = hybrid ? (float)((double)vf / k_d) : vf;
-                    pf += (double)(vh * vh);
+        const bool bad_d = any4(badd, col), bad_s = any4(bads, col);
+        int z = 0;   // 0 update, 1 all members 0, 2 static members 0
+        if (!live || bad_d || is_masked(nl, a.flevel[fsc], a.fsurf[fsc], a.surfk))
+            z = 1;
+        else if (bad_s)
+            z = 2;
+        const bool z2 = (z == 2);
+        const double v1_f = sum4(p, col) / (double)md;
+        // anomalies (a field with z = 1 is written as zeros whatever they are;
+        // its row of the product involves no other field)
+        double pa = 0.0, pf = 0.0;
+        float amax = 0.0f;
+#pragma unroll
+        for (int s = 0; s < KT; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int e = 16 * s + 4 * kq + c;
+                const bool inm = (16 * (s + 1) <= m) || e < m;
+                const bool inmd = (s < OT) && ((16 * (s + 1) <= md) || e < md);
+                // a static member of a field with a bad static member is 0
+                // before the mean is taken off (update.c:337-371)
+                float xin = x[s][c];
+                if (s >= OT - 1)
+                    xin = (z2 && !inmd) ? 0.0f : xin;
+                float vf = (float)((double)xin - v1_f);
+                if (s == KT - 1)
+                    vf = inm ? vf : 0.0f;
+                x[s][c] = vf;
+                pa += (double)(vf * sh.wc[e]);   // wc is 0 past m
+                if (s < OT) {
+                    const float vh = hybrid ? (float)((double)vf * inv_kd) : vf;
+                    pf += inmd ? (double)(vh * vh) : 0.0;
                 }
-                amax = fmaxf(amax, fabsf(vf));
+                if (MIX)
+                    amax = fmaxf(amax, fabsf(vf));
             }
-            s0 = __shfl(pa, lane & 15);
-            s1 = __shfl(pa, (lane & 15) + 16);
-            s2 = __shfl(pa, (lane & 15) + 32);
-            s3 = __shfl(pa, (lane & 15) + 48);
-            const double v1_a = (((s0 + s1) + s2) + s3) + v1_f;
-            s0 = __shfl(pf, lane & 15);
-            s1 = __shfl(pf, (lane & 15) + 16);
-            s2 = __shfl(pf, (lane & 15) + 32);
-            s3 = __shfl(pf, (lane & 15) + 48);
-            const double v2_f = ((s0 + s1) + s2) + s3;
+        const double v1_a = sum4(pa, col) + v1_f;
+        const double v2_f = sum4(pf, col);
+        float scale = 1.0f;
+        if constexpr (MIX) {
             amax = fmaxf(amax, __shfl_xor(amax, 16));
             amax = fmaxf(amax, __shfl_xor(amax, 32));
-            if (g == 0) {
-                sh.zero[fl] = z;
-                sh.v1f[fl] = v1_f;
-                sh.v1a[fl] = v1_a;
-                sh.v2f[fl] = v2_f;
-                // power of two that brings the largest anomaly to [2^9, 2^10)
-                // (FP16 normal range for the split; exact scaling)
-                sh.scale[fl] = (amax > 0.0f && isfinite(amax)) ? ldexpf(1.0f, max(-100, min(100, 9 - ilogbf(amax))))
-                                                               : 1.0f;
-            }
+            // power of two that brings the largest anomaly to [2^9, 2^10) (FP16
+            // normal range for the split; exact scaling)
+            if (amax > 0.0f && isfinite(amax))
+                scale = ldexpf(1.0f, max(-100, min(100, 9 - ilogbf(amax))));
         }
-        __syncthreads();
-        // ---- 3. V^a (16 fields of wave w) x (MT member tiles)
-        flt4 acc[MT];
+
+        // ---- 3. V^a for the tile's 16 fields x 16 OT members; the LDS
+        // offset of the lane's B values is opaque to the compiler, so that
+        // T_cell is read per tile rather than held in registers
+        flt4 acc[OT];
 #pragma unroll
-        for (int J = 0; J < MT; ++J)
+        for (int J = 0; J < OT; ++J)
             acc[J] = flt4{0.0f, 0.0f, 0.0f, 0.0f};
-        const int col = lane & 15, kq = lane >> 4;
-        const int fa = 16 * w + col;
+        int boff = col;
+        asm volatile("" : "+v"(boff));
         if constexpr (!MIX) {
-#pragma unroll 4
-            for (int s = 0; s < KP / 4; ++s) {
-                const int k = 4 * s + kq;
-                const float av = sh.at[k][fa];
 #pragma unroll
-                for (int J = 0; J < MT; ++J)
-                    acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sh.tt[k][16 * J + (col ^ (s & 15))], acc[J], 0,
-                                                                  0, 0);
-            }
+            for (int s = 0; s < KT; ++s)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float* brow = &sh.tk[16 * s + 4 * c + kq][0] + boff;
+#pragma unroll
+                    for (int J = 0; J < OT; ++J)
+                        acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s][c], brow[16 * J], acc[J], 0, 0, 0);
+                }
         } else {
-            const float sc = sh.scale[fa];
 #pragma unroll
-            for (int s = 0; s < KP / 32; ++s) {
-                const int k0 = 32 * s + 8 * kq;
+            for (int tt = 0; tt < (KT + 1) / 2; ++tt) {
                 half8 ah, al;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const float x = sh.at[k0 + q][fa] * sc;
-                    const _Float16 h = (_Float16)x;
-                    ah[q] = h;
-                    al[q] = (_Float16)(x - (float)h);
-                }
+                for (int pp = 0; pp < 2; ++pp)
 #pragma unroll
-                for (int J = 0; J < MT; ++J) {
-                    const half8 bh = *reinterpret_cast<const half8*>(&sh.th[16 * J + col][k0]);
-                    const half8 bl = *reinterpret_cast<const half8*>(&sh.tl[16 * J + col][k0]);
+                    for (int c = 0; c < 4; ++c) {
+                        const int s = 2 * tt + pp;
+                        const float xs = (s < KT) ? x[s < KT ? s : 0][c] * scale : 0.0f;
+                        const _Float16 h = (_Float16)xs;
+                        ah[4 * pp + c] = h;
+                        al[4 * pp + c] = (_Float16)(xs - (float)h);
+                    }
+#pragma unroll
+                for (int J = 0; J < OT; ++J) {
+                    const half8 bh = *reinterpret_cast<const half8*>(&sh.th[16 * J][32 * tt + 8 * kq] + boff * G::HS);
+                    const half8 bl = *reinterpret_cast<const half8*>(&sh.tl[16 * J][32 * tt + 8 * kq] + boff * G::HS);
                     acc[J] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[J], 0, 0, 0);
                     acc[J] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[J], 0, 0, 0);
                     acc[J] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[J], 0, 0, 0);
                 }
             }
         }
-        // ---- 4. result element (field 16w + 4 kq + r, member 16 J + col) in
-        // acc[J][r]; inflation per field from a 16-lane reduction
+
+        // ---- 4. result (field 16 t + 4 kq + r, member 16 J + col) in
+        // acc[J][r]; the field's statistics from its stats lane (4 kq + r)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int fl = 16 * w + 4 * kq + r;
-            const int fgl = f0 + fl;
-            const int z = sh.zero[fl];
-            const float unscale = MIX ? 1.0f / sh.scale[fl] : 1.0f;
+            const int fl = 4 * kq + r;
+            const int fo = 16 * t + fl;
+            const int fz = __shfl(z, fl);
+            const double f1f = __shfl(v1_f, fl), f1a = __shfl(v1_a, fl), f2f = __shfl(v2_f, fl);
+            const float unscale = MIX ? 1.0f / __shfl(scale, fl) : 1.0f;
+            float va[OT];
             double v2a = 0.0;
 #pragma unroll
-            for (int J = 0; J < MT; ++J) {
+            for (int J = 0; J < OT; ++J) {
                 const int e = 16 * J + col;
-                float va = acc[J][r] * unscale;
+                float v = acc[J][r];
+                if (MIX)
+                    v *= unscale;
                 if (hybrid)
-                    va = (float)((double)va / k_d);
-                acc[J][r] = va;
-                if (e < md)
-                    v2a += (double)(va * va);
+                    v = (float)((double)v * inv_kd);
+                va[J] = v;
+                const bool in = (16 * (J + 1) <= md) || e < md;
+                v2a += in ? (double)(v * v) : 0.0;
             }
             v2a = row16_sum(v2a);
-            if (fgl >= a.nfield)
+            if (fo >= a.nfield)
                 continue;
-            const float inflation0 = infl0_r[r];
-            const double inf_ratio = ratio_r[r];
+            const float inflation0 = a.finfl[fo];
+            const double inf_ratio = a.finfratio[fo];
             float inflation = inflation0;
             if (!isnan(inf_ratio) && v2a > 0.0) {
-                inflation = (float)(sqrt(sh.v2f[fl] / v2a) * inf_ratio + 1.0 - inf_ratio);
+                inflation = (float)(sqrt(f2f / v2a) * inf_ratio + 1.0 - inf_ratio);
                 if (inflation >= inflation0)
                     inflation = inflation0;
             }
             if (!(inflation - 1.0f > kEps))
                 inflation = 1.0f;
-            const double v1a = sh.v1a[fl], v1f = sh.v1f[fl];
-            float* rec = a.E + ((long long)fgl * a.ncell + cell) * m;
-            const bool outinc = a.outputinc != 0;
-            const bool applylog = log_r[r] != 0;
-#pragma unroll
-            for (int J = 0; J < MT; ++J) {
-                const int e = 16 * J + col;
-                if (e >= m)
-                    continue;
-                if (z == 1) {   // masked or a dynamic member missing: all members 0
+            float* rec = a.E + ((long long)fo * a.ncell + cell) * m;
+            if (fz == 1) {   // masked or a dynamic member missing: all members 0
+                for (int e = col; e < m; e += 16)
                     rec[e] = 0.0f;
-                    continue;
-                }
-                if (e >= md) {
-                    if (z == 2)
+            } else {
+                if (fz == 2)   // a static member missing: the static members 0
+                    for (int e = md + col; e < m; e += 16)
                         rec[e] = 0.0f;
-                    continue;
+                const bool outinc = a.outputinc != 0;
+                const bool applylog = outinc && a.fapplylog[fo] != 0;
+#pragma unroll
+                for (int J = 0; J < OT; ++J) {
+                    const int e = 16 * J + col;
+                    if (16 * (J + 1) <= md || e < md) {
+                        float v = va[J];
+                        if (inflation != 1.0f)
+                            v = v * inflation;
+                        v = (float)((double)v + f1a);
+                        if (outinc) {
+                            float vf = (float)((double)rec[e] - f1f);
+                            if (hybrid)
+                                vf = (float)((double)vf * inv_kd);
+                            vf = (float)((double)vf + f1f);
+                            v = applylog ? log_increment(v, vf) : v - vf;
+                        }
+                        rec[e] = v;
+                    }
                 }
-                float va = acc[J][r];
-                if (inflation != 1.0f)
-                    va = va * inflation;
-                va = (float)((double)va + v1a);
-                float out = va;
-                if (outinc) {
-                    float vf = sh.at[e][fl];
-                    if (hybrid)
-                        vf = (float)((double)vf / k_d);
-                    vf = (float)((double)vf + v1f);
-                    out = applylog ? (isnormal(vf) ? (float)(exp10((double)va) - exp10((double)vf)) : 0.0f)
-                                   : va - vf;
-                }
-                rec[e] = out;
             }
             if (a.infl != nullptr && col == 0)
-                a.infl[(long long)fgl * a.ncell + cell] = (z == 1) ? 0.0f : inflation;
+                a.infl[(long long)fo * a.ncell + cell] = (fz == 1) ? 0.0f : inflation;
         }
     }
+}
+
+template <int KT, int OT>
+hipError_t launch_kt(hipStream_t s, const UpdArgs& a, bool mixed, dim3 grid)
+{
+    const bool vec = (a.m % 4 == 0);
+    const dim3 block(64 * kWaves);
+    if (mixed) {
+        if (vec)
+            hipLaunchKernelGGL((update_tile_kernel<KT, OT, true, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((update_tile_kernel<KT, OT, true, false>), grid, block, 0, s, a);
+    } else {
+        if (vec)
+            hipLaunchKernelGGL((update_tile_kernel<KT, OT, false, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((update_tile_kernel<KT, OT, false, false>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -443,34 +579,31 @@ hipError_t launch_update_mfma(hipStream_t s, const UpdArgs& a, bool mixed)
 {
     if (a.ncell == 0 || a.nfield == 0)
         return hipSuccess;
-    const int MT = (a.m + 15) / 16;
+    if (a.m < 1 || a.m > 160 || a.md < 1 || a.md > a.m)
+        return hipErrorInvalidValue;
+    const int KT = (a.m + 15) / 16, OT = (a.md + 15) / 16;
     const long long chunk = (a.ncell + 7) / 8;
     if (8 * chunk > 0x7fffffffLL)
         return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(8 * chunk)), block(256);
-#define ENKFG_UPD_CASE(N)                                                                      \
-    case N:                                                                                    \
-        if (mixed)                                                                             \
-            hipLaunchKernelGGL((update_mfma_kernel<N, true>), grid, block, 0, s, a);           \
-        else                                                                                   \
-            hipLaunchKernelGGL((update_mfma_kernel<N, false>), grid, block, 0, s, a);          \
-        break;
-    switch (MT) {
-        ENKFG_UPD_CASE(1)
-        ENKFG_UPD_CASE(2)
-        ENKFG_UPD_CASE(3)
-        ENKFG_UPD_CASE(4)
-        ENKFG_UPD_CASE(5)
-        ENKFG_UPD_CASE(6)
-        ENKFG_UPD_CASE(7)
-        ENKFG_UPD_CASE(8)
-        ENKFG_UPD_CASE(9)
-        ENKFG_UPD_CASE(10)
-    default:
-        return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(8 * chunk));
+    // the output tiles cover m_d: one kernel per KT with OT = KT, and the
+    // hybrid 96 + 48 shape; any other hybrid shape computes the zero rows of
+    // T past m_d in the OT = KT kernel
+    if (KT == 9 && OT == 6)
+        return launch_kt<9, 6>(s, a, mixed, grid);
+    switch (KT) {
+    case 1: return launch_kt<1, 1>(s, a, mixed, grid);
+    case 2: return launch_kt<2, 2>(s, a, mixed, grid);
+    case 3: return launch_kt<3, 3>(s, a, mixed, grid);
+    case 4: return launch_kt<4, 4>(s, a, mixed, grid);
+    case 5: return launch_kt<5, 5>(s, a, mixed, grid);
+    case 6: return launch_kt<6, 6>(s, a, mixed, grid);
+    case 7: return launch_kt<7, 7>(s, a, mixed, grid);
+    case 8: return launch_kt<8, 8>(s, a, mixed, grid);
+    case 9: return launch_kt<9, 9>(s, a, mixed, grid);
+    case 10: return launch_kt<10, 10>(s, a, mixed, grid);
+    default: return hipErrorInvalidValue;
     }
-#undef ENKFG_UPD_CASE
-    return hipGetLastError();
 }
 
 }  // namespace enkfg
