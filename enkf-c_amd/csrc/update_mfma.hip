@@ -63,7 +63,10 @@ struct TileGeom {
 template <int KT, int OT, bool MIX>
 struct TileLds {
     using G = TileGeom<KT, OT, MIX>;
-    // FP32: tk[kpos32(e2)][e1] = T_cell[e1][e2]
+    // FP32: tk[k][e1 ^ (k / 16)] = T_cell[e1][e2], k = kpos32(e2) (the column
+    // swizzle within blocks of 16 spreads the rows that the formation's
+    // stores touch together over the banks; a B-operand read still takes 16
+    // consecutive columns)
     alignas(16) float tk[MIX ? 1 : G::NK][MIX ? 1 : G::LD];
     // mixed: th/tl[e1][kpos16(e2)] = hi/lo(T_cell[e1][e2])
     alignas(16) _Float16 th[MIX ? G::NO : 1][MIX ? G::HS : 8];
@@ -172,7 +175,8 @@ __device__ inline void store_t(TileLds<KT, OT, MIX>& sh, int e1, int e2, float t
         sh.th[e1][kpos16(e2)] = h;
         sh.tl[e1][kpos16(e2)] = (_Float16)(t - (float)h);
     } else {
-        sh.tk[kpos32(e2)][e1] = t;
+        const int k = kpos32(e2);
+        sh.tk[k][e1 ^ (k >> 4)] = t;
     }
 }
 
@@ -191,10 +195,10 @@ __device__ inline void store_t4(TileLds<KT, OT, MIX>& sh, int e1, int e2, const 
         *reinterpret_cast<half4*>(&sh.th[e1][k]) = h;
         *reinterpret_cast<half4*>(&sh.tl[e1][k]) = l;
     } else {
-        const int k = kpos32(e2);   // positions k, k + 4, k + 8, k + 12
+        const int k = kpos32(e2);   // positions k, k + 4, k + 8, k + 12 (one block of 16)
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            sh.tk[k + 4 * c][e1] = t[c];
+            sh.tk[k + 4 * c][e1 ^ (k >> 4)] = t[c];
     }
 }
 
@@ -297,7 +301,10 @@ constexpr int tile_waves()
     return KT <= 6 ? 4 : (KT <= 8 ? 3 : 2);
 }
 
-template <int KT, int OT, bool MIX, bool VEC>
+// FULL: m == 16 KT and m_d == 16 OT, so that every member test is a
+// compile-time constant (held per member, the tests would take two scalar
+// registers each)
+template <int KT, int OT, bool MIX, bool VEC, bool FULL>
 __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(tile_waves<KT>())))
 update_tile_kernel(UpdArgs a)
 {
@@ -315,7 +322,9 @@ update_tile_kernel(UpdArgs a)
     const int i = (int)(cell % a.mni);
     const int j = a.j1 + jl;
     const int m = a.m, md = a.md;
-    const bool hybrid = (a.mode == ENKFG_MODE_HYBRID);
+    // with m_d == m (FULL, OT == KT) the hybrid factor k_d is 1 and its
+    // scaling the identity: left out at compile time
+    const bool hybrid = (FULL && OT == KT) ? false : (a.mode == ENKFG_MODE_HYBRID);
     const double k_d = (md > 1) ? sqrt((double)(m - 1) / (double)(md - 1)) : 1.0;
     // the hybrid rescaling v /= k_d (update.c:420-427) as a product with the
     // reciprocal: the FP64 results differ by at most an ulp of FP64, so the
@@ -371,8 +380,9 @@ update_tile_kernel(UpdArgs a)
         const int fs = 16 * t + col;
         const bool live = fs < a.nfield;
         const int fsc = min(fs, a.nfield - 1);
-        // a member is below m / m_d when its block of 16 is (a uniform test),
-        // or by a per-lane test in the block that m / m_d cuts
+        // a member is below m / m_d when its block of 16 is (a uniform test;
+        // always, for FULL), or by a per-lane test in the block that m / m_d
+        // cuts
         bool badd = false, bads = false;
         double p = 0.0;
 #pragma unroll
@@ -380,8 +390,8 @@ update_tile_kernel(UpdArgs a)
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int e = 16 * s + 4 * kq + c;
-                const bool inm = (16 * (s + 1) <= m) || e < m;
-                const bool inmd = (s < OT) && ((16 * (s + 1) <= md) || e < md);
+                const bool inm = FULL || (16 * (s + 1) <= m) || e < m;
+                const bool inmd = (s < OT) && (FULL || (16 * (s + 1) <= md) || e < md);
                 const float v = x[s][c];
                 const bool bad = !(fabsf(v) <= kMaxVal);   // not finite, or beyond MAXOBSVAL
                 if (s < OT)
@@ -408,8 +418,8 @@ update_tile_kernel(UpdArgs a)
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int e = 16 * s + 4 * kq + c;
-                const bool inm = (16 * (s + 1) <= m) || e < m;
-                const bool inmd = (s < OT) && ((16 * (s + 1) <= md) || e < md);
+                const bool inm = FULL || (16 * (s + 1) <= m) || e < m;
+                const bool inmd = (s < OT) && (FULL || (16 * (s + 1) <= md) || e < md);
                 // a static member of a field with a bad static member is 0
                 // before the mean is taken off (update.c:337-371)
                 float xin = x[s][c];
@@ -453,7 +463,7 @@ update_tile_kernel(UpdArgs a)
             for (int s = 0; s < KT; ++s)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const float* brow = &sh.tk[16 * s + 4 * c + kq][0] + boff;
+                    const float* brow = &sh.tk[16 * s + 4 * c + kq][0] + (boff ^ s);
 #pragma unroll
                     for (int J = 0; J < OT; ++J)
                         acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s][c], brow[16 * J], acc[J], 0, 0, 0);
@@ -503,7 +513,7 @@ update_tile_kernel(UpdArgs a)
                 if (hybrid)
                     v = (float)((double)v * inv_kd);
                 va[J] = v;
-                const bool in = (16 * (J + 1) <= md) || e < md;
+                const bool in = FULL || (16 * (J + 1) <= md) || e < md;
                 v2a += in ? (double)(v * v) : 0.0;
             }
             v2a = row16_sum(v2a);
@@ -532,7 +542,7 @@ update_tile_kernel(UpdArgs a)
 #pragma unroll
                 for (int J = 0; J < OT; ++J) {
                     const int e = 16 * J + col;
-                    if (16 * (J + 1) <= md || e < md) {
+                    if (FULL || 16 * (J + 1) <= md || e < md) {
                         float v = va[J];
                         if (inflation != 1.0f)
                             v = v * inflation;
@@ -554,22 +564,25 @@ update_tile_kernel(UpdArgs a)
     }
 }
 
+template <int KT, int OT, bool MIX>
+void launch_ktm(hipStream_t s, const UpdArgs& a, dim3 grid)
+{
+    const dim3 block(64 * kWaves);
+    if (a.m == 16 * KT && a.md == 16 * OT)
+        hipLaunchKernelGGL((update_tile_kernel<KT, OT, MIX, true, true>), grid, block, 0, s, a);
+    else if (a.m % 4 == 0)
+        hipLaunchKernelGGL((update_tile_kernel<KT, OT, MIX, true, false>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((update_tile_kernel<KT, OT, MIX, false, false>), grid, block, 0, s, a);
+}
+
 template <int KT, int OT>
 hipError_t launch_kt(hipStream_t s, const UpdArgs& a, bool mixed, dim3 grid)
 {
-    const bool vec = (a.m % 4 == 0);
-    const dim3 block(64 * kWaves);
-    if (mixed) {
-        if (vec)
-            hipLaunchKernelGGL((update_tile_kernel<KT, OT, true, true>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((update_tile_kernel<KT, OT, true, false>), grid, block, 0, s, a);
-    } else {
-        if (vec)
-            hipLaunchKernelGGL((update_tile_kernel<KT, OT, false, true>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((update_tile_kernel<KT, OT, false, false>), grid, block, 0, s, a);
-    }
+    if (mixed)
+        launch_ktm<KT, OT, true>(s, a, grid);
+    else
+        launch_ktm<KT, OT, false>(s, a, grid);
     return hipGetLastError();
 }
 
