@@ -160,7 +160,7 @@ struct enkfg_ctx {
     // ensemble observations: H operator tables, obs records, spread
     DevBuf d_hg_tab, d_hg_obs, d_hg_std;
     // adjoint
-    DevBuf d_adjg, d_adjflag, d_adjlist, d_adjz, d_adjw, d_gy, d_gR;
+    DevBuf d_adjg, d_adjflag, d_adjlist, d_adjz, d_adjw, d_gy, d_gR, d_adjD, d_adjinfl, d_minv, d_adjB;
     float* attached_E = nullptr;
     enkfg_grid att_grid{};
     int att_nfield = 0;
@@ -563,7 +563,7 @@ void enkfg_destroy(enkfg_ctx* c)
                       &c->d_flevel, &c->d_fsurf, &c->d_flog, &c->d_finfl, &c->d_fratio, &c->d_adjg, &c->d_adjflag,
                       &c->d_adjlist, &c->d_adjz, &c->d_adjw, &c->d_gy, &c->d_gR, &c->d_etkf, &c->d_noise,
                       &c->d_he_order, &c->d_he_gcell, &c->d_he_goff, &c->d_he_infl, &c->d_he_ratio, &c->d_hg_tab,
-                      &c->d_hg_obs, &c->d_hg_std, &c->d_spread};
+                      &c->d_hg_obs, &c->d_hg_std, &c->d_spread, &c->d_adjD, &c->d_adjinfl, &c->d_minv, &c->d_adjB};
     for (DevBuf* b : bufs)
         b->release();
     for (auto& e : c->ev)
@@ -2083,7 +2083,7 @@ int enkfg_adjoint(enkfg_ctx* c, const float* dev_dJ_dmean, double* dJ_dy, double
     HIPCHECK(c, launch_adj_cells(c->stream, ua, dev_dJ_dmean, c->d_adjg.as<double>()));
     // nodes with a non-zero g
     HIPCHECK(c, c->d_adjflag.reserve((size_t)nn * sizeof(int)));
-    HIPCHECK(c, launch_nonzero_rows(c->stream, c->d_adjg.as<double>(), nn, m, c->d_adjflag.as<int>()));
+    HIPCHECK(c, launch_nonzero_rows(c->stream, c->d_adjg.as<double>(), nn, m, c->d_adjflag.as<int>(), false));
     std::vector<int> flag(nn);
     HIPCHECK(c, hipMemcpyAsync(flag.data(), c->d_adjflag.p, (size_t)nn * sizeof(int), hipMemcpyDeviceToHost,
                                c->stream));
@@ -2141,10 +2141,9 @@ int enkfg_adjoint(enkfg_ctx* c, const float* dev_dJ_dmean, double* dJ_dy, double
             a.nbatch = std::min(nbatch, nlist - k0);
             HIPCHECK(c, launch_gram(c->stream, a));
             HIPCHECK(c, launch_factor_wave(c->stream, a));
+            HIPCHECK(c, launch_adj_obs(c->stream, a, c->d_perm.as<int>(), c->d_type.as<short>(), c->d_estd.as<float>(),
+                                       c->d_sqrtrf.as<double>(), nullptr, c->d_gy.as<double>(), c->d_gR.as<double>()));
         }
-        HIPCHECK(c, launch_adj_obs(c->stream, a, nlist, c->d_perm.as<int>(), c->d_type.as<short>(),
-                                   c->d_estd.as<float>(), c->d_sqrtrf.as<double>(), c->d_gy.as<double>(),
-                                   c->d_gR.as<double>()));
         HIPCHECK(c, hipStreamSynchronize(c->stream));
         int err = 0;
         HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
@@ -2166,6 +2165,134 @@ int enkfg_adjoint(enkfg_ctx* c, const float* dev_dJ_dmean, double* dJ_dy, double
     }
     if (dJ_dinfl != nullptr)
         *dJ_dinfl = 0.0;   // the analysis mean does not depend on the inflation
+    return ENKFG_OK;
+}
+
+int enkfg_adjoint_ens(enkfg_ctx* c, const float* dev_dJ_dXa, double* dJ_dy, double* dJ_dR, double* dJ_dinfl)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (dev_dJ_dXa == nullptr)
+        return fail(c, ENKFG_ERR_ARG, "dJ/dXa is NULL");
+    if (!c->S_set || !c->lists_valid || !c->transforms_valid)
+        return fail(c, ENKFG_ERR_STATE, "the adjoint needs S, and enkfg_calc_transforms on the same grid");
+    if (c->attached_E == nullptr)
+        return fail(c, ENKFG_ERR_STATE, "the adjoint needs the forecast ensemble attached (enkfg_ens_attach)");
+    if (c->cfg.mode == ENKFG_MODE_ENOI)
+        return fail(c, ENKFG_ERR_ARG, "the analysis ensemble needs EnKF or hybrid mode");
+    if (c->cfg.scheme != ENKFG_SCHEME_DENKF)
+        return fail(c, ENKFG_ERR_UNSUPPORTED, "the adjoint of the analysis ensemble supports DEnKF only");
+    const int m = c->cfg.nmem, md = c->cfg.nmem_dynamic;
+    if (m > 96)
+        return fail(c, ENKFG_ERR_UNSUPPORTED, "the adjoint supports nmem <= 96 (got %d)", m);
+    const enkfg_grid& g = c->att_grid;
+    if (g.stride != c->stride || (g.mni - 1) / g.stride + 1 != c->ni_s || (g.mnj - 1) / g.stride + 1 != c->nj_s)
+        return fail(c, ENKFG_ERR_ARG, "the attached grid does not match the node grid of the last calc");
+    const int nn = c->nnode;
+    const int nt = (int)c->types.size();
+    const int nf = c->att_nfield;
+    const size_t dsz = (size_t)nn * md * m * sizeof(double);
+    HIPCHECK(c, c->d_adjg.reserve((size_t)nn * m * sizeof(double)));
+    HIPCHECK(c, c->d_adjD.reserve(dsz));
+    HIPCHECK(c, c->d_adjinfl.reserve((size_t)nf * sizeof(double)));
+    HIPCHECK(c, hipMemsetAsync(c->d_adjg.p, 0, (size_t)nn * m * sizeof(double), c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_adjD.p, 0, dsz, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_adjinfl.p, 0, (size_t)nf * sizeof(double), c->stream));
+    UpdArgs ua = upd_args(c, &g, nf, 0, g.mnj, 0);
+    ua.E = c->attached_E;
+    HIPCHECK(c, launch_adj_ens_cells(c->stream, ua, dev_dJ_dXa, c->d_adjg.as<double>(), c->d_adjD.as<double>(),
+                                     c->d_adjinfl.as<double>()));
+    // nodes with a non-zero g or D
+    HIPCHECK(c, c->d_adjflag.reserve((size_t)nn * sizeof(int)));
+    HIPCHECK(c, launch_nonzero_rows(c->stream, c->d_adjg.as<double>(), nn, m, c->d_adjflag.as<int>(), false));
+    HIPCHECK(c, launch_nonzero_rows(c->stream, c->d_adjD.as<double>(), nn, md * m, c->d_adjflag.as<int>(), true));
+    std::vector<int> flag(nn);
+    HIPCHECK(c, hipMemcpyAsync(flag.data(), c->d_adjflag.p, (size_t)nn * sizeof(int), hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<int> list;
+    for (int k = 0; k < nn; ++k)
+        if (flag[k])
+            list.push_back(k);
+    const int nlist = (int)list.size();
+    const size_t nob = (size_t)std::max(c->nobs, 1);
+    HIPCHECK(c, c->d_gy.reserve(nob * sizeof(double)));
+    HIPCHECK(c, c->d_gR.reserve(nob * sizeof(double)));
+    HIPCHECK(c, hipMemsetAsync(c->d_gy.p, 0, nob * sizeof(double), c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_gR.p, 0, nob * sizeof(double), c->stream));
+    if (nlist > 0) {
+        HIPCHECK(c, c->d_adjlist.reserve((size_t)nlist * sizeof(int)));
+        HIPCHECK(c, hipMemcpy(c->d_adjlist.p, list.data(), (size_t)nlist * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHECK(c, c->d_adjz.reserve((size_t)nlist * m * sizeof(double)));
+        HIPCHECK(c, c->d_adjw.reserve((size_t)nlist * m * sizeof(double)));
+        const int mp = 16 * ((m + 15) / 16);
+        const size_t per_node = (size_t)mp * mp * sizeof(double);
+        // Gram scratch (reused for D M^-1), M^-1 and B of a batch: 3 per node
+        int nbatch = (int)std::max<size_t>(1, std::min<size_t>(((size_t)2 << 30) / (3 * per_node), (size_t)nlist));
+        nbatch = std::min(nbatch, 65536);
+        HIPCHECK(c, c->d_gram.reserve((size_t)nbatch * per_node));
+        HIPCHECK(c, c->d_minv.reserve((size_t)nbatch * per_node));
+        HIPCHECK(c, c->d_adjB.reserve((size_t)nbatch * per_node));
+        HIPCHECK(c, c->d_sts.reserve((size_t)nbatch * mp * sizeof(double)));
+        int zero = 0;
+        HIPCHECK(c, hipMemcpy(c->d_err.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+        CalcArgs a{};
+        a.m = m;
+        a.md = md;
+        a.mode = c->cfg.mode;
+        a.scheme = c->cfg.scheme;
+        a.ntypes = nt;
+        a.pertype = 0;
+        a.nseg = 1;
+        a.alpha = c->cfg.alpha;
+        a.counts = c->d_counts.as<int>();
+        a.offsets = c->d_offsets.as<long long>();
+        a.ids = c->d_ids.as<int>();
+        a.lc = c->d_lc.as<double>();
+        a.S = c->d_S.as<float>();
+        a.sv = c->d_s.as<double>();
+        a.ms = (m + 3) & ~3;
+        a.nobs = c->nobs;
+        a.gram = c->d_gram.as<double>();
+        a.sts = c->d_sts.as<double>();
+        a.nnode = nn;
+        a.err = c->d_err.as<int>();
+        a.node_list = c->d_adjlist.as<int>();
+        a.adj_g = c->d_adjg.as<double>();
+        a.adj_z = c->d_adjz.as<double>();
+        a.adj_w = c->d_adjw.as<double>();
+        a.adj_minv = c->d_minv.as<double>();
+        // per batch: M recomputed in FP64 (z = M^-1 g, w, M^-1), B, then the obs
+        for (int k0 = 0; k0 < nlist; k0 += nbatch) {
+            a.node0 = k0;
+            a.nbatch = std::min(nbatch, nlist - k0);
+            HIPCHECK(c, launch_gram(c->stream, a));
+            HIPCHECK(c, launch_factor_wave(c->stream, a));
+            HIPCHECK(c, launch_adj_B(c->stream, a, c->d_adjD.as<double>(), c->d_adjB.as<double>()));
+            HIPCHECK(c, launch_adj_obs(c->stream, a, c->d_perm.as<int>(), c->d_type.as<short>(), c->d_estd.as<float>(),
+                                       c->d_sqrtrf.as<double>(), c->d_adjB.as<double>(), c->d_gy.as<double>(),
+                                       c->d_gR.as<double>()));
+        }
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        int err = 0;
+        HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
+        if (err != 0)
+            return fail(c, ENKFG_ERR_LINALG, "non-positive pivot in the factorisation of I + S^T S");
+    }
+    std::vector<double> gy((size_t)c->nobs), gR((size_t)c->nobs);
+    if (c->nobs > 0) {
+        HIPCHECK(c, hipMemcpy(gy.data(), c->d_gy.p, (size_t)c->nobs * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHECK(c, hipMemcpy(gR.data(), c->d_gR.p, (size_t)c->nobs * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int r = 0; r < c->nobs; ++r) {
+        const int o = c->perm[r];
+        if (dJ_dy != nullptr)
+            dJ_dy[o] = gy[r];
+        if (dJ_dR != nullptr)
+            dJ_dR[o] = gR[r];
+    }
+    if (dJ_dinfl != nullptr)
+        HIPCHECK(c, hipMemcpy(dJ_dinfl, c->d_adjinfl.p, (size_t)nf * sizeof(double), hipMemcpyDeviceToHost));
     return ENKFG_OK;
 }
 

@@ -119,6 +119,9 @@ struct CalcArgs {
     const double* adj_g;
     double* adj_z;
     double* adj_w;
+    // ... and, when set, M^-1 of each batch node ([bnode][mp][mp], mp = 16
+    // ceil(m / 16), zero outside m x m) for the adjoint through T
+    double* adj_minv;
     // Gram item kernels (m <= 96): tile groups (wavefronts) per segment, 2 or
     // 3; any other value takes the kernel's default (3 FP64, 2 mixed)
     int tile_groups;
@@ -206,11 +209,19 @@ hipError_t launch_he_clear_bad(hipStream_t s, const short* status, int nobs, int
 // adjoint of the analysis mean (adjoint.hip)
 // g_nodes[node][m] += sum over cells of beta(cell, node) sum_f dJ/dmean[f][cell] A^f[f][cell][:]
 hipError_t launch_adj_cells(hipStream_t s, const UpdArgs& a, const float* dJ_dmean, double* g_nodes);
-// flags[r] = 1 if row r of the nrow x width matrix has a non-zero entry
-hipError_t launch_nonzero_rows(hipStream_t s, const double* g, int nrow, int width, int* flags);
-// per listed node k (nodes[k]): dJ/dy and dJ/dR of its local obs, by device row
-hipError_t launch_adj_obs(hipStream_t s, const CalcArgs& a, int nlist, const int* perm, const short* type,
-                          const float* estd, const double* sqrt_rfactor, double* dJ_dy, double* dJ_dR);
+// adjoint of the analysis ensemble (adjoint.hip): per node, g_n (as above)
+// and D_n = dJ/dT_n ([node][md][m]); per field, dJ/dinflation0 (summed)
+hipError_t launch_adj_ens_cells(hipStream_t s, const UpdArgs& a, const float* dJ_dXa, double* g_nodes,
+                                double* D_nodes, double* g_infl);
+// flags[r] |= 1 if row r of the nrow x width matrix has a non-zero entry
+hipError_t launch_nonzero_rows(hipStream_t s, const double* g, int nrow, int width, int* flags, bool accumulate);
+// B_k = alpha / 2 M_k^-1 D_k M_k^-1 for the batch's nodes (k = node0 ..
+// node0 + nbatch - 1 of the list; M^-1 from adj_minv, D from D_nodes)
+hipError_t launch_adj_B(hipStream_t s, const CalcArgs& a, const double* D_nodes, double* B);
+// per listed node k of the batch (node_list[node0 + k]): dJ/dy and dJ/dR of
+// its local obs, by device row; with B, the term of T as well
+hipError_t launch_adj_obs(hipStream_t s, const CalcArgs& a, const int* perm, const short* type, const float* estd,
+                          const double* sqrt_rfactor, const double* B, double* dJ_dy, double* dJ_dR);
 hipError_t launch_update_bg(hipStream_t s, const UpdArgs& a);
 // addnoise (update.c:46-87) on one field E [ncell][m]:
 // v <- (float)((double)(v * defl) + (double)mult * r[e])

@@ -78,6 +78,9 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
             a.adj_z[(size_t)(a.node0 + bnode) * m + e] = 0.0;
             a.adj_w[(size_t)(a.node0 + bnode) * m + e] = 0.0;
         }
+        if (a.adj_minv != nullptr)   // no local obs: T = I whatever M (nothing depends on it)
+            for (int k = lane; k < MP * MP; k += 256)
+                a.adj_minv[(size_t)bnode * MP * MP + k] = 0.0;
         return;
     }
     if (!a.pertype && !adjoint) {
@@ -223,6 +226,16 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
                 a.adj_z[k * m + e] = yz;
                 a.adj_w[k * m + e] = yw;
             }
+        }
+        if (a.adj_minv != nullptr) {
+            double* mi = a.adj_minv + (size_t)bnode * MP * MP;
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+#pragma unroll
+                for (int c = 0; c < RC; ++c) {
+                    const int i = lr * RR + r, j = lc * RC + c;
+                    mi[(size_t)i * MP + j] = (i < m && j < m) ? x[r][c] : 0.0;
+                }
         }
         return;
     }

@@ -34,7 +34,7 @@ EXPORTED = [
     "enkfg_set_transforms", "enkfg_update_fields", "enkfg_update_bg", "enkfg_ens_attach", "enkfg_ens_update",
     "enkfg_get_timings", "enkfg_get_tuning", "enkfg_adjoint", "enkfg_set_rand48", "enkfg_get_rand48",
     "enkfg_rand48_normals", "enkfg_update_HE", "enkfg_update_fields_noalloc", "enkfg_calc_HE",
-    "enkfg_ens_spread", "enkfg_spread_fields",
+    "enkfg_ens_spread", "enkfg_spread_fields", "enkfg_adjoint_ens",
 ]
 
 
@@ -119,6 +119,7 @@ def lib():
         L.enkfg_get_timings.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
         L.enkfg_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
         L.enkfg_adjoint.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
+        L.enkfg_adjoint_ens.argtypes = [vp, vp, vp, vp, vp]
         L.enkfg_update_HE.argtypes = [vp, C.POINTER(Grid), C.c_int, vp, vp, vp, vp, vp, vp]
         L.enkfg_calc_HE.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Hop), vp, vp, vp, vp, vp, vp, vp]
         L.enkfg_update_fields_noalloc.argtypes = [vp, C.POINTER(Grid), C.c_int, C.POINTER(Field), C.c_int, C.c_int,
@@ -418,6 +419,17 @@ class Engine:
         self._check(lib().enkfg_update_HE(self.h, C.byref(g), nobs, _p(fi), _p(fj), _p(status), _p(inflation),
                                           _p(inf_ratio), _p(out)))
         return out
+
+    def adjoint_ens(self, dev_dJ_dXa, nobs, nfield):
+        """Gradients of J with respect to the observation values, the
+        observation-error variances and each field's inflation, given
+        dJ/d(analysis ensemble) as a device pointer to float
+        [nfield][ncell][nmem]; returns (dJ/dy, dJ/dR, dJ/dinfl[nfield])."""
+        gy = np.zeros(nobs, np.float64)
+        gR = np.zeros(nobs, np.float64)
+        gi = np.zeros(nfield, np.float64)
+        self._check(lib().enkfg_adjoint_ens(self.h, C.c_void_p(dev_dJ_dXa), _p(gy), _p(gR), _p(gi)))
+        return gy, gR, gi
 
     def update_bg(self, nlevels, stride, periodic_i, surfk, fields, buffers, j1, j2):
         g, keep = self._grid(nlevels, stride, periodic_i, surfk)

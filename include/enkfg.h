@@ -322,6 +322,27 @@ int enkfg_spread_fields(enkfg_ctx* ctx, int nfield, int nij, float* const* field
  * inverse in FP64 for the nodes that J depends on (none is stored); nmem <= 96. */
 int enkfg_adjoint(enkfg_ctx* ctx, const float* dev_dJ_dmean, double* dJ_dy, double* dJ_dR, double* dJ_dinfl);
 
+/* Adjoint of the analysis ensemble.  No counterpart in the reference; it
+ * differentiates the DEnKF forward path of the reference: M = I + S^T S and
+ * w = M^-1 S^T s per node (calc/calcs.c:189-263), T = 1/2 (I + M^-1) with the
+ * relaxation to prior of transforms.c:1270-1283, the stride-grid
+ * interpolation of T and w (update/update.c:195-272), the analysis
+ * v_a = A^f T^T, its hybrid rescaling and the capped inflation, and
+ * X^a = infl v_a + x^f + A^f w (update.c:381-458).
+ *   dev_dJ_dXa  device float [nfield][mnj * mni][nmem], the attached
+ *               ensemble's layout: dJ/dX^a of a scalar J for the dynamic
+ *               members (the static members of a hybrid ensemble are not
+ *               updated, and their entries are ignored)
+ *   dJ_dy, dJ_dR  host double [nobs], as enkfg_adjoint; either may be NULL
+ *   dJ_dinfl    host double [nfield]: dJ/d(inflation) of each attached field
+ *               (the fields' inflation values; cells where the ratio form is
+ *               below the cap, or no inflation is applied, add nothing)
+ * The rounding of the forward's FP32 arithmetic (interpolation, products,
+ * the inflation factor) is not differentiated.  Same preconditions as
+ * enkfg_adjoint; DEnKF only (ENKFG_ERR_UNSUPPORTED for ETKF), nmem <= 96.
+ * Memory: nnode * nmem_dynamic * nmem doubles for dJ/dT. */
+int enkfg_adjoint_ens(enkfg_ctx* ctx, const float* dev_dJ_dXa, double* dJ_dy, double* dJ_dR, double* dJ_dinfl);
+
 /* hipEvent-measured duration (ms) of the last calc / update kernels, per
  * phase: [0] localisation, [1] Gram, [2] factorisation, [3] update; [4] is
  * the number of Gram (and factor) launches of the last calc and [5] the
