@@ -437,3 +437,159 @@ def test_adjoint_ens_against_autograd(mode, m, md, alpha):
     assert ey <= 1e-4 and eR <= 1e-4 and ei <= 1e-4
     # the T path matters: the mean-only gradients differ
     assert np.abs(gR_ref).max() > 0 and gi_ref[1] != 0.0
+
+
+def test_adjoint_ens_c5_m96_against_autograd_and_finite_differences():
+    """enkfg_adjoint_ens at the C5 size: 32 layers x 600 x 600, m = 96, 1M
+    surface obs, LOCRAD 200 km, stride 3, DEnKF, capped inflation 1.05 (ratio
+    1.0) on every field.  J = sum over a 100 x 100 box, the top 10 layers of
+    temp and salt (20 fields) and the 96 members of (X^a - x^t)^2.  The
+    reference is the float64 torch forward over the 1,225 stride nodes whose
+    transforms reach the box (M, w, T = (I + M^-1)/2, their interpolation,
+    v_a = A^f T^T, the capped inflation), differentiated by autograd, and
+    central differences of it (h = 1e-6, relative for the variance) on the
+    most sensitive observation.  Bound: 1e-4 of the largest gradient."""
+    import time
+    import synth_gpu
+    print("case adjoint_ens C5: 32x600x600, m=96, 1M obs", flush=True)
+    X = synth_gpu.make_tile(nj=600, ni=600, nk=32, nmem=96, n_sst=1_000_000, n_sla=0, n_prof=0, corr_km=150.0,
+                            locrad=200.0, obs_std=(0.3, 0.04, 0.05), seed=5)
+    m, s_ = X.nmem, X.stride
+    nobs, ncell = X.nobs, X.ncell
+    eng = enkfg.Engine(m, mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF, keep_fp64=True)
+    eng.set_obstypes(X.obstypes)
+    eng.set_obs(X.obs_type, X.obs_status, X.obs_lon, X.obs_lat, X.obs_estd)
+    eng.set_S(X.HA, X.innov, standardised=False)
+    eng.calc_transforms(X.node_lon, X.node_lat, stride=s_)
+    eng.ens_attach(X.nlevels, s_, 0, 0, X.fields, X.E.ptr)
+    j0, i0, nb = 250, 250, 100
+    fsel = list(range(10)) + list(range(X.nk, X.nk + 10))
+    nf = len(fsel)
+    rows = np.arange(j0, j0 + nb)
+    cols = np.arange(i0, i0 + nb)
+    cells = (rows[:, None] * X.ni + cols[None, :]).ravel()
+    nc = len(cells)
+    wet = np.stack([X.nlevels.ravel()[cells] > X.fields[f]["level"] for f in fsel]).astype(np.float64)
+    Ef = np.stack([np.concatenate([X.E.read((f * ncell + j * X.ni + i0) * m, nb * m).reshape(nb, m) for j in rows])
+                   for f in fsel])                                           # [f][cell][member] float32
+    xf = Ef.astype(np.float64).mean(2)
+    Af = (Ef.astype(np.float64) - xf[:, :, None]).astype(np.float32).astype(np.float64)
+    # the four interpolation nodes and weights of each box cell
+    jjs = sorted(set(rows // s_) | set(np.minimum(rows // s_ + 1, X.nj_s - 1)))
+    iis = sorted(set(cols // s_) | set(np.minimum(cols // s_ + 1, X.ni_s - 1)))
+    nodes = np.array([jj * X.ni_s + ii for jj in jjs for ii in iis])
+    pos = {int(n): k for k, n in enumerate(nodes)}
+    idx = np.zeros((nc, 4), np.int64)
+    wgt = np.zeros((nc, 4))
+    one = np.float32(1.0)
+    for c, (j, i) in enumerate((j, i) for j in rows for i in cols):
+        jj, sj, ii, si = j // s_, j % s_, i // s_, i % s_
+        jj2, ii2 = min(jj + 1, X.nj_s - 1), min(ii + 1, X.ni_s - 1)
+        wj2 = np.float32(sj) / np.float32(s_)
+        wi2 = np.float32(si) / np.float32(s_)
+        bj = (1.0, 0.0) if sj == 0 else (float(one - wj2), float(wj2))
+        bi = (1.0, 0.0) if si == 0 else (float(one - wi2), float(wi2))
+        q = 0
+        for nj_, bjv in ((jj, bj[0]), (jj2, bj[1])):
+            for ni_, biv in ((ii, bi[0]), (ii2, bi[1])):
+                idx[c, q] = pos[nj_ * X.ni_s + ni_]
+                wgt[c, q] = bjv * biv
+                q += 1
+    idxt = torch.tensor(idx)
+    wgtt = torch.tensor(wgt)
+    Aft = torch.tensor(Af)
+    xft = torch.tensor(xf)
+    wett = torch.tensor(wet)[:, :, None]
+    jg, ig = np.meshgrid(rows, cols, indexing="ij")
+    patt = (np.cos(2 * np.pi * ig / 50.0) * np.sin(2 * np.pi * jg / 70.0)).ravel()
+    xt = torch.tensor(xf + np.array([0.3 if f < X.nk else 0.03 for f in fsel])[:, None] * patt[None, :])[:, :, None]
+    ob = O.OracleObs(X.obstypes, X.obs_type, X.obs_lon, X.obs_lat, X.obs_estd)
+    lists = [ob.findlocal(X.node_lon.ravel()[n], X.node_lat.ravel()[n]) for n in nodes]
+    HA = torch.tensor(X.HA.astype(np.float64))
+    sq = math.sqrt(m - 1)
+    eye = torch.eye(m, dtype=torch.float64)
+
+    def node_wT(k, innov, R):
+        ids, lc = lists[k]
+        if len(ids) == 0:
+            return torch.zeros(m, dtype=torch.float64), eye
+        it = torch.tensor(ids, dtype=torch.long)
+        d = torch.sqrt(R[it]) * sq
+        lct = torch.tensor(lc)
+        S = HA[:, it].T / d[:, None] * lct[:, None]
+        sv = innov[it] / d * lct
+        Mi = torch.linalg.inv(eye + S.T @ S)
+        return Mi @ (S.T @ sv), 0.5 * (eye + Mi)
+
+    def analysis(W, T, infl):
+        Wc = (W[idxt] * wgtt[:, :, None]).sum(1)                            # [cell][m]
+        Tc = (T[idxt] * wgtt[:, :, None, None]).sum(1)                      # [cell][m][m]
+        va = torch.einsum("fce,cie->fci", Aft, Tc)
+        xa = xft + (Aft * Wc[None]).sum(2)
+        v2f = (Aft ** 2).sum(2)
+        v2a = (va ** 2).sum(2)
+        raw = torch.sqrt(v2f / v2a)                                         # ratio 1.0
+        inf = torch.where(raw < infl[:, None], raw, infl[:, None].expand_as(raw))
+        inf = torch.where(inf - 1.0 > 1e-6, inf, torch.ones_like(inf))
+        return inf[:, :, None] * va + xa[:, :, None]
+
+    def J(Xa):
+        return (((Xa - xt) ** 2) * wett).sum()
+
+    R0 = torch.tensor(X.obs_estd.astype(np.float64) ** 2)
+    inn0 = torch.tensor(X.innov)
+    iv = inn0.clone().requires_grad_(True)
+    Rv = R0.clone().requires_grad_(True)
+    fv = torch.full((nf,), 1.05, dtype=torch.float64, requires_grad=True)
+    t = time.perf_counter()
+    WT = [node_wT(k, iv, Rv) for k in range(len(nodes))]
+    W = torch.stack([a for a, _ in WT])
+    T = torch.stack([b for _, b in WT])
+    Xa = analysis(W, T, fv)
+    Jr = J(Xa)
+    Jr.backward()
+    t_ref = time.perf_counter() - t
+    gy_ref, gR_ref, gi_ref = iv.grad.numpy(), Rv.grad.numpy(), fv.grad.numpy()
+    # GPU adjoint of dJ/dX^a
+    G = (2.0 * (Xa.detach() - xt) * wett).numpy().astype(np.float32)      # [f][cell][m]
+    dG = synth_gpu.DeviceArray((X.nfield, ncell, m), np.float32)
+    dG.zero()
+    for k, f in enumerate(fsel):
+        for r, j in enumerate(rows):
+            dG.write((f * ncell + j * X.ni + i0) * m, G[k, r * nb:(r + 1) * nb].ravel())
+    t = time.perf_counter()
+    gy, gR, gi = eng.adjoint_ens(dG.ptr, nobs, X.nfield)
+    t_adj = time.perf_counter() - t
+    eng.close()
+    ey = float(np.abs(gy - gy_ref).max() / np.abs(gy_ref).max())
+    eR = float(np.abs(gR - gR_ref).max() / np.abs(gR_ref).max())
+    ei = float(np.abs(gi[fsel] - gi_ref).max() / np.abs(gi_ref).max())
+    others = np.delete(gi, fsel)
+    print("  %d nodes reach the box; J %.6g; dJ/dinfl of the box fields %.3g .. %.3g" % (
+        len(nodes), Jr.item(), gi_ref.min(), gi_ref.max()), flush=True)
+    print("  enkfg_adjoint_ens %.2f s (torch float64 reference %.1f s); error vs autograd: y %.2g  R %.2g  "
+          "inflation %.2g" % (t_adj, t_ref, ey, eR, ei), flush=True)
+    assert (gy_ref != 0).sum() > 10000 and np.abs(gi_ref).max() > 0
+    assert ey <= 1e-4 and eR <= 1e-4 and ei <= 1e-4 and np.all(others == 0.0)
+    # central differences on the most sensitive obs (only the nodes whose list
+    # holds it change)
+    with torch.no_grad():
+        W0, T0 = W.detach().clone(), T.detach().clone()
+        f0 = torch.full((nf,), 1.05, dtype=torch.float64)
+        for name, base, gg in (("y", inn0, gy), ("R", R0, gR)):
+            o = int(np.argmax(np.abs(gg)))
+            ks = [k for k in range(len(nodes)) if o in set(lists[k][0].tolist())]
+            h = 1e-6 if name == "y" else 1e-6 * float(base[o])
+            Js = []
+            for sgn in (1.0, -1.0):
+                x = base.clone()
+                x[o] += sgn * h
+                Wp, Tp = W0.clone(), T0.clone()
+                for k in ks:
+                    Wp[k], Tp[k] = node_wT(k, x, R0) if name == "y" else node_wT(k, inn0, x)
+                Js.append(J(analysis(Wp, Tp, f0)).item())
+            fd = (Js[0] - Js[1]) / (2 * h)
+            err = abs(fd - float(gg[o])) / float(np.abs(gg).max())
+            print("  finite differences %s (obs %d, %d nodes): %.6g vs %.6g, error %.2g" % (
+                name, o, len(ks), fd, float(gg[o]), err), flush=True)
+            assert err <= 1e-4
