@@ -430,7 +430,11 @@ def main():
     kern = {"gram": phases[1], "update": phases[3], "loc": phases[0]}
     dom = max(kern, key=kern.get)
     mt = (m + 15) // 16
-    uname = "update_mfma_kernel<%d, %s>" % (mt, "true" if V is Rm else "false")
+    kt, ot = (m + 15) // 16, (md + 15) // 16
+    full = (m == 16 * kt and md == 16 * ot)
+    uname = "update_tile_kernel<%d, %d, %s, %s, %s>" % (kt, kt if not (kt == 9 and ot == 6) else ot,
+                                                      "true" if V is Rm else "false",
+                                                      "true" if m % 4 == 0 else "false", "true" if full else "false")
     uach = B_upd / (phases[3] * 1e-3)
     upd_roof = dict(bound="hbm", achieved=uach / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=uach / PEAK_HBM,
                     traffic=profiled_traffic(tag, uname), kernel=uname, launches_per_step=1,

@@ -32,8 +32,22 @@ _MSG = {EIO: "open, read or write failed", EFORMAT: "not a netCDF classic file, 
 
 EXPORTED = ["ncf_create", "ncf_def_dim", "ncf_def_var", "ncf_put_att", "ncf_put_att_text", "ncf_enddef",
             "ncf_put_slab", "ncf_open", "ncf_inq", "ncf_inq_varid", "ncf_inq_varname", "ncf_inq_var",
-            "ncf_inq_dimlen", "ncf_get_att", "ncf_get_slab", "ncf_close", "enkfio_write_transforms",
-            "enkfio_inq_transforms", "enkfio_read_transforms", "enkfio_write_diag", "enkfio_write_fields2d"]
+            "ncf_inq_dimlen", "ncf_get_att", "ncf_get_slab", "ncf_get_vara", "ncf_get_vara_as", "ncf_inq_natts",
+            "ncf_inq_attname", "ncf_close", "enkfio_write_transforms", "enkfio_inq_transforms",
+            "enkfio_read_transforms", "enkfio_write_diag", "enkfio_write_fields2d", "enkfio_field_nlevels",
+            "enkfio_read_field", "enkfio_read_obs"]
+MAXTYPES, NAMELEN = 64, 64
+
+
+class ObsStruct(C.Structure):
+    """enkfio_obs (include/enkfio.h)"""
+    _fields_ = [("nobs", C.c_longlong), ("da_day", C.c_double), ("ntypes", C.c_int),
+                ("typenames", (C.c_char * NAMELEN) * MAXTYPES)] + \
+        [(n, C.POINTER(C.c_int)) for n in ("id", "id_orig", "batch", "aux")] + \
+        [(n, C.POINTER(C.c_short)) for n in ("status", "type", "product", "instrument", "fid")] + \
+        [(n, C.POINTER(C.c_float)) for n in ("value", "estd", "footprint", "lon", "lat", "depth", "model_depth",
+                                             "fk", "time")] + \
+        [(n, C.POINTER(C.c_double)) for n in ("fi", "fj", "fi2")]
 
 
 class EnkfioError(RuntimeError):
@@ -72,6 +86,16 @@ def lib():
         L.enkfio_read_transforms.argtypes = [cp, ip, ip, vp, vp]
         L.enkfio_write_diag.argtypes = [cp, ip, C.POINTER(C.c_char_p), ip, ip, ip, vp, vp, vp]
         L.enkfio_write_fields2d.argtypes = [cp, ip, C.POINTER(C.c_char_p), ip, ip, vp]
+        szp = C.POINTER(C.c_size_t)
+        L.ncf_get_vara.argtypes = [vp, ip, szp, szp, vp]
+        L.ncf_get_vara_as.argtypes = [vp, ip, szp, szp, ip, vp]
+        L.ncf_inq_natts.argtypes = [vp, ip, C.POINTER(C.c_int)]
+        L.ncf_inq_attname.argtypes = [vp, ip, ip, C.c_char_p, sz]
+        L.enkfio_field_nlevels.argtypes = [cp, cp, C.POINTER(C.c_int)]
+        L.enkfio_read_field.argtypes = [cp, cp, ip, ip, ip, ip, ip, ip, ip, vp]
+        L.enkfio_read_obs.argtypes = [cp, C.POINTER(ObsStruct)]
+        L.enkfio_free_obs.argtypes = [C.POINTER(ObsStruct)]
+        L.enkfio_free_obs.restype = None
         for name in EXPORTED:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -164,6 +188,40 @@ class NcReader:
         if t.value == CHAR:
             return bytes(out[:n.value]).decode()
         return out[:n.value]
+
+
+def field_nlevels(path, varname):
+    """ncu_getnlevels of a structured-grid variable"""
+    n = C.c_int()
+    _chk(lib().enkfio_field_nlevels(_b(path), _b(varname), C.byref(n)), "%s: %s" % (path, varname))
+    return n.value
+
+
+def read_field(path, varname, k, ni, nj, nk, r=-1, j1=0, j2=-1):
+    """ncu_readfield_part: rows j1 .. j2 (all by default) of layer k as float32
+    [rows][ni], fill / missing / out-of-range values NaN, scale and offset applied"""
+    rows = nj if (j1 <= 0 and j2 < 0) else j2 - j1 + 1
+    out = np.empty((rows, ni), np.float32)
+    _chk(lib().enkfio_read_field(_b(path), _b(varname), r, k, j1, j2, ni, nj, nk, _ptr(out)),
+         "%s: %s" % (path, varname))
+    return out
+
+
+def read_obs(path):
+    """observations.nc as obs_read reads it: a dict of numpy arrays (copies),
+    plus 'nobs', 'da_day' and 'typenames'"""
+    o = ObsStruct()
+    _chk(lib().enkfio_read_obs(_b(path), C.byref(o)), path)
+    try:
+        n = o.nobs
+        out = dict(nobs=n, da_day=o.da_day,
+                   typenames=[bytes(o.typenames[k].value).decode() for k in range(o.ntypes)])
+        for name, typ in ObsStruct._fields_[4:]:
+            p = getattr(o, name)
+            out[name] = np.ctypeslib.as_array(p, (n,)).copy() if (p and n > 0) else None
+        return out
+    finally:
+        lib().enkfio_free_obs(C.byref(o))
 
 
 def write_transforms(path, w, T=None, stride=1, grid_name=""):

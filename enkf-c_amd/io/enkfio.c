@@ -22,6 +22,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
+#include <float.h>
 
 #include "enkfio.h"
 
@@ -45,7 +47,9 @@ typedef struct {
     ncatt* atts;
     int natt;
     uint64_t begin;
-    uint64_t nbytes;     /* unpadded data size */
+    uint64_t nbytes;     /* unpadded data size (all records) */
+    int isrec;           /* along the record dimension */
+    uint64_t recbytes;   /* unpadded bytes per record (record variables) */
 } ncvar;
 
 typedef struct {
@@ -64,6 +68,9 @@ struct ncfile {
     int ngatt;
     ncvar* vars;
     int nvars;
+    int recdim;          /* the record (unlimited) dimension, -1 if none */
+    uint64_t numrecs;
+    uint64_t recsize;    /* bytes of one record of all record variables */
 };
 
 static size_t type_size(int t)
@@ -269,6 +276,7 @@ int ncf_create(const char* path, ncfile** out)
     f->writing = 1;
     f->defmode = 1;
     f->version = 2;
+    f->recdim = -1;
     *out = f;
     return ENKFIO_OK;
 }
@@ -564,8 +572,10 @@ int ncf_open(const char* path, ncfile** out)
         f->version = hdr[3];
         r.pos = 4;
         e = get_u32(&r, &nrec);
-        if (!e && nrec != 0)
-            e = ENKFIO_EFORMAT;        /* record variables are not read here */
+        if (!e && nrec == 0xFFFFFFFFu)
+            e = ENKFIO_EFORMAT;        /* a file still being written (streaming) */
+        f->numrecs = nrec;
+        f->recdim = -1;
         /* dimensions */
         if (!e)
             e = get_u32(&r, &tag);
@@ -583,6 +593,12 @@ int ncf_open(const char* path, ncfile** out)
                 e = get_name(&r, &f->dims[k].name);
                 if (!e)
                     e = get_u32(&r, &len);
+                if (!e && len == 0) {  /* the record dimension */
+                    if (f->recdim >= 0)
+                        e = ENKFIO_EFORMAT;
+                    f->recdim = (int) k;
+                    len = nrec;
+                }
                 f->dims[k].len = len;
                 if (!e)
                     f->ndims++;
@@ -641,9 +657,30 @@ int ncf_open(const char* path, ncfile** out)
                     e = ENKFIO_EFORMAT;
                 v->type = (int) type;
                 v->nbytes = nel * type_size((int) type);
+                v->isrec = (nd > 0 && f->recdim >= 0 && v->dimids[0] == f->recdim);
+                if (v->isrec)
+                    v->recbytes = (nrec > 0 ? nel / nrec : 0) * type_size((int) type);
+                if (v->isrec && nrec == 0) {   /* no records yet: per-record size from the other dims */
+                    uint64_t per = 1;
+                    for (d = 1; d < nd; ++d)
+                        per *= f->dims[v->dimids[d]].len;
+                    v->recbytes = per * type_size((int) type);
+                }
                 if (!e)
                     f->nvars++;
             }
+        }
+        if (!e) {
+            /* records hold every record variable in order, each padded to
+             * 4 bytes, except when there is only one */
+            int nrv = 0, k2;
+
+            f->recsize = 0;
+            for (k2 = 0; k2 < f->nvars; ++k2)
+                nrv += f->vars[k2].isrec;
+            for (k2 = 0; k2 < f->nvars; ++k2)
+                if (f->vars[k2].isrec)
+                    f->recsize += (nrv == 1) ? f->vars[k2].recbytes : pad4(f->vars[k2].recbytes);
         }
         if (e == ENKFIO_EFORMAT && got == cap && cap < ((size_t) 1 << 30)) {
             /* the header may continue past the prefix read: a longer one */
@@ -798,14 +835,203 @@ int ncf_get_slab(ncfile* f, int varid, size_t start, size_t count, void* data)
     if (f == NULL || varid < 0 || varid >= f->nvars || (count > 0 && data == NULL))
         return ENKFIO_EARG;
     v = &f->vars[varid];
+    ts = type_size(v->type);
+    if (v->isrec) {
+        size_t k, per = (size_t) (v->recbytes / ts);
+
+        if (start > f->numrecs || count > f->numrecs - start)
+            return ENKFIO_ERANGE;
+        for (k = 0; k < count; ++k) {
+            unsigned char* dst = (unsigned char*) data + k * v->recbytes;
+
+            if (fseeko(f->fp, (off_t) (v->begin + (start + k) * f->recsize), SEEK_SET) != 0)
+                return ENKFIO_EIO;
+            if (per > 0 && fread(dst, ts, per, f->fp) != per)
+                return ENKFIO_EIO;
+            swap_be(dst, per, ts);
+        }
+        return ENKFIO_OK;
+    }
     if ((r = slab_range(f, v, start, count, &off, &nel)))
         return r;
-    ts = type_size(v->type);
     if (fseeko(f->fp, (off_t) off, SEEK_SET) != 0)
         return ENKFIO_EIO;
     if (nel > 0 && fread(data, ts, (size_t) nel, f->fp) != (size_t) nel)
         return ENKFIO_EIO;
     swap_be(data, (size_t) nel, ts);
+    return ENKFIO_OK;
+}
+
+int ncf_get_vara(ncfile* f, int varid, const size_t* start, const size_t* count, void* data)
+{
+    const ncvar* v;
+    size_t len[ENKFIO_MAXDIMS], idx[ENKFIO_MAXDIMS], ts, run, nrow = 1;
+    uint64_t stride[ENKFIO_MAXDIMS];
+    unsigned char* dst = data;
+    int nd, d;
+
+    if (f == NULL || varid < 0 || varid >= f->nvars)
+        return ENKFIO_EARG;
+    v = &f->vars[varid];
+    nd = v->ndims;
+    ts = type_size(v->type);
+    if (nd == 0)
+        return ncf_get_slab(f, varid, 0, 1, data);
+    if (start == NULL || count == NULL || data == NULL)
+        return ENKFIO_EARG;
+    if (v->isrec && nd == 1)   /* records are not contiguous */
+        return ncf_get_slab(f, varid, start[0], count[0], data);
+    for (d = 0; d < nd; ++d) {
+        len[d] = f->dims[v->dimids[d]].len;
+        if (start[d] > len[d] || count[d] > len[d] - start[d])
+            return ENKFIO_ERANGE;
+        if (count[d] == 0)
+            return ENKFIO_OK;
+    }
+    /* element strides within a record (record variables) or the variable */
+    stride[nd - 1] = 1;
+    for (d = nd - 2; d >= 0; --d)
+        stride[d] = stride[d + 1] * len[d + 1];
+    for (d = 0; d < nd - 1; ++d)
+        nrow *= count[d];
+    run = count[nd - 1];
+    for (d = 0; d < nd; ++d)
+        idx[d] = start[d];
+    while (nrow-- > 0) {
+        uint64_t off;
+
+        if (v->isrec) {
+            uint64_t e = 0;
+
+            for (d = 1; d < nd; ++d)
+                e += idx[d] * stride[d];
+            off = v->begin + idx[0] * f->recsize + e * ts;
+        } else {
+            uint64_t e = 0;
+
+            for (d = 0; d < nd; ++d)
+                e += idx[d] * stride[d];
+            off = v->begin + e * ts;
+        }
+        if (fseeko(f->fp, (off_t) off, SEEK_SET) != 0 || fread(dst, ts, run, f->fp) != run)
+            return ENKFIO_EIO;
+        swap_be(dst, run, ts);
+        dst += run * ts;
+        /* next row: odometer over dimensions 0 .. nd - 2 */
+        for (d = nd - 2; d >= 0; --d) {
+            if (++idx[d] < start[d] + count[d])
+                break;
+            idx[d] = start[d];
+        }
+    }
+    return ENKFIO_OK;
+}
+
+/* value i of a native-endian array of netCDF type t, as double */
+static double get_as_double(const void* p, int t, size_t i)
+{
+    switch (t) {
+    case ENKFIO_BYTE:
+        return (double) ((const signed char*) p)[i];
+    case ENKFIO_CHAR:
+        return (double) ((const unsigned char*) p)[i];
+    case ENKFIO_SHORT:
+        return (double) ((const int16_t*) p)[i];
+    case ENKFIO_INT:
+        return (double) ((const int32_t*) p)[i];
+    case ENKFIO_FLOAT:
+        return (double) ((const float*) p)[i];
+    default:
+        return ((const double*) p)[i];
+    }
+}
+
+int ncf_get_vara_as(ncfile* f, int varid, const size_t* start, const size_t* count, int type, void* out)
+{
+    int vt, nd, d, r;
+    size_t n = 1, dl[ENKFIO_MAXDIMS], st0[ENKFIO_MAXDIMS], i, ts;
+    const size_t* st = start;
+    const size_t* ct = count;
+    void* raw;
+
+    if ((r = ncf_inq_var(f, varid, &vt, &nd, dl)) != ENKFIO_OK)
+        return r;
+    if (type_size(type) == 0 || vt == ENKFIO_CHAR || type == ENKFIO_CHAR || out == NULL)
+        return ENKFIO_EARG;
+    if (st == NULL) {   /* the whole variable */
+        for (d = 0; d < nd; ++d)
+            st0[d] = 0;
+        st = st0;
+        ct = dl;
+    }
+    for (d = 0; d < nd; ++d)
+        n *= ct[d];
+    ts = type_size(vt);
+    raw = malloc(n * ts + 1);
+    if (raw == NULL)
+        return ENKFIO_ENOMEM;
+    if ((r = ncf_get_vara(f, varid, st, ct, raw)) != ENKFIO_OK) {
+        free(raw);
+        return r;
+    }
+    for (i = 0; i < n; ++i) {
+        const double x = get_as_double(raw, vt, i);
+
+        switch (type) {
+        case ENKFIO_BYTE:
+            ((signed char*) out)[i] = (signed char) x;
+            break;
+        case ENKFIO_SHORT:
+            ((int16_t*) out)[i] = (int16_t) x;
+            break;
+        case ENKFIO_INT:
+            ((int32_t*) out)[i] = (int32_t) x;
+            break;
+        case ENKFIO_FLOAT:
+            /* a double beyond the float range reads as NaN
+             * (ncw_get_vara_float_fixerange, common/ncw.c:1078-1112) */
+            ((float*) out)[i] = (!isfinite(x) || x < -FLT_MAX || x > FLT_MAX) && vt == ENKFIO_DOUBLE ? NAN : (float) x;
+            break;
+        default:
+            ((double*) out)[i] = x;
+        }
+    }
+    free(raw);
+    return ENKFIO_OK;
+}
+
+int ncf_inq_natts(const ncfile* f, int varid, int* natts)
+{
+    if (f == NULL || natts == NULL)
+        return ENKFIO_EARG;
+    if (varid == ENKFIO_GLOBAL)
+        *natts = f->ngatt;
+    else if (varid >= 0 && varid < f->nvars)
+        *natts = f->vars[varid].natt;
+    else
+        return ENKFIO_EARG;
+    return ENKFIO_OK;
+}
+
+int ncf_inq_attname(const ncfile* f, int varid, int k, char* name, size_t cap)
+{
+    const ncatt* a;
+    int na;
+
+    if (f == NULL || name == NULL || cap == 0)
+        return ENKFIO_EARG;
+    if (varid == ENKFIO_GLOBAL) {
+        a = f->gatts;
+        na = f->ngatt;
+    } else if (varid >= 0 && varid < f->nvars) {
+        a = f->vars[varid].atts;
+        na = f->vars[varid].natt;
+    } else
+        return ENKFIO_EARG;
+    if (k < 0 || k >= na)
+        return ENKFIO_EARG;
+    strncpy(name, a[k].name, cap - 1);
+    name[cap - 1] = 0;
     return ENKFIO_OK;
 }
 
@@ -1001,4 +1227,344 @@ int enkfio_write_fields2d(const char* path, int nfield, const char* const* names
         TRY(ncf_put_slab(f, vid[k], 0, (size_t) nj, v + (size_t) k * nj * ni));
     free(vid);
     return ncf_close(f);
+}
+
+/* ---------------------------------------------------------------- model fields and observations */
+#undef TRY
+#define TRY(x) do { int r_ = (x); if (r_ < 0) { ncf_close(f); return r_; } } while (0)
+
+/* the default fill value of a classic file's type, in the type's bytes */
+static void default_fill(int t, unsigned char* v)
+{
+    const signed char b = -127;
+    const int16_t sh = -32767;
+    const int32_t in = -2147483647;
+    const float fl = 9.9692099683868690e+36f;
+    const double db = 9.9692099683868690e+36;
+
+    switch (t) {
+    case ENKFIO_BYTE:
+        memcpy(v, &b, 1);
+        break;
+    case ENKFIO_CHAR:
+        v[0] = 0;
+        break;
+    case ENKFIO_SHORT:
+        memcpy(v, &sh, 2);
+        break;
+    case ENKFIO_INT:
+        memcpy(v, &in, 4);
+        break;
+    case ENKFIO_FLOAT:
+        memcpy(v, &fl, 4);
+        break;
+    default:
+        memcpy(v, &db, 8);
+    }
+}
+
+int enkfio_field_nlevels(const char* path, const char* varname, int* nlevels)
+{
+    ncfile* f;
+    size_t dl[ENKFIO_MAXDIMS];
+    int vid, nd, rec;
+
+    if (nlevels == NULL)
+        return ENKFIO_EARG;
+    TRY(ncf_open(path, &f));
+    TRY(vid = ncf_inq_varid(f, varname));
+    TRY(ncf_inq_var(f, vid, NULL, &nd, dl));
+    rec = f->vars[vid].isrec || (nd > 0 && dl[0] == 1);
+    /* common/ncutils.c:54-91, structured grids */
+    if (nd > 4 || (nd == 4 && !rec) || nd < 2) {
+        ncf_close(f);
+        return ENKFIO_EARG;
+    }
+    *nlevels = (nd == 4) ? (int) dl[1] : (nd == 3) ? (rec ? 1 : (int) dl[0]) : (rec ? 0 : 1);
+    return ncf_close(f);
+}
+
+int enkfio_read_field(const char* path, const char* varname, int r, int k, int j1, int j2, int ni, int nj, int nk, float* v)
+{
+    ncfile* f;
+    size_t dl[ENKFIO_MAXDIMS], start[4] = {0, 0, 0, 0}, count[4] = {1, 1, 1, 1};
+    int vid, nd, vt, isrec, natt, a;
+    size_t n, i, ts;
+    unsigned char* raw;
+
+    if (v == NULL || ni <= 0 || nj <= 0)
+        return ENKFIO_EARG;
+    if (j1 <= 0 && j2 < 0) {
+        j1 = 0;
+        j2 = nj - 1;
+    }
+    if (j2 < j1 || j1 < 0 || j2 > nj - 1)
+        return ENKFIO_EARG;
+    TRY(ncf_open(path, &f));
+    TRY(vid = ncf_inq_varid(f, varname));
+    TRY(ncf_inq_var(f, vid, &vt, &nd, dl));
+    isrec = f->vars[vid].isrec;
+    if (vt == ENKFIO_CHAR || (isrec && dl[0] == 0))
+        TRY(ENKFIO_EARG);
+    if (isrec && r < 0) {
+        if (dl[0] > 1)
+            TRY(ENKFIO_EARG);   /* several records: which one must be given */
+        r = 0;
+    }
+    /* the dimensions ncu_readfield_part() accepts (common/ncutils.c:99-293) */
+    if (nd == 4) {
+        if (r < 0) {
+            if (dl[0] > 1)
+                TRY(ENKFIO_EARG);
+            r = 0;
+        }
+        if ((size_t) r >= dl[0] || dl[3] != (size_t) ni || dl[2] != (size_t) nj)
+            TRY(ENKFIO_EARG);
+        if (dl[1] != (size_t) nk) {
+            if (dl[1] != 1)
+                TRY(ENKFIO_EARG);
+            k = 0;
+        }
+        start[0] = (size_t) r;
+        start[1] = (size_t) k;
+        start[2] = (size_t) j1;
+        count[2] = (size_t) (j2 - j1 + 1);
+        count[3] = dl[3];
+    } else if (nd == 3) {
+        if (dl[2] != (size_t) ni || dl[1] != (size_t) nj)
+            TRY(ENKFIO_EARG);
+        if (!isrec) {
+            if (dl[0] != (size_t) nk && !(dl[0] == 1 && (k <= 0 || k == nk - 1)))
+                TRY(ENKFIO_EARG);
+            start[0] = (dl[0] == 1) ? 0 : (size_t) k;
+        } else {
+            if ((size_t) r >= dl[0])
+                TRY(ENKFIO_EARG);
+            start[0] = (size_t) r;   /* a layer of a record variable: k is ignored */
+        }
+        start[1] = (size_t) j1;
+        count[1] = (size_t) (j2 - j1 + 1);
+        count[2] = dl[2];
+    } else if (nd == 2) {
+        if (isrec || dl[1] != (size_t) ni || dl[0] != (size_t) nj)
+            TRY(ENKFIO_EARG);
+        start[0] = (size_t) j1;
+        count[0] = (size_t) (j2 - j1 + 1);
+        count[1] = dl[1];
+    } else
+        TRY(ENKFIO_EARG);
+    n = (size_t) (j2 - j1 + 1) * (size_t) ni;
+    TRY(ncf_get_vara_as(f, vid, start, count, ENKFIO_FLOAT, v));
+    /* fill, missing and valid values by the raw values (ncutils.c:296-510) */
+    ts = type_size(vt);
+    raw = malloc(n * ts + 1);
+    if (raw == NULL)
+        TRY(ENKFIO_ENOMEM);
+    if (ncf_get_vara(f, vid, start, count, raw) != ENKFIO_OK) {
+        free(raw);
+        TRY(ENKFIO_EIO);
+    }
+    {
+        unsigned char fill[8], mv[8];
+        int tt;
+        size_t na;
+
+        if (ncf_get_att(f, vid, "_FillValue", &tt, &na, NULL, 0) == ENKFIO_OK && na == 1 && tt == vt)
+            ncf_get_att(f, vid, "_FillValue", NULL, NULL, fill, 1);
+        else
+            default_fill(vt, fill);
+        for (i = 0; i < n; ++i)
+            if (memcmp(raw + i * ts, fill, ts) == 0)
+                v[i] = NAN;
+        if (ncf_get_att(f, vid, "missing_value", &tt, &na, NULL, 0) == ENKFIO_OK && na == 1 && tt == vt) {
+            ncf_get_att(f, vid, "missing_value", NULL, NULL, mv, 1);
+            for (i = 0; i < n; ++i)
+                if (memcmp(raw + i * ts, mv, ts) == 0)
+                    v[i] = NAN;
+        }
+        {
+            double lim[2];
+            unsigned char lb[16];
+
+            if (ncf_get_att(f, vid, "valid_min", &tt, &na, NULL, 0) == ENKFIO_OK && na == 1 && tt == vt) {
+                ncf_get_att(f, vid, "valid_min", NULL, NULL, lb, 1);
+                lim[0] = get_as_double(lb, vt, 0);
+                for (i = 0; i < n; ++i)
+                    if (get_as_double(raw, vt, i) < lim[0])
+                        v[i] = NAN;
+            }
+            if (ncf_get_att(f, vid, "valid_max", &tt, &na, NULL, 0) == ENKFIO_OK && na == 1 && tt == vt) {
+                ncf_get_att(f, vid, "valid_max", NULL, NULL, lb, 1);
+                lim[1] = get_as_double(lb, vt, 0);
+                for (i = 0; i < n; ++i)
+                    if (get_as_double(raw, vt, i) > lim[1])
+                        v[i] = NAN;
+            }
+            if (ncf_get_att(f, vid, "valid_range", &tt, &na, NULL, 0) == ENKFIO_OK && na == 2 && tt == vt) {
+                ncf_get_att(f, vid, "valid_range", NULL, NULL, lb, 2);
+                lim[0] = get_as_double(lb, vt, 0);
+                lim[1] = get_as_double(lb, vt, 1);
+                for (i = 0; i < n; ++i) {
+                    const double x = get_as_double(raw, vt, i);
+
+                    if (x < lim[0] || x > lim[1])
+                        v[i] = NAN;
+                }
+            }
+        }
+    }
+    free(raw);
+    /* scale_factor, then add_offset, as floats (ncutils.c:513-531) */
+    TRY(ncf_inq_natts(f, vid, &natt));
+    for (a = 0; a < 2; ++a) {
+        const char* an = a == 0 ? "scale_factor" : "add_offset";
+        int tt;
+        size_t na;
+        double x[1];
+
+        if (ncf_get_att(f, vid, an, &tt, &na, NULL, 0) != ENKFIO_OK || tt == ENKFIO_CHAR)
+            continue;
+        if (na != 1)
+            TRY(ENKFIO_EFORMAT);
+        {
+            unsigned char b[8];
+            float c;
+
+            ncf_get_att(f, vid, an, NULL, NULL, b, 1);
+            x[0] = get_as_double(b, tt, 0);
+            c = (float) x[0];
+            for (i = 0; i < n; ++i)
+                v[i] = (a == 0) ? v[i] * c : v[i] + c;
+        }
+    }
+    return ncf_close(f);
+}
+
+static int read_var_as(ncfile* f, const char* name, int type, size_t n, void** out)
+{
+    int vid = ncf_inq_varid(f, name);
+    size_t dl[ENKFIO_MAXDIMS];
+    int nd;
+
+    if (vid < 0)
+        return vid;
+    if (ncf_inq_var(f, vid, NULL, &nd, dl) != ENKFIO_OK || nd != 1 || dl[0] != n)
+        return ENKFIO_EFORMAT;
+    *out = malloc(n * type_size(type) + 1);
+    if (*out == NULL)
+        return ENKFIO_ENOMEM;
+    return ncf_get_vara_as(f, vid, NULL, NULL, type, *out);
+}
+
+void enkfio_free_obs(enkfio_obs* o)
+{
+    if (o == NULL)
+        return;
+    free(o->id);
+    free(o->id_orig);
+    free(o->status);
+    free(o->type);
+    free(o->product);
+    free(o->instrument);
+    free(o->fid);
+    free(o->batch);
+    free(o->value);
+    free(o->estd);
+    free(o->footprint);
+    free(o->lon);
+    free(o->lat);
+    free(o->depth);
+    free(o->model_depth);
+    free(o->fi);
+    free(o->fj);
+    free(o->fi2);
+    free(o->fk);
+    free(o->time);
+    free(o->aux);
+    memset(o, 0, sizeof(*o));
+}
+
+int enkfio_read_obs(const char* path, enkfio_obs* o)
+{
+    ncfile* f;
+    size_t n = 0, i;
+    int vid, natt, k, r;
+
+    if (o == NULL)
+        return ENKFIO_EARG;
+    memset(o, 0, sizeof(*o));
+    o->da_day = NAN;
+    TRY(ncf_open(path, &f));
+    TRY(ncf_inq_dimlen(f, "nobs", &n));
+    o->nobs = (long long) n;
+    {
+        int tt;
+        size_t na;
+
+        if (ncf_get_att(f, ENKFIO_GLOBAL, "DA_DAY", &tt, &na, NULL, 0) == ENKFIO_OK && na >= 1 && tt != ENKFIO_CHAR) {
+            unsigned char b[8];
+
+            ncf_get_att(f, ENKFIO_GLOBAL, "DA_DAY", NULL, NULL, b, 1);
+            o->da_day = get_as_double(b, tt, 0);
+        }
+    }
+    /* obs type names: the int attributes of "type" (observations.c:515-557) */
+    TRY(vid = ncf_inq_varid(f, "type"));
+    TRY(ncf_inq_natts(f, vid, &natt));
+    for (k = 0; k < natt; ++k) {
+        char name[ENKFIO_NAMELEN];
+        int tt, id;
+        size_t na;
+
+        TRY(ncf_inq_attname(f, vid, k, name, sizeof name));
+        if (ncf_get_att(f, vid, name, &tt, &na, NULL, 0) != ENKFIO_OK || tt != ENKFIO_INT || na != 1)
+            continue;
+        ncf_get_att(f, vid, name, NULL, NULL, &id, 1);
+        if (id < 0 || id >= ENKFIO_MAXTYPES)
+            continue;
+        snprintf(o->typenames[id], ENKFIO_NAMELEN, "%s", name);
+        if (id + 1 > o->ntypes)
+            o->ntypes = id + 1;
+    }
+#undef TRY
+#define TRY(x) do { int r_ = (x); if (r_ < 0) { ncf_close(f); enkfio_free_obs(o); return r_; } } while (0)
+    if (n == 0)
+        return ncf_close(f);
+    TRY(read_var_as(f, "id", ENKFIO_INT, n, (void**) &o->id));
+    TRY(read_var_as(f, "id_orig", ENKFIO_INT, n, (void**) &o->id_orig));
+    TRY(read_var_as(f, "status", ENKFIO_SHORT, n, (void**) &o->status));
+    TRY(read_var_as(f, "type", ENKFIO_SHORT, n, (void**) &o->type));
+    TRY(read_var_as(f, "product", ENKFIO_SHORT, n, (void**) &o->product));
+    TRY(read_var_as(f, "instrument", ENKFIO_SHORT, n, (void**) &o->instrument));
+    TRY(read_var_as(f, "fid", ENKFIO_SHORT, n, (void**) &o->fid));
+    TRY(read_var_as(f, "batch", ENKFIO_INT, n, (void**) &o->batch));
+    TRY(read_var_as(f, "value", ENKFIO_FLOAT, n, (void**) &o->value));
+    TRY(read_var_as(f, ncf_inq_varid(f, "estd_orig") >= 0 ? "estd_orig" : "estd", ENKFIO_FLOAT, n,
+                    (void**) &o->estd));
+    if (ncf_inq_varid(f, "footprint") >= 0)
+        TRY(read_var_as(f, "footprint", ENKFIO_FLOAT, n, (void**) &o->footprint));
+    TRY(read_var_as(f, "lon", ENKFIO_FLOAT, n, (void**) &o->lon));
+    TRY(read_var_as(f, "lat", ENKFIO_FLOAT, n, (void**) &o->lat));
+    TRY(read_var_as(f, "depth", ENKFIO_FLOAT, n, (void**) &o->depth));
+    TRY(read_var_as(f, "model_depth", ENKFIO_FLOAT, n, (void**) &o->model_depth));
+    if (ncf_inq_varid(f, "fi") >= 0) {
+        TRY(read_var_as(f, "fi", ENKFIO_DOUBLE, n, (void**) &o->fi));
+        TRY(read_var_as(f, "fj", ENKFIO_DOUBLE, n, (void**) &o->fj));
+    } else {
+        TRY(read_var_as(f, "fi0", ENKFIO_DOUBLE, n, (void**) &o->fi));
+        TRY(read_var_as(f, "fi1", ENKFIO_DOUBLE, n, (void**) &o->fj));
+        TRY(read_var_as(f, "fi2", ENKFIO_DOUBLE, n, (void**) &o->fi2));
+    }
+    TRY(read_var_as(f, "fk", ENKFIO_FLOAT, n, (void**) &o->fk));
+    TRY(read_var_as(f, "time", ENKFIO_FLOAT, n, (void**) &o->time));
+    TRY(read_var_as(f, "aux", ENKFIO_INT, n, (void**) &o->aux));
+    /* a batch rejected by an earlier run is an OK observation again
+     * (observations.c:786-796; STATUS_BADBATCH = 5, STATUS_OK = 0) */
+    for (i = 0; i < n; ++i)
+        if (o->status[i] == 5)
+            o->status[i] = 0;
+    r = ncf_close(f);
+    if (r != ENKFIO_OK)
+        enkfio_free_obs(o);
+    return r;
 }

@@ -74,6 +74,15 @@ int ncf_inq_dimlen(const ncfile* f, const char* name, size_t* len);
  * values != NULL, up to cap values (text: cap bytes) */
 int ncf_get_att(const ncfile* f, int varid, const char* name, int* type, size_t* n, void* values, size_t cap);
 int ncf_get_slab(ncfile* f, int varid, size_t start, size_t count, void* data);
+/* a hyperslab start[d] .. start[d] + count[d] - 1 of every dimension, in the
+ * variable's type (native-endian); record variables are read record by record */
+int ncf_get_vara(ncfile* f, int varid, const size_t* start, const size_t* count, void* data);
+/* the same converted to `type` (ENKFIO_BYTE .. ENKFIO_DOUBLE, not CHAR), as C
+ * casts; a double beyond the float range reads as NaN (common/ncw.c:1078);
+ * start == NULL reads the whole variable */
+int ncf_get_vara_as(ncfile* f, int varid, const size_t* start, const size_t* count, int type, void* out);
+int ncf_inq_natts(const ncfile* f, int varid, int* natts);
+int ncf_inq_attname(const ncfile* f, int varid, int k, char* name, size_t cap);
 
 int ncf_close(ncfile* f);
 
@@ -90,6 +99,40 @@ int enkfio_read_transforms(const char* path, int j0, int nrow, float* T, float* 
 int enkfio_write_diag(const char* path, int nobstypes, const char* const* typenames, int nj, int ni, int stride, const int* pnlobs, const float* pdfs, const float* psrf);
 /* spread.nc / inflation.nc: nfield float [nj][ni] variables named names[f] */
 int enkfio_write_fields2d(const char* path, int nfield, const char* const* names, int nj, int ni, const float* v);
+
+/* ---- model fields (ensemble members, backgrounds), structured grids */
+/* ncu_getnlevels (common/ncutils.c:54-91): layers of a [(t,)k,]j,i variable */
+int enkfio_field_nlevels(const char* path, const char* varname, int* nlevels);
+/* ncu_readfield_part (common/ncutils.c:99-531): rows j1 .. j2 (j1 <= 0 and
+ * j2 < 0: all) of layer k, record r (-1: the only one), of a [(t,)k,]j,i or
+ * j,i variable on an nj x ni grid of nk layers, as float [j2-j1+1][ni];
+ * values equal to _FillValue (or the type's default fill), missing_value, or
+ * outside valid_min / valid_max / valid_range become NaN; then scale_factor
+ * and add_offset apply, as floats.  Dimensions that do not fit the grid
+ * return ENKFIO_EARG (the reference quits). */
+int enkfio_read_field(const char* path, const char* varname, int r, int k, int j1, int j2, int ni, int nj, int nk, float* v);
+
+/* ---- observations.nc (common/observations.c:452-802) */
+#define ENKFIO_MAXTYPES 64
+#define ENKFIO_NAMELEN 64
+typedef struct {
+    long long nobs;
+    double da_day;                      /* global DA_DAY; NaN when absent */
+    int ntypes;                         /* the obs type names: the int attributes of "type" */
+    char typenames[ENKFIO_MAXTYPES][ENKFIO_NAMELEN];
+    int *id, *id_orig, *batch, *aux;
+    short *status, *type, *product, *instrument, *fid;
+    float *value, *estd;                /* estd_orig when present, as obs_read */
+    float *footprint;                   /* NULL when absent */
+    float *lon, *lat, *depth, *model_depth, *fk, *time;
+    double *fi, *fj;                    /* fi, fj; or fi0, fi1 with fi2 (unstructured grids) */
+    double *fi2;                        /* NULL for structured grids */
+} enkfio_obs;
+/* reads every record of the file; STATUS_BADBATCH becomes STATUS_OK as in
+ * obs_read.  The times are not clipped to the obs types' windows (that needs
+ * the obs type parameters).  Free with enkfio_free_obs. */
+int enkfio_read_obs(const char* path, enkfio_obs* obs);
+void enkfio_free_obs(enkfio_obs* obs);
 
 #ifdef __cplusplus
 }

@@ -194,12 +194,164 @@ def test_writer_api_and_errors(tmp_path):
         assert e.value.code == enkfio.EIO
 
 
-def test_record_files_are_refused(tmp_path):
+def test_record_variables(tmp_path):
+    """Record (unlimited-dimension) variables: records interleave the record
+    variables, each padded to 4 bytes, except a lone record variable, which
+    is not padded (a byte variable is the case that shows it)."""
     p = str(tmp_path / "rec.nc")
-    with netcdf_file(p, "w") as f:
+    with netcdf_file(p, "w", version=2) as f:
         f.createDimension("t", None)
-        v = f.createVariable("x", np.float32, ("t",))
-        v[:3] = [1, 2, 3]
-    with pytest.raises(enkfio.EnkfioError) as e:
-        enkfio.NcReader(p)
-    assert e.value.code == enkfio.EFORMAT
+        f.createDimension("n", 3)
+        f.createVariable("x", np.float32, ("t",))[:3] = [1, 2, 3]
+        f.createVariable("b", np.int8, ("t", "n"))[:3] = np.arange(9).reshape(3, 3) - 4
+        f.createVariable("fixed", np.float64, ("n",))[...] = [0.5, 1.5, 2.5]
+    with enkfio.NcReader(p) as r:
+        assert r.dimlen("t") == 3
+        assert list(r.read("x")) == [1, 2, 3] and list(r.read("x", 1, 2)) == [2, 3]
+        assert np.array_equal(r.read("b"), np.arange(9).reshape(3, 3) - 4)
+        assert list(r.read("fixed")) == [0.5, 1.5, 2.5]
+    p1 = str(tmp_path / "rec1.nc")
+    with netcdf_file(p1, "w", version=1) as f:
+        f.createDimension("t", None)
+        f.createVariable("c", np.int8, ("t",))[:5] = [1, -2, 3, -4, 5]
+    with enkfio.NcReader(p1) as r:
+        assert list(r.read("c")) == [1, -2, 3, -4, 5]
+
+
+def write_obs_file(path, n, rng, structured=True, estd_orig=False, footprint=False, nobs_zero=False):
+    """observations.nc in the layout obs_write gives it (common/observations.c:807-1010)"""
+    n = 0 if nobs_zero else n
+    d = dict(id=np.arange(n, dtype=np.int32), id_orig=np.arange(n, dtype=np.int32) * 3,
+             status=rng.choice(np.array([0, 0, 0, 5, 2], np.int8), n), type=rng.integers(0, 3, n).astype(np.int16),
+             product=rng.integers(0, 2, n).astype(np.int16), instrument=rng.integers(0, 2, n).astype(np.int16),
+             fid=rng.integers(0, 4, n).astype(np.int16), batch=rng.integers(0, 9, n).astype(np.int32),
+             value=rng.standard_normal(n).astype(np.float32), estd=rng.uniform(0.1, 1, n).astype(np.float32),
+             lon=rng.uniform(0, 360, n).astype(np.float32), lat=rng.uniform(-80, 80, n).astype(np.float32),
+             depth=rng.uniform(0, 500, n).astype(np.float32), model_depth=rng.uniform(0, 5000, n).astype(np.float32),
+             fk=rng.uniform(0, 10, n).astype(np.float32), time=rng.uniform(-1, 1, n).astype(np.float32),
+             aux=rng.integers(0, 100, n).astype(np.int32))
+    if structured:
+        d.update(fi=rng.uniform(0, 100, n), fj=rng.uniform(0, 80, n))
+    else:
+        d.update(fi0=rng.uniform(0, 100, n), fi1=rng.uniform(0, 80, n), fi2=rng.uniform(0, 1, n))
+    if estd_orig:
+        d["estd_orig"] = (d["estd"] * 2).astype(np.float32)
+    if footprint:
+        d["footprint"] = rng.uniform(0, 50, n).astype(np.float32)
+    with netcdf_file(path, "w", version=2) as f:
+        f.createDimension("nobs", max(n, 1) if not nobs_zero else None)
+        f.DA_DAY = 25000.5
+        for k, a in d.items():
+            v = f.createVariable(k, a.dtype, ("nobs",))
+            if n:
+                v[...] = a
+            v.long_name = "observation " + k
+        t = f.variables["type"]
+        for i, name in enumerate(["SST", "SLA", "TEM"]):
+            setattr(t, name, np.int32(i))
+        setattr(t, "SLA:LOGAPPLIED", "true")
+    return d
+
+
+@pytest.mark.parametrize("structured,estd_orig,footprint", [(True, False, False), (False, True, True)])
+def test_read_observations_file(tmp_path, structured, estd_orig, footprint):
+    rng = np.random.default_rng(7)
+    p = str(tmp_path / "observations.nc")
+    d = write_obs_file(p, 57, rng, structured, estd_orig, footprint)
+    o = enkfio.read_obs(p)
+    assert o["nobs"] == 57 and o["da_day"] == 25000.5
+    assert o["typenames"] == ["SST", "SLA", "TEM"]
+    st = d["status"].astype(np.int16)
+    st[st == 5] = 0                               # STATUS_BADBATCH -> STATUS_OK
+    assert np.array_equal(o["status"], st)
+    for k in ("id", "id_orig", "type", "product", "instrument", "fid", "batch", "value", "lon", "lat", "depth",
+              "model_depth", "fk", "time", "aux"):
+        assert np.array_equal(o[k], d[k].astype(o[k].dtype)), k
+    assert np.array_equal(o["estd"], d["estd_orig"] if estd_orig else d["estd"])
+    if structured:
+        assert np.array_equal(o["fi"], d["fi"]) and np.array_equal(o["fj"], d["fj"]) and o["fi2"] is None
+    else:
+        assert np.array_equal(o["fi"], d["fi0"]) and np.array_equal(o["fj"], d["fi1"])
+        assert np.array_equal(o["fi2"], d["fi2"])
+    assert (o["footprint"] is not None) == footprint
+
+
+def test_read_observations_errors(tmp_path):
+    p = str(tmp_path / "bad.nc")
+    with netcdf_file(p, "w", version=2) as f:
+        f.createDimension("nobs", 3)
+        f.createVariable("type", np.int16, ("nobs",))[...] = [0, 1, 0]
+    with pytest.raises(enkfio.EnkfioError):
+        enkfio.read_obs(p)                        # no "id" and the rest
+
+
+def write_member(path, var, data, dims, rec=False, dtype=np.float32, **atts):
+    with netcdf_file(path, "w", version=2) as f:
+        for k, (name, n) in enumerate(zip(dims, data.shape)):
+            f.createDimension(name, None if (rec and k == 0) else n)
+        v = f.createVariable(var, dtype, dims)
+        for a, val in atts.items():
+            setattr(v, a, val)
+        if rec:
+            v[:data.shape[0]] = data.astype(dtype)
+            # a second record variable, so that records interleave
+            w = f.createVariable("other", np.int16, (dims[0],))
+            w[:data.shape[0]] = np.arange(data.shape[0], dtype=np.int16)
+        else:
+            v[...] = data.astype(dtype)
+
+
+def test_read_field_layouts_and_records(tmp_path):
+    rng = np.random.default_rng(9)
+    nk, nj, ni = 4, 6, 7
+    a4 = rng.standard_normal((3, nk, nj, ni)).astype(np.float32)
+    p = str(tmp_path / "mem001_temp.nc")
+    write_member(p, "temp", a4, ("time", "zt", "yt", "xt"), rec=True)
+    assert enkfio.field_nlevels(p, "temp") == nk
+    for r in range(3):
+        for k in (0, 3):
+            assert np.array_equal(enkfio.read_field(p, "temp", k, ni, nj, nk, r=r), a4[r, k])
+    assert np.array_equal(enkfio.read_field(p, "temp", 2, ni, nj, nk, r=1, j1=2, j2=4), a4[1, 2, 2:5])
+    with pytest.raises(enkfio.EnkfioError):
+        enkfio.read_field(p, "temp", 0, ni, nj, nk)           # three records: one must be named
+    with pytest.raises(enkfio.EnkfioError):
+        enkfio.read_field(p, "temp", 0, ni + 1, nj, nk, r=0)  # grid mismatch
+    # 3D without a record dimension; a 2D field; a single-layer 3D field
+    a3 = rng.standard_normal((nk, nj, ni))
+    p3 = str(tmp_path / "salt.nc")
+    write_member(p3, "salt", a3, ("z", "y", "x"), dtype=np.float64)
+    assert enkfio.field_nlevels(p3, "salt") == nk
+    assert np.array_equal(enkfio.read_field(p3, "salt", 1, ni, nj, nk), a3[1].astype(np.float32))
+    a2 = rng.standard_normal((nj, ni)).astype(np.float32)
+    p2 = str(tmp_path / "eta.nc")
+    write_member(p2, "eta", a2, ("y", "x"))
+    assert enkfio.field_nlevels(p2, "eta") == 1
+    assert np.array_equal(enkfio.read_field(p2, "eta", 0, ni, nj, nk), a2)
+    p1 = str(tmp_path / "one.nc")
+    write_member(p1, "t1", a3[:1], ("z", "y", "x"))
+    assert np.array_equal(enkfio.read_field(p1, "t1", nk - 1, ni, nj, nk), a3[0].astype(np.float32))
+
+
+def test_read_field_fill_missing_valid_scale_offset(tmp_path):
+    nj, ni = 5, 6
+    raw = (np.arange(nj * ni, dtype=np.int16) - 10).reshape(nj, ni)
+    raw[0, 0] = -999
+    raw[1, 1] = -32767                                        # default short fill, no _FillValue attribute
+    raw[2, 2] = 7777
+    p = str(tmp_path / "packed.nc")
+    write_member(p, "sla", raw, ("y", "x"), dtype=np.int16, scale_factor=np.float32(0.01),
+                 add_offset=np.float32(0.5), missing_value=np.int16(-999), valid_max=np.int16(1000))
+    v = enkfio.read_field(p, "sla", 0, ni, nj, 1)
+    ref = raw.astype(np.float32) * np.float32(0.01) + np.float32(0.5)
+    bad = np.zeros((nj, ni), bool)
+    bad[0, 0] = bad[1, 1] = bad[2, 2] = True
+    assert np.isnan(v[bad]).all()
+    assert np.array_equal(v[~bad], ref[~bad])
+    f = raw.astype(np.float32)
+    f[3, 3] = 1e30
+    p2 = str(tmp_path / "fillf.nc")
+    write_member(p2, "t", f, ("y", "x"), _FillValue=np.float32(1e30), valid_range=np.array([-20, 1e6], np.float32))
+    v2 = enkfio.read_field(p2, "t", 0, ni, nj, 1)
+    assert np.isnan(v2[3, 3]) and v2[4, 5] == 19.0            # _FillValue; an ordinary value
+    assert np.isnan(v2[raw < -20]).all()                      # below valid_range
+    assert not np.isnan(v2[(raw >= -20) & (np.arange(nj * ni).reshape(nj, ni) != 21)]).any()
