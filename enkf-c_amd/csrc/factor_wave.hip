@@ -9,7 +9,9 @@
 // form a 16 x 16 grid; lane (lr, lc) holds the MT x MT block of rows lr MT..,
 // columns lc MT.. of the padded 16 MT x 16 MT matrix in registers (padding
 // kept as identity, which every step leaves intact): 36 doubles per lane for
-// m = 96, so that four matrices fit on a CU.
+// m = 96, so that four matrices fit on a CU.  The symmetric-sweep kernels
+// below take m <= 160 (MT <= 10); the Gauss-Jordan kernel, which only the
+// adjoint uses, m <= 96.
 //
 // M^-1 comes from unblocked Gauss-Jordan elimination with diagonal pivots
 // (stable for SPD matrices, whose pivots are positive Schur complements):
@@ -333,6 +335,11 @@ __global__ void __launch_bounds__(256) factor_wave_kernel(CalcArgs a)
 // elimination.
 constexpr int kTriBlocks = 136;   // 16 * 17 / 2
 
+// waves per SIMD that the sweep kernels' registers allow: a lane holds its
+// MT x MT block and two MT-vectors in doubles (about 2 MT^2 + 4 MT + 20
+// VGPRs): 4 waves up to MT = 6 (m <= 96), 3 at MT = 7, 2 beyond
+__host__ __device__ constexpr int sweep_waves(int MT) { return MT <= 6 ? 4 : (MT == 7 ? 3 : 2); }
+
 // 1/d for a pivot d >= 1 (the pivots of I + S^T S are Schur complements of
 // at least 1): the hardware estimate refined by two Newton-Raphson steps
 // r <- r + r (1 - d r), within an ulp of 1/d
@@ -360,7 +367,7 @@ __device__ inline double group_sum3(double v, double* tmp, int lane)
 }
 
 template <int MT>
-__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) factor_trace_kernel(CalcArgs a)
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(sweep_waves(MT)))) factor_trace_kernel(CalcArgs a)
 {
     constexpr int MP = 16 * MT;
     __shared__ double col[2][MP];
@@ -440,8 +447,14 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
 #pragma unroll
         for (int rk = 0; rk < MT; ++rk) {
             const int k = kb * MT + rk;
-            if (k >= m)
-                break;   // the remaining pivots are identity padding
+            if (k >= m) {
+                // the remaining pivots are identity padding.  Past 8 steps a
+                // loop with an early exit is not unrolled (and the blocks
+                // would go to scratch): such loops skip instead
+                if (MT <= 8)
+                    break;
+                continue;
+            }
             __syncthreads();
             const int buf = k & 1;   // pivot k reads buffer k % 2 (a constant per unrolled step for even MT)
             const double d = col[buf][k];
@@ -506,7 +519,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
 //         block order by one lane per row;
 //   dfs = m - tr(M^-1), srf = sqrt(tr(G0) / dfs) - 1.
 template <int MT>
-__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) factor_sym_kernel(CalcArgs a)
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(sweep_waves(MT)))) factor_sym_kernel(CalcArgs a)
 {
     constexpr int MP = 16 * MT;
     __shared__ double col[2][MP];
@@ -615,8 +628,14 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
 #pragma unroll
         for (int rk = 0; rk < MT; ++rk) {
             const int k = kb * MT + rk;
-            if (k >= m)
-                break;   // the remaining pivots are identity padding
+            if (k >= m) {
+                // the remaining pivots are identity padding.  Past 8 steps a
+                // loop with an early exit is not unrolled (and the blocks
+                // would go to scratch): such loops skip instead
+                if (MT <= 8)
+                    break;
+                continue;
+            }
             __syncthreads();
             const int buf = k & 1;   // pivot k reads buffer k % 2 (a constant per unrolled step for even MT)
             const double d = col[buf][k];
@@ -753,7 +772,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(4))) f
 bool factor_wave_supported(const CalcArgs& a)
 {
     const bool etkf = a.scheme == ENKFG_SCHEME_ETKF && a.mode != ENKFG_MODE_ENOI;
-    return !etkf && a.m <= 96;
+    return !etkf && a.m <= 160;
 }
 
 hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a)
@@ -781,6 +800,10 @@ hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a)
         case 4: hipLaunchKernelGGL(factor_sym_kernel<4>, grid, dim3(192), 0, s, a); break;
         case 5: hipLaunchKernelGGL(factor_sym_kernel<5>, grid, dim3(192), 0, s, a); break;
         case 6: hipLaunchKernelGGL(factor_sym_kernel<6>, grid, dim3(192), 0, s, a); break;
+        case 7: hipLaunchKernelGGL(factor_sym_kernel<7>, grid, dim3(192), 0, s, a); break;
+        case 8: hipLaunchKernelGGL(factor_sym_kernel<8>, grid, dim3(192), 0, s, a); break;
+        case 9: hipLaunchKernelGGL(factor_sym_kernel<9>, grid, dim3(192), 0, s, a); break;
+        case 10: hipLaunchKernelGGL(factor_sym_kernel<10>, grid, dim3(192), 0, s, a); break;
         default: return hipErrorInvalidValue;
         }
     }
@@ -796,6 +819,10 @@ hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a)
     case 4: hipLaunchKernelGGL(factor_trace_kernel<4>, tgrid, tblock, 0, s, a); break;
     case 5: hipLaunchKernelGGL(factor_trace_kernel<5>, tgrid, tblock, 0, s, a); break;
     case 6: hipLaunchKernelGGL(factor_trace_kernel<6>, tgrid, tblock, 0, s, a); break;
+    case 7: hipLaunchKernelGGL(factor_trace_kernel<7>, tgrid, tblock, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(factor_trace_kernel<8>, tgrid, tblock, 0, s, a); break;
+    case 9: hipLaunchKernelGGL(factor_trace_kernel<9>, tgrid, tblock, 0, s, a); break;
+    case 10: hipLaunchKernelGGL(factor_trace_kernel<10>, tgrid, tblock, 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

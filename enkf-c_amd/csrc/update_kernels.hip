@@ -179,20 +179,72 @@ __global__ void __launch_bounds__(64 * NWT) update_bg_kernel(UpdArgs a)
     }
 }
 
-// das_sethybridensemble on E[field][cell][m]
-__global__ void hybrid_condition_kernel(UpdArgs a, double k_d, double k_s)
+// das_sethybridensemble on E[field][cell][m]: one wave stages kHybRecs
+// consecutive records (one record = one cell of one field, m contiguous
+// floats) in LDS with coalesced loads (float4 when VEC), rows padded to
+// m + 1 words; lane r < kHybRecs forms the FP64 mean of record r's dynamic
+// members in member order, as the reference does; then the lanes rewrite the
+// staged span with coalesced stores:
+//   dynamic: (x - mean) * k_d + mean,  static: x * k_s + mean  (in double)
+constexpr int kHybRecs = 32;
+
+template <bool VEC>
+__global__ void __launch_bounds__(64) hybrid_condition_kernel(float* __restrict__ E, long long nrec, int m, int md,
+                                                              double k_d, double k_s)
 {
-    const long long n = (long long)a.nfield * a.ncell;
-    for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
-        float* rec = a.E + r * a.m;
+    extern __shared__ float tile[];
+    __shared__ double smean[kHybRecs];
+    const long long r0 = (long long)blockIdx.x * kHybRecs;
+    const int nr = (int)((nrec - r0) < kHybRecs ? (nrec - r0) : kHybRecs);
+    float* span = E + r0 * m;
+    const int ld = m + 1;
+    const int n = nr * m;
+    const int lane = threadIdx.x;
+    if (VEC) {
+        const float4* s4 = reinterpret_cast<const float4*>(span);
+        for (int k4 = lane; k4 < n / 4; k4 += 64) {
+            const float4 x = s4[k4];
+            const int k = 4 * k4, r = k / m, e = k - r * m;   // m % 4 == 0: the four stay in record r
+            float* t = tile + r * ld + e;
+            t[0] = x.x;
+            t[1] = x.y;
+            t[2] = x.z;
+            t[3] = x.w;
+        }
+    } else {
+        for (int k = lane; k < n; k += 64) {
+            const int r = k / m;
+            tile[r * ld + (k - r * m)] = span[k];
+        }
+    }
+    __syncthreads();
+    if (lane < nr) {
+        const float* t = tile + lane * ld;
         double mean = 0.0;
-        for (int e = 0; e < a.md; ++e)
-            mean += (double)rec[e];
-        mean /= (double)a.md;
-        for (int e = 0; e < a.md; ++e)
-            rec[e] = (float)(((double)rec[e] - mean) * k_d + mean);
-        for (int e = a.md; e < a.m; ++e)
-            rec[e] = (float)((double)rec[e] * k_s + mean);
+        for (int e = 0; e < md; ++e)
+            mean += (double)t[e];
+        smean[lane] = mean / (double)md;
+    }
+    __syncthreads();
+    if (VEC) {
+        float4* s4 = reinterpret_cast<float4*>(span);
+        for (int k4 = lane; k4 < n / 4; k4 += 64) {
+            const int k = 4 * k4, r = k / m, e = k - r * m;
+            const float* t = tile + r * ld + e;
+            const double mean = smean[r];
+            float v[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                v[c] = (e + c < md) ? (float)(((double)t[c] - mean) * k_d + mean)
+                                    : (float)((double)t[c] * k_s + mean);
+            s4[k4] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    } else {
+        for (int k = lane; k < n; k += 64) {
+            const int r = k / m, e = k - r * m;
+            const double x = (double)tile[r * ld + e], mean = smean[r];
+            span[k] = (e < md) ? (float)((x - mean) * k_d + mean) : (float)(x * k_s + mean);
+        }
     }
 }
 
@@ -282,12 +334,19 @@ hipError_t launch_hybrid_condition(hipStream_t s, const UpdArgs& a, double gamma
     const double k_d = (a.md > 1) ? sqrt((double)(a.m - 1) / (double)(a.md - 1)) : 0.0;
     const double k_s = sqrt(gamma * (double)(a.m - 1) / (double)(ms - 1));
     const long long n = (long long)a.nfield * a.ncell;
-    long long nb = (n + 255) / 256;
-    if (nb > 65536)
-        nb = 65536;
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(hybrid_condition_kernel, dim3((unsigned)nb), dim3(256), 0, s, a, k_d, k_s);
+    const long long nb = (n + kHybRecs - 1) / kHybRecs;
+    if (nb > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    const size_t lds = (size_t)kHybRecs * (a.m + 1) * sizeof(float);
+    // float4 staging needs 16-byte aligned spans: m % 4 == 0 and E aligned
+    if (a.m % 4 == 0 && (reinterpret_cast<uintptr_t>(a.E) & 15) == 0)
+        hipLaunchKernelGGL(hybrid_condition_kernel<true>, dim3((unsigned)nb), dim3(64), lds, s, a.E, n, a.m, a.md,
+                           k_d, k_s);
+    else
+        hipLaunchKernelGGL(hybrid_condition_kernel<false>, dim3((unsigned)nb), dim3(64), lds, s, a.E, n, a.m, a.md,
+                           k_d, k_s);
     return hipGetLastError();
 }
 

@@ -245,6 +245,11 @@ CASES = [
     (enkfg.MODE_HYBRID, enkfg.SCHEME_DENKF, 144, 96, 1.0, 20),    # 96 dynamic + 48 static members
     (enkfg.MODE_HYBRID, enkfg.SCHEME_ETKF, 144, 96, 1.0, 20),     # ETKF, m > 96: iterates in device memory
     (enkfg.MODE_ENKF, enkfg.SCHEME_ETKF, 160, 160, 0.8, 10),      # the largest supported m
+    # the symmetric sweep past m = 96: 7, 8 and 10 blocks of 16 per side
+    (enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, 100, 100, 1.0, 10),
+    (enkfg.MODE_HYBRID, enkfg.SCHEME_DENKF, 128, 80, 0.7, 10),
+    (enkfg.MODE_ENOI, enkfg.SCHEME_DENKF, 150, 150, 1.0, 10),
+    (enkfg.MODE_ENKF, enkfg.SCHEME_DENKF, 160, 160, 1.0, 10),
 ]
 
 
