@@ -427,6 +427,14 @@ __global__ void __launch_bounds__(kScanThreads) scan_final_kernel(const int* __r
 // each list into its place in the CSR arrays.  A walk that runs out of pool
 // raises the overflow flag; the host then redoes the localisation with the
 // two walks.  Types with NLOBSMAX take the two-walk path.
+//
+// Each thread keeps its last kLocBurst entries in registers and writes them
+// as one aligned run (32 bytes of ids, 64 of tapers): single 4- and 8-byte
+// stores from 64 threads walking different lists leave their cache lines
+// part-written, and such lines are written back to memory several times over.
+constexpr int kLocBurst = 8;
+static_assert(kLocChunk % kLocBurst == 0, "bursts must not straddle chunks");
+
 __global__ void __launch_bounds__(64) loc_walk_kernel(int nnode, int ni, const double* __restrict__ qxyz,
                                                        const unsigned char* __restrict__ isglobal,
                                                        const unsigned char* __restrict__ visible,
@@ -451,32 +459,66 @@ __global__ void __launch_bounds__(64) loc_walk_kernel(int nnode, int ni, const d
             n = ti.nids;   // obs_getglobal: all obs of the type, no status check (written by the compaction)
             head = -2;
         } else if (ti.nnode > 0) {
-            int cur = -1, pos = kLocChunk;
+            int cur = -1, pos = kLocChunk;   // pos: entries of chunk cur, written or in the burst
+            int nb = 0;                      // entries in the burst
+            int bid[kLocBurst];
+            double blc[kLocBurst];
+#pragma unroll
+            for (int b = 0; b < kLocBurst; ++b) {
+                bid[b] = 0;
+                blc[b] = 0.0;
+            }
             bool full = false;
             range_walk(nodes + ti.node0, ti.ndim, q, ti.range, [&](int v, double distsq) {
                 const int row = ti.node0 + v;   // device row of the node's observation
                 if (status[row] != kStatusOK)
                     return;
-                if (pos == kLocChunk && !full) {
+                n++;
+                if (pos == kLocChunk) {   // the burst is empty here (chunks hold whole bursts)
+                    if (full)
+                        return;
                     const int nc = atomicAdd(pool, 1);
                     if (nc >= pool_cap) {
                         full = true;
-                    } else {
-                        if (cur >= 0)
-                            cnext[cur] = nc;
-                        else
-                            head = nc;
-                        cur = nc;
-                        pos = 0;
+                        return;
                     }
+                    if (cur >= 0)
+                        cnext[cur] = nc;
+                    else
+                        head = nc;
+                    cur = nc;
+                    pos = 0;
                 }
-                if (!full) {
-                    cids[(size_t)cur * kLocChunk + pos] = row;
-                    clc[(size_t)cur * kLocChunk + pos] = calclcoeff(ti, sqrt(distsq));
-                    pos++;
+                const double c = calclcoeff(ti, sqrt(distsq));
+#pragma unroll
+                for (int b = 0; b < kLocBurst; ++b) {
+                    bid[b] = (b == nb) ? row : bid[b];
+                    blc[b] = (b == nb) ? c : blc[b];
                 }
-                n++;
+                ++nb;
+                ++pos;
+                if (nb == kLocBurst) {
+                    const size_t o = (size_t)cur * kLocChunk + (pos - kLocBurst);   // a multiple of kLocBurst
+                    int4* di = reinterpret_cast<int4*>(cids + o);
+                    double2* dl = reinterpret_cast<double2*>(clc + o);
+#pragma unroll
+                    for (int b = 0; b < kLocBurst; b += 4)
+                        di[b / 4] = make_int4(bid[b], bid[b + 1], bid[b + 2], bid[b + 3]);
+#pragma unroll
+                    for (int b = 0; b < kLocBurst; b += 2)
+                        dl[b / 2] = make_double2(blc[b], blc[b + 1]);
+                    nb = 0;
+                }
             });
+            if (nb > 0) {
+                const size_t o = (size_t)cur * kLocChunk + (pos - nb);
+#pragma unroll
+                for (int b = 0; b < kLocBurst; ++b)
+                    if (b < nb) {
+                        cids[o + b] = bid[b];
+                        clc[o + b] = blc[b];
+                    }
+            }
             if (cur >= 0)
                 cnext[cur] = -1;
             if (full)
