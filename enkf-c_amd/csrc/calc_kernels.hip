@@ -86,256 +86,13 @@ __global__ void permute_rows_kernel(const T* __restrict__ src, const int* __rest
     }
 }
 
-// ------------------------------------------------------------------ Gram (MFMA)
-
-// Chunks of KC local obs are staged in LDS as tapered FP64 rows,
-// (double) S * lc as the reference forms Sloc (double buffered: the gathers
-// of chunk c+1, and the list entries of chunk c+2, are in flight while chunk
-// c feeds the MFMAs), so that an MFMA operand is one LDS read.  Row stride RS
-// (doubles) is congruent to 16 mod 32, so the four rows a k-step reads fall
-// in disjoint LDS banks.
-constexpr int KC = 32;
-// the last chunk c of a segment loads list entries up to (c + 4) KC - 1, at
-// most 4 KC - 1 past the segment's end
-static_assert(kListPad >= 4 * KC, "list padding too small for the FP64 Gram pipeline");
-template <int MT>
-struct GramLds {
-    static constexpr int MP = 16 * MT;
-    static constexpr int RS = MP + ((16 - MP % 32) + 32) % 32;
-    alignas(16) double rows[2][KC + 1][RS];   // (double) S * lc of chunk c in rows[c % 2]; row KC: discard
-    double sl[2][KC];         // s_o * lc_o of the chunk's rows, chunk c in [c % 2]
-    double lc[3][KC];         // taper coefficients, chunk c in [c % 3]
-    int id[3][KC];            // obs ids (-1 past the end), chunk c in [c % 3]
-};
-
-// Threads t and t + KC hold the same entry, so every thread stores: the
-// stores are unconditional (a store under a branch leaves its load counted
-// as in flight past the branch, and the loads after it wait for it).
-template <int MT>
-__device__ inline void chunk_meta_store(GramLds<MT>& sh, int buf, int id, double c)
-{
-    sh.id[buf][threadIdx.x & (KC - 1)] = id;
-    sh.lc[buf][threadIdx.x & (KC - 1)] = c;
-}
-
-// rows of a chunk into row buffer rbuf, tapered in FP64 as the reference
-// forms Sloc ((double) S * lc, calc/transforms.c:794-802); s * lc with the
-// tapers of meta buffer mbuf
-template <int MT>
-__device__ inline void chunk_store(const CalcArgs& a, GramLds<MT>& sh, int rbuf, int mbuf, const GramRegs<MT, KC>& L)
-{
-    const int qr = a.ms >> 2;
-#pragma unroll
-    for (int j = 0; j < GramRegs<MT, KC>::NQ; ++j) {
-        const int q = (int)threadIdx.x + 256 * j;
-        const int row = q / qr;
-        const int qd = q - row * qr;
-        // quads past the chunk go to the discard row, unconditionally (as
-        // in chunk_meta_store)
-        const int rr = row < KC ? row : KC;
-        const double c = sh.lc[mbuf][rr < KC ? rr : 0];
-        double2* dst = reinterpret_cast<double2*>(&sh.rows[rbuf][rr][4 * qd]);
-        dst[0] = double2{(double)L.v[j].x * c, (double)L.v[j].y * c};
-        dst[1] = double2{(double)L.v[j].z * c, (double)L.v[j].w * c};
-    }
-    sh.sl[rbuf][threadIdx.x & (KC - 1)] = L.s * sh.lc[mbuf][threadIdx.x & (KC - 1)];
-}
-
-// acc[T] += frag[I] frag[J]^T for the tiles (I, J) of wave W, with the tile
-// coordinates compile-time constants (register arrays stay in registers)
-template <int MT, int W, int T>
-__device__ inline void mfma_tiles(const double (&frag)[MT], dbl4* acc)
-{
-    if constexpr (W + 4 * T < tri_count(MT)) {
-        constexpr int f = W + 4 * T;
-        constexpr int I = tri_I(MT, f), J = tri_J(MT, f);
-        acc[T] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[I], frag[J], acc[T], 0, 0, 0);
-        mfma_tiles<MT, W, T + 1>(frag, acc);
-    }
-}
-
-template <int MT, int W>
-__device__ inline void gram_wave_chunk(const GramLds<MT>& sh, int buf, int mbuf, dbl4 (&acc)[(tri_count(MT) - W + 3) / 4 > 0 ? (tri_count(MT) - W + 3) / 4 : 1], double (&sts)[MT])
-{
-    const int lane = threadIdx.x & 63;
-    const int kq = lane >> 4;
-    const int col = lane & 15;
-#pragma unroll 2
-    for (int s = 0; s < KC / 4; ++s) {
-        const int r = 4 * s + kq;
-        double frag[MT];
-#pragma unroll
-        for (int X = 0; X < MT; ++X)
-            frag[X] = sh.rows[buf][r][16 * X + col];
-        mfma_tiles<MT, W, 0>(frag, acc);
-        if (W == 0) {
-            const double sl = sh.sl[buf][r];
-#pragma unroll
-            for (int X = 0; X < MT; ++X)
-                sts[X] += frag[X] * sl;
-        }
-    }
-}
-
-// One iteration of the chunk pipeline (chunk c): Lg receives the gathers of
-// chunk c+2, Ls holds chunk c+1.  The loads are unconditional: past the end
-// they read the padding of the lists and the zero row.
-template <int MT, int W>
-__device__ inline void gram_step(const CalcArgs& a, long long off, int cnt, int nchunk, int c, GramLds<MT>& sh,
-                                 GramRegs<MT, KC>& Lg, const GramRegs<MT, KC>& Ls,
-                                 dbl4 (&acc)[(tri_count(MT) - W + 3) / 4 > 0 ? (tri_count(MT) - W + 3) / 4 : 1],
-                                 double (&sts)[MT])
-{
-    int mid;
-    double mc;
-    // list entries first: the wait for them at the end of the step then
-    // leaves the younger gathers in flight
-    gram_meta_load<KC>(a, off, cnt, (c + 3) * KC, mid, mc);
-    gram_gather<MT, KC>(a, sh.id[(c + 2) % 3], Lg);
-    // keep the loads ahead of the MFMAs (the scheduler would sink them to
-    // their use, exposing their latency every chunk)
-    __builtin_amdgcn_sched_barrier(0);
-    gram_wave_chunk<MT, W>(sh, c & 1, c % 3, acc, sts);
-    // Both stores are unconditional, so that no load of this step or the
-    // previous one is in flight on any path into the next step (the wait
-    // counts are computed over all paths); past the segment's end they store
-    // zero rows and entries with id -1, taper 0, which nothing reads.  Meta
-    // buffer c % 3 (chunk c's) is free: chunk c's rows were stored in the
-    // previous step.
-    chunk_store<MT>(a, sh, (c + 1) & 1, (c + 1) % 3, Ls);
-    chunk_meta_store<MT>(sh, c % 3, mid, mc);
-    __syncthreads();
-}
-
-template <int MT, int W>
-__device__ inline void gram_wave(const CalcArgs& a, long long off, int cnt, double* __restrict__ gout,
-                                 double* __restrict__ stsout, GramLds<MT>& sh)
-{
-    constexpr int NT = tri_count(MT);
-    constexpr int NTW = (NT - W + 3) / 4;   // tiles of this wave
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int kq = lane >> 4;   // which of the 4 obs of a k-step
-    const int col = lane & 15;
-
-    dbl4 acc[NTW > 0 ? NTW : 1];
-#pragma unroll
-    for (int t = 0; t < NTW; ++t)
-        acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-    double sts[MT];
-#pragma unroll
-    for (int X = 0; X < MT; ++X)
-        sts[X] = 0.0;
-
-    // zero both row buffers once: padding columns stay zero
-    for (int q = tid; q < 2 * (KC + 1) * GramLds<MT>::RS; q += 256)
-        (&sh.rows[0][0][0])[q] = 0.0;
-    const int nchunk = (cnt + KC - 1) / KC;
-    int mid;
-    double mc;
-    // prologue: list entries of chunks 0, 1 and 2; rows of chunk 0 in LDS,
-    // gathers of chunk 1 in flight
-    gram_meta_load<KC>(a, off, cnt, 0, mid, mc);
-    chunk_meta_store<MT>(sh, 0, mid, mc);
-    gram_meta_load<KC>(a, off, cnt, KC, mid, mc);
-    chunk_meta_store<MT>(sh, 1, mid, mc);
-    gram_meta_load<KC>(a, off, cnt, 2 * KC, mid, mc);
-    chunk_meta_store<MT>(sh, 2, mid, mc);
-    __syncthreads();
-    GramRegs<MT, KC> L0, L1;
-    gram_gather<MT, KC>(a, sh.id[0], L0);
-    chunk_store<MT>(a, sh, 0, 0, L0);
-    gram_gather<MT, KC>(a, sh.id[1], L1);
-    __syncthreads();
-    // iteration c: the gathers of chunk c+2 (into one register set) and the
-    // list entries of chunk c+3 are issued, chunk c feeds the MFMAs, and
-    // chunk c+1 (gathered during iteration c-1, in the other register set)
-    // is stored; a gather thus has two chunks of MFMA work to land.  The
-    // loop is unrolled by two so that the register sets are never indexed
-    // at run time; one barrier per chunk.
-    // (the odd last chunk is peeled off, so that the loop's back edge is
-    // reached from the second step only and the wait counts at its top see
-    // no gathers into the first register set in flight)
-    int c = 0;
-    for (; c + 1 < nchunk; c += 2) {
-        gram_step<MT, W>(a, off, cnt, nchunk, c, sh, L0, L1, acc, sts);
-        gram_step<MT, W>(a, off, cnt, nchunk, c + 1, sh, L1, L0, acc, sts);
-    }
-    if (c < nchunk)
-        gram_step<MT, W>(a, off, cnt, nchunk, c, sh, L0, L1, acc, sts);
-    // Operand layouts of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l
-    // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register
-    // i of lane l holds D[4 i + l / 16][l % 16].
-    // Stored in the factor kernel's register-block layout (gram_index), both
-    // triangles.
-#pragma unroll
-    for (int t = 0; t < NTW; ++t) {
-        const int f = W + 4 * t;
-        const int I = tri_I(MT, f), J = tri_J(MT, f);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = 16 * I + 4 * i + kq;
-            const int cc = 16 * J + col;
-            gout[gram_index<MT>(r, cc)] = acc[t][i];
-            if (I != J)
-                gout[gram_index<MT>(cc, r)] = acc[t][i];
-        }
-    }
-    if (W == 0) {
-#pragma unroll
-        for (int X = 0; X < MT; ++X) {
-            double v = sts[X];
-            v += __shfl_xor(v, 16);
-            v += __shfl_xor(v, 32);
-            if (kq == 0)
-                stsout[16 * X + col] = v;
-        }
-    }
-}
-
-template <int MT>
-__global__ void __launch_bounds__(256) gram_kernel(CalcArgs a)
-{
-    constexpr int MP = 16 * MT;
-    // blocks are dispatched to the 8 XCDs round-robin; give each XCD a
-    // contiguous range of (node, segment) work items so that neighbouring
-    // nodes, whose local obs mostly coincide, share that XCD's L2
-    const int nwork = a.nbatch * a.nseg;
-    const int per = (nwork + 7) / 8;
-    const int b = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (b >= nwork)
-        return;
-    const int node = a.node_list ? a.node_list[a.node0 + b / a.nseg] : a.node0 + b / a.nseg;
-    const int seg = b % a.nseg;
-    long long off;
-    int cnt;
-    if (a.pertype) {
-        off = a.offsets[(size_t)node * a.ntypes + seg];
-        cnt = a.counts[(size_t)node * a.ntypes + seg];
-    } else {
-        off = a.offsets[(size_t)node * a.ntypes];
-        cnt = (int)(a.offsets[(size_t)(node + 1) * a.ntypes] - off);
-    }
-    if (cnt == 0)
-        return;
-    __shared__ GramLds<MT> sh;
-    double* gout = a.gram + (size_t)b * MP * MP;
-    double* stsout = a.sts + (size_t)b * MP;
-    switch (threadIdx.x >> 6) {
-    case 0: gram_wave<MT, 0>(a, off, cnt, gout, stsout, sh); break;
-    case 1: gram_wave<MT, 1>(a, off, cnt, gout, stsout, sh); break;
-    case 2: gram_wave<MT, 2>(a, off, cnt, gout, stsout, sh); break;
-    default: gram_wave<MT, 3>(a, off, cnt, gout, stsout, sh); break;
-    }
-}
-
 // ------------------------------------------------------------------ Gram, independent wavefronts
 //
 // Each wavefront computes one group of upper-triangle 16 x 16 tiles (tiles
 // f = g, g + NG, ... of tri_count(MT); NG = 3 groups of 7 tiles or NG = 2
-// groups of 11 and 10 for m = 96, chosen per obs-count bucket by the
-// host's autotuner) of one (node, obstype segment), with no workgroup
-// barrier: the wave
+// groups of 11 and 10 for m = 96, 4 or 5 groups of the 45 tiles for m = 144:
+// item_groups_lo, chosen per obs-count bucket by the host's autotuner) of
+// one (node, obstype segment), with no workgroup barrier: the wave
 // stages its own chunks of IKC local obs through a private LDS area, so the
 // waves of one segment share nothing but L2 (the other waves' gathers of the
 // same rows hit there).  Per chunk: the gathers of chunk c+1 and the
@@ -1230,44 +987,37 @@ hipError_t launch_scatter_rows(hipStream_t s, const double* src, const int* perm
 template <int M>
 void launch_gram_item(hipStream_t s, dim3 grid, const CalcArgs& a, int ng)
 {
-    if (ng == 2)
-        hipLaunchKernelGGL((gram_item_kernel<M, 2>), grid, dim3(64), 0, s, a);
+    constexpr int lo = item_groups_lo(M);
+    if (ng == lo)
+        hipLaunchKernelGGL((gram_item_kernel<M, lo>), grid, dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL((gram_item_kernel<M, 3>), grid, dim3(64), 0, s, a);
+        hipLaunchKernelGGL((gram_item_kernel<M, lo + 1>), grid, dim3(64), 0, s, a);
 }
 
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a)
 {
     const int MT = (a.m + 15) / 16;
-    dim3 grid((unsigned)(8 * ((a.nbatch * a.nseg + 7) / 8)));
     if (a.nbatch == 0)
         return hipSuccess;
-    if (MT <= 6) {
-        const int ng = (a.tile_groups == 2) ? 2 : 3;
-        const long long nitem = (long long)a.nbatch * a.nseg * ng;
-        const dim3 igrid((unsigned)(8 * ((nitem + 7) / 8)));
-        switch (MT) {
-        case 1: launch_gram_item<1>(s, igrid, a, ng); break;
-        case 2: launch_gram_item<2>(s, igrid, a, ng); break;
-        case 3: launch_gram_item<3>(s, igrid, a, ng); break;
-        case 4: launch_gram_item<4>(s, igrid, a, ng); break;
-        case 5: launch_gram_item<5>(s, igrid, a, ng); break;
-        default: launch_gram_item<6>(s, igrid, a, ng); break;
-        }
-        return hipGetLastError();
-    }
+    if (MT < 1 || MT > 10)
+        return hipErrorInvalidValue;
+    // tile groups per segment: one of the two candidates of this MT (the
+    // larger by default)
+    const int lo = item_groups_lo(MT);
+    const int ng = (a.tile_groups == lo) ? lo : lo + 1;
+    const long long nitem = (long long)a.nbatch * a.nseg * ng;
+    const dim3 igrid((unsigned)(8 * ((nitem + 7) / 8)));
     switch (MT) {
-    case 1: hipLaunchKernelGGL(gram_kernel<1>, grid, dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(gram_kernel<2>, grid, dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(gram_kernel<3>, grid, dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(gram_kernel<4>, grid, dim3(256), 0, s, a); break;
-    case 5: hipLaunchKernelGGL(gram_kernel<5>, grid, dim3(256), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(gram_kernel<6>, grid, dim3(256), 0, s, a); break;
-    case 7: hipLaunchKernelGGL(gram_kernel<7>, grid, dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(gram_kernel<8>, grid, dim3(256), 0, s, a); break;
-    case 9: hipLaunchKernelGGL(gram_kernel<9>, grid, dim3(256), 0, s, a); break;
-    case 10: hipLaunchKernelGGL(gram_kernel<10>, grid, dim3(256), 0, s, a); break;
-    default: return hipErrorInvalidValue;
+    case 1: launch_gram_item<1>(s, igrid, a, ng); break;
+    case 2: launch_gram_item<2>(s, igrid, a, ng); break;
+    case 3: launch_gram_item<3>(s, igrid, a, ng); break;
+    case 4: launch_gram_item<4>(s, igrid, a, ng); break;
+    case 5: launch_gram_item<5>(s, igrid, a, ng); break;
+    case 6: launch_gram_item<6>(s, igrid, a, ng); break;
+    case 7: launch_gram_item<7>(s, igrid, a, ng); break;
+    case 8: launch_gram_item<8>(s, igrid, a, ng); break;
+    case 9: launch_gram_item<9>(s, igrid, a, ng); break;
+    default: launch_gram_item<10>(s, igrid, a, ng); break;
     }
     return hipGetLastError();
 }

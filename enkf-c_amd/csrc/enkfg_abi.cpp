@@ -13,6 +13,7 @@
 
 #include "enkfg.h"
 #include "enkfg_internal.h"
+#include "gram_layout.h"
 
 // Scratch of enkfg_update_fields_noalloc: a file-scope device array, one copy
 // per GPU, held until the process ends.  Host code never uses its name as an
@@ -1157,8 +1158,9 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     for (int q = 0; q < kBuckets; ++q)
         c->bucket_nodes[q] = bcount[q];
 
-    // Autotuner: the item Gram kernels (m <= 96) run a bucket's segments
-    // with 2 or 3 tile groups (wavefronts) per segment.  The first batch of
+    // Autotuner: the item Gram kernels run a bucket's segments with one of
+    // two tile-group counts (wavefronts) per segment, item_groups_lo(MT) or
+    // one more (2 or 3 for m <= 96, 4 or 5 for m = 144).  The first batch of
     // a bucket that is not yet tuned for this precision and holds at least
     // kTuneMin nodes runs both and keeps the faster (both write the same
     // Gram matrices, bit for bit); the choice holds for the engine's life.
@@ -1166,11 +1168,12 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     // buckets share one choice, the nodes run in plain grid order (better L2
     // reuse between neighbours) with that choice.
     const bool mixed = c->cfg.precision == ENKFG_PREC_MIXED;
-    const bool items = (m + 15) / 16 <= 6 &&
+    const bool items = (m + 15) / 16 <= 10 &&
                        (!mixed || (unsigned long long)(c->nobs + 1) * (unsigned long long)a.ms * 4ull < (1ull << 32));
     const int kTuneMin = std::min(nbatch, 256);
     int* tg = c->tile_groups[mixed ? 1 : 0];
-    const int tg_default = mixed ? 2 : 3;
+    const int tg_lo = item_groups_lo((m + 15) / 16);
+    const int tg_default = mixed ? tg_lo : tg_lo + 1;
     bool plain = !items;
     if (items) {
         bool tuned = true, same = true;
@@ -1330,7 +1333,7 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
             // the default shape first; the other is kept only when it is
             // clearly faster (3%), so that run-to-run noise does not flip
             // the choice
-            const int cands[2] = {tg_default, 5 - tg_default};
+            const int cands[2] = {tg_default, 2 * tg_lo + 1 - tg_default};
             float ms[2] = {0.0f, 0.0f};
             hipEvent_t t0 = c->bev[4 * (size_t)nb], t1 = c->bev[4 * (size_t)nb + 1];
             for (int k = 0; k < 2; ++k) {

@@ -392,6 +392,19 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(sweep_
     }
     if (lane == 0)
         a.pnlobs[o] = pt;
+    // every local obs of the node is of this type: M_t is the node's own M,
+    // whose diagnostics factor_sym_kernel (earlier on this stream) has
+    // written; the reference's re-solve works on the same matrix
+    const long long p = a.offsets[(size_t)(node + 1) * nt] - a.offsets[(size_t)node * nt];
+    if (pt == p) {
+        if (lane == 0) {
+            const size_t ot = (size_t)nt * a.nnode + node;
+            const float fd = a.pdfs[ot];
+            a.pdfs[o] = fd;
+            a.psrf[o] = ((double)fd > kDfsMin) ? a.psrf[ot] : 0.0f;
+        }
+        return;
+    }
 
     // block (br, bc), br >= bc, of lane br (br + 1) / 2 + bc; lanes past the
     // last block compute on block (0, 0)'s column data but never publish,

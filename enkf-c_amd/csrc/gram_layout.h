@@ -12,6 +12,15 @@ namespace enkfg {
 // row by row; wave w of the 4 waves of a Gram workgroup owns tiles w, w + 4,
 // w + 8, ...
 __host__ __device__ constexpr int tri_count(int MT) { return MT * (MT + 1) / 2; }
+
+// The item Gram kernels split a segment's tri_count(MT) tiles into NG groups,
+// one wavefront each; the two candidates per MT are lo and lo + 1, lo chosen
+// so that a group's accumulators and operands fit the 256 registers of a
+// lane (21 tiles: 2 or 3 groups; 28: 3 or 4; 36 and 45: 4 or 5; 55: 5 or 6)
+__host__ __device__ constexpr int item_groups_lo(int MT)
+{
+    return MT <= 6 ? 2 : (MT == 7 ? 3 : (MT <= 9 ? 4 : 5));
+}
 __host__ __device__ constexpr int tri_I(int MT, int f)
 {
     int I = 0;

@@ -564,10 +564,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gr
 template <int M>
 void launch_mix_item(hipStream_t s, dim3 grid, const CalcArgs& a, int ng)
 {
-    if (ng == 2)
-        hipLaunchKernelGGL((gram_mixed_item_kernel<M, 2>), grid, dim3(64), 0, s, a);
+    constexpr int lo = item_groups_lo(M);
+    if (ng == lo + 1)
+        hipLaunchKernelGGL((gram_mixed_item_kernel<M, lo + 1>), grid, dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL((gram_mixed_item_kernel<M, 3>), grid, dim3(64), 0, s, a);
+        hipLaunchKernelGGL((gram_mixed_item_kernel<M, lo>), grid, dim3(64), 0, s, a);
 }
 
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
@@ -578,8 +579,11 @@ hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
         return hipSuccess;
     // the item kernel addresses S with 32-bit byte offsets
     const bool small_s = (unsigned long long)(a.nobs + 1) * (unsigned long long)a.ms * 4ull < (1ull << 32);
-    if (MT <= 6 && small_s) {
-        const int ng = (a.tile_groups == 3) ? 3 : 2;
+    if (MT <= 10 && small_s) {
+        // tile groups per segment: one of the two candidates of this MT (the
+        // smaller by default)
+        const int lo = item_groups_lo(MT);
+        const int ng = (a.tile_groups == lo + 1) ? lo + 1 : lo;
         const long long nitem = (long long)a.nbatch * a.nseg * ng;
         const dim3 igrid((unsigned)(8 * ((nitem + 7) / 8)));
         switch (MT) {
@@ -588,7 +592,11 @@ hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
         case 3: launch_mix_item<3>(s, igrid, a, ng); break;
         case 4: launch_mix_item<4>(s, igrid, a, ng); break;
         case 5: launch_mix_item<5>(s, igrid, a, ng); break;
-        default: launch_mix_item<6>(s, igrid, a, ng); break;
+        case 6: launch_mix_item<6>(s, igrid, a, ng); break;
+        case 7: launch_mix_item<7>(s, igrid, a, ng); break;
+        case 8: launch_mix_item<8>(s, igrid, a, ng); break;
+        case 9: launch_mix_item<9>(s, igrid, a, ng); break;
+        default: launch_mix_item<10>(s, igrid, a, ng); break;
         }
         return hipGetLastError();
     }

@@ -358,7 +358,8 @@ int enkfg_get_timings(enkfg_ctx* ctx, double* ms, int n);
  * transforms.c:700-716).  enkfg_calc_transforms orders the stride nodes by
  * obs-count bucket (the largest (node, obstype) list: [0, 256), [256, 1024),
  * [1024, 4096), [4096, ...)), heaviest bucket first, and runs each bucket's
- * Gram matrices with the tile groups per list (2 or 3 wavefronts) that the
+ * Gram matrices with the tile groups per list (wavefronts: 2 or 3 for
+ * nmem <= 96, 3 or 4 up to 112, 4 or 5 up to 144, 5 or 6 up to 160) that the
  * first sufficiently large batch of the bucket measured fastest.  Writes up
  * to n ints: out[0] = number of buckets B, then per bucket the lower bound,
  * the nodes in it at the last calc, and the tile groups chosen for FP64 and

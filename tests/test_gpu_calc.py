@@ -107,16 +107,19 @@ def test_localisation_one_walk_and_pool_overflow():
     E.close()
 
 
-@pytest.mark.parametrize("precision", [enkfg.PREC_FP64, enkfg.PREC_MIXED])
-def test_calc_buckets_and_autotune_bitwise(precision):
+@pytest.mark.parametrize("precision,nmem", [(enkfg.PREC_FP64, None), (enkfg.PREC_MIXED, None),
+                                            (enkfg.PREC_FP64, 144), (enkfg.PREC_MIXED, 144)])
+def test_calc_buckets_and_autotune_bitwise(precision, nmem):
     """The calc orders the nodes by obs-count bucket and tunes each bucket's
-    Gram tile groups (2 or 3 wavefronts per list) on its first large batch,
-    running both.  A node's Gram matrix does not depend on the order or the
-    tile groups, so a second calc on the same engine (tuned choices) gives
-    T, w and the diagnostics of the first (tuning) calc bit for bit; the
-    buckets account for every node."""
-    print("case calc buckets / autotune precision=%d" % precision, flush=True)
-    P = problem(seed=4, profiles=30, nobs=8000, locrad=150.0)
+    Gram tile groups (wavefronts per list: 2 or 3 for m <= 96, 4 or 5 for
+    m = 144) on its first large batch, running both.  A node's Gram matrix
+    does not depend on the order or the tile groups, so a second calc on the
+    same engine (tuned choices) gives T, w and the diagnostics of the first
+    (tuning) calc bit for bit; the buckets account for every node."""
+    print("case calc buckets / autotune precision=%d nmem=%s" % (precision, nmem), flush=True)
+    kw = {} if nmem is None else dict(nmem=nmem)
+    P = problem(seed=4, profiles=30, nobs=8000, locrad=150.0, **kw)
+    lo = 2 if P.nmem <= 96 else 4
     E = enkfg.Engine(P.nmem, mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF, precision=precision,
                      pertype_diag=True, keep_fp64=True)
     E.set_obstypes(P.obstypes)
@@ -129,8 +132,8 @@ def test_calc_buckets_and_autotune_bitwise(precision):
     print("  buckets", tu, flush=True)
     assert sum(b["nodes"] for b in tu) == P.node_lon.size
     key = "groups_fp64" if precision == enkfg.PREC_FP64 else "groups_mixed"
-    assert all(b[key] in (0, 2, 3) for b in tu)
-    assert any(b[key] in (2, 3) for b in tu), "no bucket was tuned"
+    assert all(b[key] in (0, lo, lo + 1) for b in tu)
+    assert any(b[key] in (lo, lo + 1) for b in tu), "no bucket was tuned"
     assert len([b for b in tu if b["nodes"] > 0]) >= 2, "the problem should span several buckets"
     E.calc_transforms(P.node_lon, P.node_lat, stride=P.stride)
     T2, w2, T642, w642 = E.transforms(fp64=True)
