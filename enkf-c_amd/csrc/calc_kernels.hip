@@ -7,9 +7,9 @@
 //
 //   gram   (MFMA FP64)  G0 = S^T S and S^T s, streamed from the gathered
 //                       FP32 rows of S, tapered in FP64 on their way to LDS;
-//   factor (LDS/VGPR)   M = I + G0 inverted by in-register Gauss-Jordan, then
-//                         DEnKF  T = (I + M^-1)/2          (= I - G S / 2)
-//                         ETKF   T = M^-1/2                (= I - (M+M^1/2)^-1 S^T S)
+//   factor (LDS/VGPR)   for ETKF (DEnKF and EnOI: factor_wave.hip), M = I + G0
+//                       inverted by in-register Gauss-Jordan, then
+//                         T = M^-1/2                (= I - (M+M^1/2)^-1 S^T S)
 //                         w = M^-1 S^T s,  dfs = tr(M^-1 G0) (= tr(G S)),
 //                         srf = sqrt(tr(G0) / dfs) - 1
 //                       followed by relaxation to prior and the FP32 cast.
@@ -19,8 +19,10 @@
 //   Matrices", SIAM 2008, eq. 6.15 and sec. 6.3).
 //
 // The identities used are those of the User Guide (doc/enkf-userguide.tex:
-// 440-451, 490-522); the per-type diagnostics re-solve the analysis with only
-// the obs of one type, as the USE_MPIQUEUE build does (transforms.c:853-879).
+// 440-451, 490-522); the per-type diagnostics, which re-solve the analysis
+// with only the obs of one type as the USE_MPIQUEUE build does
+// (transforms.c:853-879), are factor_wave.hip's factor_trace_kernel for
+// every scheme.
 #include <algorithm>
 
 #include "enkfg_internal.h"
@@ -757,7 +759,7 @@ __device__ inline void etkf_ws_iterate(const CalcArgs& a, int bnode, int node, d
     write_T<B>(a, node, x, 1.0, 0.0, sh);
 }
 
-template <int B, bool ETKF>
+template <int B>
 __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
 {
     __shared__ FactorShared<B> sh;
@@ -797,7 +799,7 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
 
     double x[B][B];
     const double trG = load_gram<B>(a, bnode, -1, 1.0, x, sh);
-    const double logdetA = gj_invert<B, ETKF>(x, m, sh);
+    const double logdetA = gj_invert<B, true>(x, m, sh);
     // w = M^-1 S^T s
     matvec_sts<B>(a, bnode, x, sh);
     if (tid < m) {
@@ -817,9 +819,7 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
     }
 
     if (dotT) {
-        if constexpr (!ETKF) {
-            write_T<B>(a, node, x, 0.5, 0.5, sh);
-        } else if constexpr (B > 6) {
+        if constexpr (B > 6) {
             etkf_ws_iterate<B>(a, bnode, node, logdetA, x, sh);
         } else {
             // scaled coupled Denman-Beavers: Y -> M^1/2, Z -> M^-1/2; x holds
@@ -883,37 +883,9 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
         }
     }
 
-    // per-type diagnostics: re-solve with the obs of each type only
-    if (a.pertype) {
-        for (int t = 0; t < nt; ++t) {
-            const int pt = a.counts[(size_t)node * nt + t];
-            if (tid == 0)
-                a.pnlobs[(size_t)t * a.nnode + node] = pt;
-            if (pt == 0) {
-                if (tid == 0) {
-                    a.pdfs[(size_t)t * a.nnode + node] = 0.0f;
-                    a.psrf[(size_t)t * a.nnode + node] = 0.0f;
-                }
-                continue;
-            }
-            if (pt == p) {
-                if (tid == 0) {
-                    a.pdfs[(size_t)t * a.nnode + node] = a.pdfs[(size_t)nt * a.nnode + node];
-                    const float fd = (float)dfs;
-                    a.psrf[(size_t)t * a.nnode + node] = ((double)fd > 1.0e-6) ? a.psrf[(size_t)nt * a.nnode + node] : 0.0f;
-                }
-                continue;
-            }
-            const double trt = load_gram<B>(a, bnode, t, 1.0, x, sh);
-            gj_invert<B>(x, m, sh);
-            const double dt = dot_gram<B>(a, bnode, t, x, sh);
-            if (tid == 0) {
-                const float fd = (float)dt;
-                a.pdfs[(size_t)t * a.nnode + node] = fd;
-                a.psrf[(size_t)t * a.nnode + node] = ((double)fd > 1.0e-6) ? (float)(sqrt(trt / (double)fd) - 1.0) : 0.0f;
-            }
-        }
-    } else if (tid < nt) {
+    // the per-type diagnostics (the analysis re-solved with the obs of one
+    // type) are factor_trace_kernel's work items, launched after this kernel
+    if (!a.pertype && tid < nt) {
         a.pnlobs[(size_t)tid * a.nnode + node] = a.counts[(size_t)node * nt + tid];
         a.pdfs[(size_t)tid * a.nnode + node] = 0.0f;
         a.psrf[(size_t)tid * a.nnode + node] = 0.0f;
@@ -1023,37 +995,36 @@ hipError_t launch_gram(hipStream_t s, const CalcArgs& a)
 }
 
 template <int B>
-void launch_factor_b(hipStream_t s, const CalcArgs& a, bool etkf)
+void launch_factor_b(hipStream_t s, const CalcArgs& a)
 {
-    const dim3 grid((unsigned)a.nbatch);
-    if (etkf)
-        hipLaunchKernelGGL((factor_kernel<B, true>), grid, dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((factor_kernel<B, false>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((factor_kernel<B>), dim3((unsigned)a.nbatch), dim3(256), 0, s, a);
 }
 
 hipError_t launch_factor(hipStream_t s, const CalcArgs& a)
 {
     if (factor_wave_supported(a))
-        return launch_factor_wave(s, a);   // factor_wave.hip: one wavefront per matrix
+        return launch_factor_wave(s, a);   // DEnKF, EnOI: the symmetric sweep (factor_wave.hip)
+    // ETKF
     const int B = (a.m + 15) / 16;
-    const bool etkf = a.scheme == ENKFG_SCHEME_ETKF && a.mode != ENKFG_MODE_ENOI;
     if (a.nbatch == 0)
         return hipSuccess;
     switch (B) {
-    case 1: launch_factor_b<1>(s, a, etkf); break;
-    case 2: launch_factor_b<2>(s, a, etkf); break;
-    case 3: launch_factor_b<3>(s, a, etkf); break;
-    case 4: launch_factor_b<4>(s, a, etkf); break;
-    case 5: launch_factor_b<5>(s, a, etkf); break;
-    case 6: launch_factor_b<6>(s, a, etkf); break;
-    case 7: launch_factor_b<7>(s, a, etkf); break;
-    case 8: launch_factor_b<8>(s, a, etkf); break;
-    case 9: launch_factor_b<9>(s, a, etkf); break;
-    case 10: launch_factor_b<10>(s, a, etkf); break;
+    case 1: launch_factor_b<1>(s, a); break;
+    case 2: launch_factor_b<2>(s, a); break;
+    case 3: launch_factor_b<3>(s, a); break;
+    case 4: launch_factor_b<4>(s, a); break;
+    case 5: launch_factor_b<5>(s, a); break;
+    case 6: launch_factor_b<6>(s, a); break;
+    case 7: launch_factor_b<7>(s, a); break;
+    case 8: launch_factor_b<8>(s, a); break;
+    case 9: launch_factor_b<9>(s, a); break;
+    case 10: launch_factor_b<10>(s, a); break;
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !a.pertype)
+        return e;
+    return launch_factor_trace(s, a);   // per-type diagnostics, after the node's own
 }
 
 }  // namespace enkfg

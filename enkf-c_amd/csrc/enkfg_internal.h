@@ -139,9 +139,12 @@ __host__ __device__ inline size_t etkf_ws_per_node(int m)
 hipError_t launch_gram(hipStream_t s, const CalcArgs& a);
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a);   // gram_mixed.hip
 hipError_t launch_factor(hipStream_t s, const CalcArgs& a);
-// factor_wave.hip: DEnKF / EnOI with m <= 96, one wavefront per matrix
+// factor_wave.hip: DEnKF / EnOI with m <= 160 by the symmetric sweep; the
+// per-type diagnostics of every scheme (factor_trace_kernel, after the
+// node's own factorisation on the same stream)
 bool factor_wave_supported(const CalcArgs& a);
 hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a);
+hipError_t launch_factor_trace(hipStream_t s, const CalcArgs& a);
 
 // update (update_kernels.hip)
 struct UpdArgs {

@@ -821,8 +821,16 @@ hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a)
         }
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !a.pertype || a.ntypes <= 0)
+    if (e != hipSuccess || !a.pertype)
         return e;
+    return launch_factor_trace(s, a);
+}
+
+hipError_t launch_factor_trace(hipStream_t s, const CalcArgs& a)
+{
+    if (a.nbatch == 0 || a.ntypes <= 0)
+        return hipSuccess;
+    const int MT = (a.m + 15) / 16;
     // per-obstype diagnostics, one work item per (node, obstype)
     const dim3 tgrid((unsigned)((long long)a.nbatch * a.ntypes)), tblock(192);
     switch (MT) {
