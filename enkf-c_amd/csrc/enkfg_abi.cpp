@@ -423,19 +423,11 @@ int do_localise(enkfg_ctx* c, const enkfg_nodes* nodes)
         HIPCHECK(c, c->d_srow.reserve(std::max<long long>(nraw, 1) * sizeof(int)));
         HIPCHECK(c, c->d_rd2.reserve(std::max<long long>(nraw, 1) * sizeof(double)));
         HIPCHECK(c, c->d_sd2.reserve(std::max<long long>(nraw, 1) * sizeof(double)));
-        int zero = 0;
-        HIPCHECK(c, hipMemcpy(c->d_err.p, &zero, sizeof zero, hipMemcpyHostToDevice));
         HIPCHECK(c, launch_loc_nlobsmax(c->stream, c->ni_s, c->nj_s, c->d_qxyz.as<double>(),
                                         c->d_glob.as<unsigned char>(), vis, c->d_nodes.as<KdNode>(),
                                         c->d_types.as<TypeInfo>(), nt, c->d_status.as<short>(), c->d_raw.as<int>(),
                                         c->d_rawoff.as<long long>(), c->d_rrow.as<int>(), c->d_rd2.as<double>(),
-                                        c->d_srow.as<int>(), c->d_sd2.as<double>(), c->d_counts.as<int>(),
-                                        c->d_err.as<int>()));
-        HIPCHECK(c, hipStreamSynchronize(c->stream));
-        int err = 0;
-        HIPCHECK(c, hipMemcpy(&err, c->d_err.p, sizeof err, hipMemcpyDeviceToHost));
-        if (err != 0)
-            return fail(c, ENKFG_ERR_UNSUPPORTED, "NLOBSMAX: more than 8192 candidate obs of one type at a node");
+                                        c->d_srow.as<int>(), c->d_sd2.as<double>(), c->d_counts.as<int>()));
     }
     HIPCHECK(c, launch_scan_counts(c->stream, c->d_counts.as<int>(), c->d_offsets.as<long long>(), ncnt,
                                    c->d_scantmp.p, &tmpb));

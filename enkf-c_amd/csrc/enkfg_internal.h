@@ -45,12 +45,12 @@ hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, c
                             const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                             const short* status, int* counts, int* raw);
 // NLOBSMAX types: candidates (raw counts scanned into rawoff) gathered, sorted
-// by distance and selected; writes the list lengths into counts; *err = 1
-// when a (node, type) has more candidates than the sort handles (8192)
+// by distance and selected; writes the list lengths into counts (rrow and
+// rd2 are overwritten)
 hipError_t launch_loc_nlobsmax(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                                const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                                const short* status, const int* raw, const long long* rawoff, int* rrow, double* rd2,
-                               int* srow, double* sd2, int* counts, int* err);
+                               int* srow, double* sd2, int* counts);
 // one-walk localisation (loc_kernels.hip): lists appended to a pool of
 // chunks of kLocChunk entries, then compacted into the CSR arrays
 constexpr int kLocChunk = 512;

@@ -215,7 +215,8 @@ __global__ void __launch_bounds__(64) loc_rawfill_kernel(int nnode, int ni, cons
 // platform, common/kdtree.c:569-570), by a bitonic sort of (distsq, index)
 // keys in LDS; then obs_findlocal's selection (observations.c:1899-1908):
 // the first min(n, NLOBSMAX) candidates, skipping those whose status is not
-// OK.  Writes the sorted candidates and the list length.
+// OK.  Writes the sorted candidates and the list length.  Lists of more
+// than kSortMax candidates are loc_sort_big_kernel's.
 constexpr int kSortMax = 8192;
 
 __global__ void __launch_bounds__(256) loc_sort_kernel(int nnode, const TypeInfo* __restrict__ types, int ntypes,
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(256) loc_sort_kernel(int nnode, const TypeInfo
                                                        const long long* __restrict__ rawoff,
                                                        const int* __restrict__ rrow, const double* __restrict__ rd2,
                                                        int* __restrict__ srow, double* __restrict__ sd2,
-                                                       int* __restrict__ counts, int* __restrict__ err)
+                                                       int* __restrict__ counts)
 {
     __shared__ double kd[kSortMax];
     __shared__ int ki[kSortMax];
@@ -233,13 +234,8 @@ __global__ void __launch_bounds__(256) loc_sort_kernel(int nnode, const TypeInfo
     if (item >= (long long)nnode * ntypes)
         return;
     const int n = raw[item];
-    if (n == 0)
+    if (n == 0 || n > kSortMax)
         return;
-    if (n > kSortMax) {
-        if (threadIdx.x == 0)
-            atomicExch(err, 1);
-        return;
-    }
     const TypeInfo& ti = types[item % ntypes];
     const long long off = rawoff[item];
     int P = 1;
@@ -280,6 +276,89 @@ __global__ void __launch_bounds__(256) loc_sort_kernel(int nnode, const TypeInfo
         if (k < nsel && status[row] == kStatusOK)
             mine++;
     }
+    atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        counts[item] = cnt;
+}
+
+// Lists of more than kSortMax candidates (the reference has no limit): one
+// workgroup sorts a (node, type)'s candidates in place in global memory by a
+// bottom-up merge sort on the squared distance, stable -- equal distances
+// keep their visiting order, the order loc_sort_kernel's (distance, index)
+// keys give -- ping-ponging between (rd2, rrow) and (sd2, srow); each
+// element finds its place in the merged run by a binary search in the other
+// run.  Then the same selection.
+__global__ void __launch_bounds__(256) loc_sort_big_kernel(int nnode, const TypeInfo* __restrict__ types,
+                                                           int ntypes, const short* __restrict__ status,
+                                                           const int* __restrict__ raw,
+                                                           const long long* __restrict__ rawoff, int* __restrict__ rrow,
+                                                           double* __restrict__ rd2, int* __restrict__ srow,
+                                                           double* __restrict__ sd2, int* __restrict__ counts)
+{
+    __shared__ int cnt;
+    const long long item = blockIdx.x;
+    if (item >= (long long)nnode * ntypes)
+        return;
+    const int n = raw[item];
+    if (n <= kSortMax)
+        return;
+    const TypeInfo& ti = types[item % ntypes];
+    const long long off = rawoff[item];
+    double* keys[2] = {rd2 + off, sd2 + off};
+    int* rows[2] = {rrow + off, srow + off};
+    int src = 0;
+    for (int width = 1; width < n; width <<= 1) {
+        const double* ik = keys[src];
+        const int* ir = rows[src];
+        double* ok = keys[src ^ 1];
+        int* orow = rows[src ^ 1];
+        for (int k = threadIdx.x; k < n; k += 256) {
+            const int r0 = (k / (2 * width)) * (2 * width);   // the pair of runs [r0, mid) [mid, end)
+            const int mid = min(r0 + width, n), end = min(r0 + 2 * width, n);
+            const double key = ik[k];
+            int pos;
+            if (k < mid) {   // before the right run's elements >= key
+                int lo = mid, hi = end;
+                while (lo < hi) {
+                    const int h = (lo + hi) >> 1;
+                    if (ik[h] < key)
+                        lo = h + 1;
+                    else
+                        hi = h;
+                }
+                pos = k + (lo - mid);
+            } else {         // after the left run's elements <= key
+                int lo = r0, hi = mid;
+                while (lo < hi) {
+                    const int h = (lo + hi) >> 1;
+                    if (ik[h] <= key)
+                        lo = h + 1;
+                    else
+                        hi = h;
+                }
+                pos = (k - mid) + lo;
+            }
+            ok[pos] = key;
+            orow[pos] = ir[k];
+        }
+        __syncthreads();   // the pass is complete before the next reads it
+        src ^= 1;
+    }
+    if (src == 0) {   // the sorted candidates end in (sd2, srow)
+        for (int k = threadIdx.x; k < n; k += 256) {
+            sd2[off + k] = rd2[off + k];
+            srow[off + k] = rrow[off + k];
+        }
+    }
+    if (threadIdx.x == 0)
+        cnt = 0;
+    __syncthreads();
+    const int nsel = min(n, ti.nlobsmax);
+    int mine = 0;
+    for (int k = threadIdx.x; k < nsel; k += 256)
+        if (status[srow[off + k]] == kStatusOK)
+            mine++;
     atomicAdd(&cnt, mine);
     __syncthreads();
     if (threadIdx.x == 0)
@@ -591,7 +670,7 @@ hipError_t launch_loc_count(hipStream_t s, int ni, int nj, const double* qxyz, c
 hipError_t launch_loc_nlobsmax(hipStream_t s, int ni, int nj, const double* qxyz, const unsigned char* isglobal,
                                const unsigned char* visible, const KdNode* nodes, const TypeInfo* types, int ntypes,
                                const short* status, const int* raw, const long long* rawoff, int* rrow, double* rd2,
-                               int* srow, double* sd2, int* counts, int* err)
+                               int* srow, double* sd2, int* counts)
 {
     const long long nt = (long long)ni * nj * ntypes;
     if (nt <= 0)
@@ -601,7 +680,9 @@ hipError_t launch_loc_nlobsmax(hipStream_t s, int ni, int nj, const double* qxyz
     hipLaunchKernelGGL(loc_rawfill_kernel, walk_grid(ni, nj, ntypes), dim3(64), 0, s, ni * nj, ni, qxyz, isglobal,
                        visible, nodes, types, ntypes, rawoff, rrow, rd2);
     hipLaunchKernelGGL(loc_sort_kernel, dim3((unsigned)nt), dim3(256), 0, s, ni * nj, types, ntypes, status, raw,
-                       rawoff, rrow, rd2, srow, sd2, counts, err);
+                       rawoff, rrow, rd2, srow, sd2, counts);
+    hipLaunchKernelGGL(loc_sort_big_kernel, dim3((unsigned)nt), dim3(256), 0, s, ni * nj, types, ntypes, status, raw,
+                       rawoff, rrow, rd2, srow, sd2, counts);
     return hipGetLastError();
 }
 

@@ -226,6 +226,36 @@ def test_localisation_nlobsmax(nmax):
     E.close()
 
 
+def test_localisation_nlobsmax_long_candidate_lists():
+    """NLOBSMAX with more candidates within the radius than the LDS sort
+    holds (8192): those lists are sorted in global memory, stably by
+    distance, and must still equal the reference's ordered search, with the
+    selection and the status filter, entry for entry."""
+    print("case localisation nlobsmax, candidate lists above 8192", flush=True)
+    P = problem(seed=12, profiles=20, nobs=40000, locrad=220.0, nj=30, ni=45)
+    P.obstypes[0]["nlobsmax"] = 10000
+    P.obstypes[1]["nlobsmax"] = 1000000
+    rng = np.random.default_rng(2)
+    status = P.obs_status.copy()
+    status[rng.random(P.nobs) < 0.1] = 7
+    E = engine_for(P, enkfg.MODE_ENKF, enkfg.SCHEME_DENKF)
+    E.set_obs_status(status)
+    E.localise(P.node_lon, P.node_lat, stride=P.stride)
+    counts = E.local_counts()
+    ob = oracle_obs(P, status=status, status_build=P.obs_status)
+    big = 0
+    for k in range(P.node_lon.size):
+        ids_o, lc_o = ob.findlocal(P.node_lon.flat[k], P.node_lat.flat[k])
+        assert counts[k] == len(ids_o), "node %d count" % k
+        ids_g, lc_g = E.localobs(k, counts[k])
+        assert np.array_equal(ids_g[:counts[k]], ids_o), "node %d ids" % k
+        assert np.array_equal(lc_g[:counts[k]].view(np.uint64), lc_o.view(np.uint64))
+        big += int(np.count_nonzero(P.obs_type[ids_o] == 0) > 8192)   # so more candidates than that
+    print("  %d nodes, %d with a type-0 list of more than 8192" % (P.node_lon.size, big), flush=True)
+    assert big > 0, "the case must hold lists beyond the LDS sort"
+    E.close()
+
+
 def relerr(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
