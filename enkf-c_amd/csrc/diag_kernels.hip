@@ -75,6 +75,80 @@ __global__ __launch_bounds__(kCells) void spread_cells_kernel(const float* __res
     out[(long long)f * ncell + c0 + threadIdx.x] = finish_spread(v1, v2, m);
 }
 
+// The same for m % 4 == 0, as a loop of each wave over chunks of 64 cells
+// (chunk c of field c / nchunk): the float4 records of the next chunk are
+// loaded into registers while the lanes sum the current one from LDS, so a
+// wave keeps its loads in flight through its sums (the LDS per wave allows
+// only about six waves per CU).  NQ float4 per lane hold a chunk
+// (64 m / 256 <= NQ).
+template <int NQ>
+__global__ __launch_bounds__(kCells) void spread_cells_pipe_kernel(const float* __restrict__ E, long long ncell,
+                                                                   int m, int nfield, float* __restrict__ out)
+{
+    extern __shared__ float tile[];
+    const long long nchunk = (ncell + kCells - 1) / kCells;
+    const long long total = nchunk * nfield;
+    const int ld = m + 1;
+    const int lane = threadIdx.x;
+    float4 q[NQ];
+    long long base = 0;   // first record of the chunk in q
+    int nc = 0;           // its cells
+    auto load = [&](long long c) {
+        const long long f = c / nchunk;
+        const long long c0 = (c - f * nchunk) * kCells;
+        nc = (int)((ncell - c0) < kCells ? (ncell - c0) : kCells);
+        base = f * ncell + c0;
+        const float4* src4 = reinterpret_cast<const float4*>(E + base * m);
+        const int n4 = nc * m / 4;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            const int k4 = lane + kCells * j;
+            q[j] = (k4 < n4) ? src4[k4] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    };
+    long long c = blockIdx.x;
+    if (c >= total)
+        return;
+    load(c);
+    for (;;) {
+        // the chunk in q to LDS (m % 4 == 0: the four values share a cell)
+        const int n4 = nc * m / 4;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            const int k4 = lane + kCells * j;
+            if (k4 < n4) {
+                const int k = 4 * k4, r = k / m;
+                float* d = tile + r * ld + (k - r * m);
+                d[0] = q[j].x;
+                d[1] = q[j].y;
+                d[2] = q[j].z;
+                d[3] = q[j].w;
+            }
+        }
+        const long long cur = base;
+        const int ncur = nc;
+        __syncthreads();
+        const long long next = c + gridDim.x;
+        if (next < total)
+            load(next);   // in flight during the sums
+        if (lane < ncur) {
+            const float* row = tile + lane * ld;
+            double v1 = 0.0, v2 = 0.0;
+            for (int e = 0; e < m; ++e) {
+                const float x = row[e];
+                const float x2 = x * x;
+                v1 += (double)x;
+                v2 += (double)x2;
+            }
+            out[cur + lane] = finish_spread(v1, v2, m);
+        }
+        __syncthreads();   // the sums have read the tile before it is refilled
+        if (next >= total)
+            break;
+        c = next;
+    }
+}
+
 // E [field][m][cell] (the reference's field buffers, members outermost): one
 // lane per cell, coalesced across lanes for every member.
 __global__ __launch_bounds__(256) void spread_members_kernel(const float* __restrict__ E, long long ncell, int m,
@@ -112,7 +186,16 @@ hipError_t launch_spread(hipStream_t s, const float* E, int nfield, long long nc
         if (lds > 64 * 1024)
             return hipErrorInvalidValue;
         const bool vec = (m % 4 == 0) && (reinterpret_cast<uintptr_t>(E) % 16 == 0);
-        if (vec)
+        // the looping kernel: enough waves to fill the CUs at the LDS limit
+        const long long total = nb * nfield;
+        const unsigned pgrid = (unsigned)(total < 4096 ? total : 4096);
+        if (vec && m <= 96)
+            hipLaunchKernelGGL(spread_cells_pipe_kernel<24>, dim3(pgrid), dim3(kCells), lds, s, E, ncell, m, nfield,
+                               out);
+        else if (vec && m <= 160)
+            hipLaunchKernelGGL(spread_cells_pipe_kernel<40>, dim3(pgrid), dim3(kCells), lds, s, E, ncell, m, nfield,
+                               out);
+        else if (vec)
             hipLaunchKernelGGL(spread_cells_kernel<true>, dim3((unsigned)nb, nfield), dim3(kCells), lds, s, E, ncell,
                                m, out);
         else
