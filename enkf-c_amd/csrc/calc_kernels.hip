@@ -7,8 +7,9 @@
 //
 //   gram   (MFMA FP64)  G0 = S^T S and S^T s, streamed from the gathered
 //                       FP32 rows of S, tapered in FP64 on their way to LDS;
-//   factor (LDS/VGPR)   for ETKF (DEnKF and EnOI: factor_wave.hip), M = I + G0
-//                       inverted by in-register Gauss-Jordan, then
+//   factor (LDS/VGPR)   for ETKF with m > 96 (m <= 96, and DEnKF and EnOI:
+//                       factor_wave.hip), M = I + G0 inverted by in-register
+//                       Gauss-Jordan, then
 //                         T = M^-1/2                (= I - (M+M^1/2)^-1 S^T S)
 //                         w = M^-1 S^T s,  dfs = tr(M^-1 G0) (= tr(G S)),
 //                         srf = sqrt(tr(G0) / dfs) - 1
@@ -819,68 +820,7 @@ __global__ void __launch_bounds__(256) factor_kernel(CalcArgs a)
     }
 
     if (dotT) {
-        if constexpr (B > 6) {
-            etkf_ws_iterate<B>(a, bnode, node, logdetA, x, sh);
-        } else {
-            // scaled coupled Denman-Beavers: Y -> M^1/2, Z -> M^-1/2; x holds
-            // M^-1 = Y0^-1 already.  Z lives in shared-by-owner registers.
-            double y[B][B], z[B][B], u[B][B];
-            load_gram<B>(a, bnode, -1, 1.0, y, sh);
-            const int ti = tid >> 4, tj = tid & 15;
-#pragma unroll
-            for (int r = 0; r < B; ++r)
-#pragma unroll
-                for (int c = 0; c < B; ++c)
-                    z[r][c] = (ti * B + r == tj * B + c) ? 1.0 : 0.0;
-            double logdetY = logdetA;
-            double logdetZ = 0.0;
-            bool last = false;
-            for (int it = 0; it < 60; ++it) {
-                // x = Y^-1 (for it == 0 already computed)
-                if (it > 0) {
-#pragma unroll
-                    for (int r = 0; r < B; ++r)
-#pragma unroll
-                        for (int c = 0; c < B; ++c)
-                            x[r][c] = y[r][c];
-                    logdetY = gj_invert<B, true>(x, m, sh);
-                }
-#pragma unroll
-                for (int r = 0; r < B; ++r)
-#pragma unroll
-                    for (int c = 0; c < B; ++c)
-                        u[r][c] = z[r][c];
-                if (it > 0)
-                    logdetZ = gj_invert<B, true>(u, m, sh);
-                // (for it == 0, Z = I is its own inverse)
-                const double mu = last ? 1.0 : exp(-(logdetY + logdetZ) / (2.0 * m));
-                const double hm = 0.5 * mu, hi = 0.5 / mu;
-                double dn = 0.0, nn = 0.0;
-#pragma unroll
-                for (int r = 0; r < B; ++r)
-#pragma unroll
-                    for (int c = 0; c < B; ++c) {
-                        const double yn = hm * y[r][c] + hi * u[r][c];
-                        const double zn = hm * z[r][c] + hi * x[r][c];
-                        dn += (zn - z[r][c]) * (zn - z[r][c]);
-                        nn += zn * zn;
-                        y[r][c] = yn;
-                        z[r][c] = zn;
-                    }
-                dn = block_sum(dn, sh);
-                nn = block_sum(nn, sh);
-                if (last)
-                    break;
-                if (dn <= 1e-20 * nn)
-                    last = true;   // converged quadratically: one more unscaled step
-            }
-            if (!last && tid == 0)
-                sh.flag = 1;       // no convergence in 60 steps: reported, not written silently
-            // symmetrise Z into T: Z(e1,e2) and Z(e2,e1) live in different
-            // threads; exchange through shared memory row by row is avoided by
-            // writing Z as is (symmetric to rounding).
-            write_T<B>(a, node, z, 1.0, 0.0, sh);
-        }
+        etkf_ws_iterate<B>(a, bnode, node, logdetA, x, sh);
     }
 
     // the per-type diagnostics (the analysis re-solved with the obs of one
@@ -1004,17 +944,14 @@ hipError_t launch_factor(hipStream_t s, const CalcArgs& a)
 {
     if (factor_wave_supported(a))
         return launch_factor_wave(s, a);   // DEnKF, EnOI: the symmetric sweep (factor_wave.hip)
-    // ETKF
+    // ETKF: by the symmetric sweep up to m = 96; beyond, the Gauss-Jordan
+    // kernel with its iterates in device memory
+    if (a.m <= 96)
+        return launch_factor_etkf(s, a);
     const int B = (a.m + 15) / 16;
     if (a.nbatch == 0)
         return hipSuccess;
     switch (B) {
-    case 1: launch_factor_b<1>(s, a); break;
-    case 2: launch_factor_b<2>(s, a); break;
-    case 3: launch_factor_b<3>(s, a); break;
-    case 4: launch_factor_b<4>(s, a); break;
-    case 5: launch_factor_b<5>(s, a); break;
-    case 6: launch_factor_b<6>(s, a); break;
     case 7: launch_factor_b<7>(s, a); break;
     case 8: launch_factor_b<8>(s, a); break;
     case 9: launch_factor_b<9>(s, a); break;

@@ -145,6 +145,8 @@ hipError_t launch_factor(hipStream_t s, const CalcArgs& a);
 bool factor_wave_supported(const CalcArgs& a);
 hipError_t launch_factor_wave(hipStream_t s, const CalcArgs& a);
 hipError_t launch_factor_trace(hipStream_t s, const CalcArgs& a);
+// ETKF with m <= 96 by the symmetric sweep, then the per-type diagnostics
+hipError_t launch_factor_etkf(hipStream_t s, const CalcArgs& a);
 
 // update (update_kernels.hip)
 struct UpdArgs {

@@ -780,6 +780,438 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(sweep_
     }
 }
 
+// ---- ETKF, m <= 96: T = M^-1/2 by the symmetric sweep
+//
+// M = I + G0 and every iterate below are symmetric, so as in
+// factor_sym_kernel only the blocks on and below the diagonal are held, one
+// per lane, here 4 x 4 blocks of a G x G grid (4 G >= m; 300 lanes for
+// m = 96), so that four matrices fit a lane's registers.  M^-1/2 comes
+// from the scaled product form of the Denman-Beavers iteration (Higham,
+// "Functions of Matrices", SIAM 2008, sec. 6.3):
+//   W_k = M_k^-1,  mu_k = |det M_k|^(-1/(2m)),
+//   Z_{k+1} = mu_k / 2 Z_k (I + mu_k^-2 W_k),  Z_0 = I   (-> M^-1/2)
+//   M_{k+1} = (I + (mu_k^2 M_k + mu_k^-2 W_k) / 2) / 2,   M_0 = M   (-> I)
+// one sweep inverse and one product per step, against two inverses per
+// step for the coupled form.  The product Z_k W_k (the two commute, so it
+// is symmetric) is formed block by block: for each block column k the
+// lanes holding its blocks publish them (and, from the upper triangle,
+// their transposes) in LDS, and every lane adds Z(br, k) W(k, bc).  The
+// first inverse gives w, dfs and srf as in factor_sym_kernel; the step
+// after the change of Z drops below 1e-10 of its norm is taken unscaled
+// and is the last, as in factor_kernel.  Entries outside m x m are kept as
+// identity.
+constexpr int kEtkfBS = 4;
+
+template <int G>
+struct EtkfGeom {
+    static constexpr int MP = kEtkfBS * G;
+    static constexpr int NB = G * (G + 1) / 2;      // lower-triangle blocks
+    static constexpr int NT = 64 * ((NB + 63) / 64);
+};
+
+// element (i, j) of a segment's Gram matrix in the scratch layout of the
+// Gram kernels (gram_index with B = MT blocks of a 16 x 16 grid)
+__device__ inline size_t gram_at(int MT, int i, int j)
+{
+    return ((size_t)((i / MT) * 16 + j / MT) * MT + (i % MT)) * MT + (j % MT);
+}
+
+// sum over the NT threads in a fixed order (waves, then the wave totals in
+// order); every thread receives it
+template <int NT>
+__device__ inline double group_sum(double v, double* tmp, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1)
+        v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((lane & 63) == 0)
+        tmp[lane >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int q = 0; q < NT / 64; ++q)
+        s += tmp[q];
+    __syncthreads();
+    return s;
+}
+
+// x <- -x^-1 by the sweep (x symmetric positive definite in its m x m part,
+// identity outside); returns log det x and flags a pivot that is not
+// positive
+template <int G>
+__device__ inline double etkf_sweep(double (&x)[kEtkfBS][kEtkfBS], double (*col)[EtkfGeom<G>::MP], int m, int br,
+                                    int bc, bool act, bool& bad)
+{
+    constexpr int BS = kEtkfBS;
+    auto publish = [&](double* dst, int kb, int rk) {
+        if (!act)
+            return;
+        if (bc == kb) {
+#pragma unroll
+            for (int r = 0; r < BS; ++r)
+                dst[br * BS + r] = x[r][rk];
+        } else if (br == kb) {
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+                dst[bc * BS + c] = x[rk][c];
+        }
+    };
+    double mant = 1.0;   // det = mant * 2^ex
+    int ex = 0;
+    __syncthreads();     // the previous sweep's last column has been read
+    publish(col[0], 0, 0);
+#pragma unroll 1
+    for (int kb = 0; kb < G; ++kb) {
+        if (kb * BS >= m)
+            break;
+#pragma unroll
+        for (int rk = 0; rk < BS; ++rk) {
+            const int k = kb * BS + rk;
+            if (k >= m)
+                break;
+            __syncthreads();
+            const int buf = k & 1;
+            const double d = col[buf][k];
+            bad |= !(d > 0.0);
+            int e;
+            mant = frexp(mant * d, &e);
+            ex += e;
+            const double pinv = 1.0 / d;
+            double u[BS], ws[BS];
+#pragma unroll
+            for (int r = 0; r < BS; ++r)
+                u[r] = col[buf][br * BS + r];
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+                ws[c] = col[buf][bc * BS + c] * pinv;
+            if (br == kb)
+                u[rk] = d - 1.0;
+            if (bc == kb)
+                ws[rk] = (d - 1.0) * pinv;
+#pragma unroll
+            for (int r = 0; r < BS; ++r)
+#pragma unroll
+                for (int c = 0; c < BS; ++c)
+                    x[r][c] = fma(-u[r], ws[c], x[r][c]);
+            if (br == kb && bc == kb)
+                x[rk][rk] = -pinv;
+            if (k + 1 < m)   // a compile-time block column per unrolled step
+                publish(col[buf ^ 1], rk + 1 < BS ? kb : kb + 1, (rk + 1) % BS);
+        }
+    }
+    return log(mant) + (double)ex * 0.69314718055994530942;
+}
+
+template <int G>
+__global__ void __launch_bounds__(EtkfGeom<G>::NT) __attribute__((amdgpu_waves_per_eu(2)))
+factor_etkf_kernel(CalcArgs a)
+{
+    constexpr int BS = kEtkfBS;
+    constexpr int MP = EtkfGeom<G>::MP, NB = EtkfGeom<G>::NB, NT = EtkfGeom<G>::NT;
+    __shared__ double col[2][MP];
+    __shared__ double zc[2][G][BS * BS];   // block column k of Z, block i at [i]
+    __shared__ double wc[2][G][BS * BS];   // ... of W
+    __shared__ double part[G][G][BS];      // w: [row block][other block][row in block]
+    __shared__ double vs[MP];
+    __shared__ double slab[BS][MP];        // BS rows of Z for the T output
+    __shared__ double mst[BS * BS][NT];    // M_{k+1} during the product, element-major
+    __shared__ double tmp[NT / 64];
+    __shared__ int flag;
+    const int lane = threadIdx.x;
+    const int nt = a.ntypes;
+    const int bnode = (int)blockIdx.x;
+    if (bnode >= a.nbatch)
+        return;
+    const int node = a.node_list ? a.node_list[a.node0 + bnode] : a.node0 + bnode;
+    const int m = a.m, md = a.md;
+    const int MT = (m + 15) / 16;   // the Gram scratch's block size
+    const int MPG = 16 * MT;
+    const long long off = a.offsets[(size_t)node * nt];
+    const int p = (int)(a.offsets[(size_t)(node + 1) * nt] - off);
+    if (!a.pertype) {
+        if (lane < nt) {
+            a.pnlobs[(size_t)lane * a.nnode + node] = a.counts[(size_t)node * nt + lane];
+            a.pdfs[(size_t)lane * a.nnode + node] = 0.0f;
+            a.psrf[(size_t)lane * a.nnode + node] = 0.0f;
+        }
+    }
+    if (p == 0) {
+        if (lane == 0) {
+            a.pnlobs[(size_t)nt * a.nnode + node] = 0;
+            a.pdfs[(size_t)nt * a.nnode + node] = 0.0f;
+            a.psrf[(size_t)nt * a.nnode + node] = 0.0f;
+        }
+        // empty column: T = I, w = 0 (transforms.c:766-780)
+        for (int k = lane; k < md * m; k += NT) {
+            const int e1 = k / m, e2 = k - e1 * m;
+            a.T[(size_t)node * md * m + k] = (e1 == e2) ? 1.0f : 0.0f;
+            if (a.T64 != nullptr)
+                a.T64[(size_t)node * md * m + k] = (e1 == e2) ? 1.0 : 0.0;
+        }
+        for (int e = lane; e < m; e += NT) {
+            a.w[(size_t)node * m + e] = 0.0f;
+            if (a.w64 != nullptr)
+                a.w64[(size_t)node * m + e] = 0.0;
+        }
+        return;
+    }
+    if (lane == 0)
+        flag = 0;
+
+    const bool act = lane < NB;
+    int br = 0;
+    if (act)
+        while ((br + 1) * (br + 2) / 2 <= lane)
+            ++br;
+    const int bc = act ? lane - br * (br + 1) / 2 : 0;
+    // entry (r, c) of this lane's block lies in the m x m part
+    auto inside = [&](int r, int c) { return act && br * BS + r < m && bc * BS + c < m; };
+
+    // ---- M = I + G0 (summed over the node's segments), tr(G0)
+    double mo[BS][BS];
+    double trg = 0.0;
+    const double* gb = a.gram + (size_t)bnode * a.nseg * MPG * MPG;
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            const int i = br * BS + r, j = bc * BS + c;
+            double v = (i == j) ? 1.0 : 0.0;
+            if (inside(r, c))
+                for (int sg = 0; sg < a.nseg; ++sg) {
+                    if (a.pertype && a.counts[(size_t)node * nt + sg] == 0)
+                        continue;
+                    const double g = gb[(size_t)sg * MPG * MPG + gram_at(MT, i, j)];
+                    v += g;
+                    if (i == j)
+                        trg += g;
+                }
+            mo[r][c] = v;
+        }
+    trg = group_sum<NT>(trg, tmp, lane);
+
+    // ---- x = -M^-1
+    double x[BS][BS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+        for (int c = 0; c < BS; ++c)
+            x[r][c] = mo[r][c];
+    bool bad = false;
+    double logdet = etkf_sweep<G>(x, col, m, br, bc, act, bad);
+
+    // S^T s into LDS, dfs = m - tr(M^-1), srf
+    if (lane < MP) {
+        double v = 0.0;
+        if (lane < m)
+            for (int sg = 0; sg < a.nseg; ++sg) {
+                if (a.pertype && a.counts[(size_t)node * nt + sg] == 0)
+                    continue;
+                v += a.sts[((size_t)bnode * a.nseg + sg) * MPG + lane];
+            }
+        vs[lane] = v;
+    }
+    double sd = 0.0;
+    if (act && br == bc) {
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+            if (br * BS + r < m)
+                sd += x[r][r];
+    }
+    sd = group_sum<NT>(sd, tmp, lane);   // (its barriers also order the vs stores before the reads)
+    if (lane == 0) {
+        const float fd = (float)((double)m + sd);
+        a.pnlobs[(size_t)nt * a.nnode + node] = p;
+        a.pdfs[(size_t)nt * a.nnode + node] = fd;
+        a.psrf[(size_t)nt * a.nnode + node] = (float)(sqrt(trg / (double)fd) - 1.0);
+    }
+    // ---- w = M^-1 S^T s from the blocks' row products, summed in block order
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < BS; ++r) {
+            double y = 0.0;
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+                y = fma(-x[r][c], vs[bc * BS + c], y);
+            part[br][bc][r] = y;
+        }
+        if (br != bc) {
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                double y = 0.0;
+#pragma unroll
+                for (int r = 0; r < BS; ++r)
+                    y = fma(-x[r][c], vs[br * BS + r], y);
+                part[bc][br][c] = y;
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < m) {
+        const int rb = lane / BS, rr = lane - rb * BS;
+        double y = 0.0;
+        for (int sl = 0; sl < G; ++sl)
+            y += part[rb][sl][rr];
+        a.w[(size_t)node * m + lane] = (float)y;
+        if (a.w64 != nullptr)
+            a.w64[(size_t)node * m + lane] = y;
+    }
+
+    // ---- the iteration: mo = M_k, x = -W_k, z = Z_k; M_{k+1} is formed as
+    // soon as W_k is known and waits in LDS while the product runs
+    double z[BS][BS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+        for (int c = 0; c < BS; ++c)
+            z[r][c] = (br * BS + r == bc * BS + c) ? 1.0 : 0.0;
+    bool last = false;
+    for (int it = 0; it < 60; ++it) {
+        // the block coordinates, opaque to the compiler in each step: the
+        // per-element constants derived from them are formed where they are
+        // used rather than held in registers across the steps
+        int ib = br, jb = bc;
+        asm volatile("" : "+v"(ib), "+v"(jb));
+        auto in_mm = [&](int r, int c) { return act && ib * BS + r < m && jb * BS + c < m; };
+        auto delta = [&](int r, int c) { return (ib * BS + r == jb * BS + c) ? 1.0 : 0.0; };
+        const double mu = last ? 1.0 : exp(-logdet / (2.0 * m));
+        const double hm = 0.5 * mu, hi = 0.5 / mu, qm = 0.25 * mu * mu, qi = 0.25 / (mu * mu);
+        // M_{k+1} = 1/2 I + mu^2/4 M_k + 1/(4 mu^2) W_k (identity outside m x m)
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const double dlt = delta(r, c);
+                mst[r * BS + c][lane] = in_mm(r, c) ? 0.5 * dlt + qm * mo[r][c] - qi * x[r][c] : dlt;
+            }
+        // pc = Z_k (-W_k): block column k of Z and of x, k = 0 .. G-1
+        double pc[BS][BS];
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+                pc[r][c] = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < G; ++k) {
+            const int buf = k & 1;
+            if (act && bc == k) {          // block (br, k), br >= k
+#pragma unroll
+                for (int r = 0; r < BS; ++r)
+#pragma unroll
+                    for (int c = 0; c < BS; ++c) {
+                        zc[buf][br][r * BS + c] = z[r][c];
+                        wc[buf][br][r * BS + c] = x[r][c];
+                    }
+            } else if (act && br == k) {   // block (bc, k) = block (k, bc)^T, bc < k
+#pragma unroll
+                for (int r = 0; r < BS; ++r)
+#pragma unroll
+                    for (int c = 0; c < BS; ++c) {
+                        zc[buf][bc][c * BS + r] = z[r][c];
+                        wc[buf][bc][c * BS + r] = x[r][c];
+                    }
+            }
+            __syncthreads();
+            if (act) {
+                // Z(br, k) (-W)(k, bc), with (-W)(k, bc) = (-W)(bc, k)^T
+#pragma unroll
+                for (int q = 0; q < BS; ++q) {
+                    double zr[BS], wq[BS];
+#pragma unroll
+                    for (int r = 0; r < BS; ++r)
+                        zr[r] = zc[buf][br][r * BS + q];
+#pragma unroll
+                    for (int c = 0; c < BS; ++c)
+                        wq[c] = wc[buf][bc][c * BS + q];
+#pragma unroll
+                    for (int r = 0; r < BS; ++r)
+#pragma unroll
+                        for (int c = 0; c < BS; ++c)
+                            pc[r][c] = fma(zr[r], wq[c], pc[r][c]);
+                }
+            }
+        }
+        // the diagonal blocks of the symmetric product, symmetrised
+        if (br == bc) {
+#pragma unroll
+            for (int r = 0; r < BS; ++r)
+#pragma unroll
+                for (int c = 0; c < r; ++c) {
+                    const double sy = 0.5 * (pc[r][c] + pc[c][r]);
+                    pc[r][c] = sy;
+                    pc[c][r] = sy;
+                }
+        }
+        // Z_{k+1} = mu/2 Z + 1/(2 mu) Z W
+        double dn = 0.0, nn = 0.0;
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const bool in = in_mm(r, c);
+                const double zn = in ? hm * z[r][c] - hi * pc[r][c] : delta(r, c);
+                const double dz = zn - z[r][c];
+                // off-diagonal blocks stand for themselves and their transposes
+                const double wgt = (ib == jb) ? 1.0 : 2.0;
+                dn += in ? wgt * dz * dz : 0.0;
+                nn += in ? wgt * zn * zn : 0.0;
+                z[r][c] = zn;
+            }
+        dn = group_sum<NT>(dn, tmp, lane);   // (its barriers also order the mst stores before the reads)
+        nn = group_sum<NT>(nn, tmp, lane);
+        if (last)
+            break;
+        if (dn <= 1e-20 * nn)
+            last = true;   // converged quadratically: one more unscaled step
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                mo[r][c] = mst[r * BS + c][lane];
+                x[r][c] = mo[r][c];
+            }
+        logdet = etkf_sweep<G>(x, col, m, br, bc, act, bad);
+    }
+    if (!last || bad)
+        flag = 1;   // a non-positive pivot, or no convergence in 60 steps: reported
+    __syncthreads();
+    if (lane == 0 && flag)
+        atomicExch(a.err, ENKFG_ERR_LINALG);
+
+    // ---- T = Z, relaxation T <- (T - I) alpha + I, FP32: Z one block row at a
+    // time through LDS (rows R BS.., from the blocks (R, bc) and, transposed,
+    // (br, R)), then written as whole rows of T
+    for (int R = 0; R < G && R * BS < md; ++R) {
+        __syncthreads();   // the previous slab has been written out
+        if (act && br == R) {
+#pragma unroll
+            for (int r = 0; r < BS; ++r)
+#pragma unroll
+                for (int c = 0; c < BS; ++c)
+                    slab[r][bc * BS + c] = z[r][c];
+        }
+        if (act && bc == R && br != R) {
+#pragma unroll
+            for (int c = 0; c < BS; ++c)
+#pragma unroll
+                for (int r = 0; r < BS; ++r)
+                    slab[c][br * BS + r] = z[r][c];
+        }
+        __syncthreads();
+        for (int k = lane; k < BS * m; k += NT) {
+            const int rr = k / m, e2 = k - rr * m;
+            const int e1 = R * BS + rr;
+            if (e1 < md) {
+                const double dlt = (e1 == e2) ? 1.0 : 0.0;
+                const double t = (slab[rr][e2] - dlt) * a.alpha + dlt;
+                const size_t o = ((size_t)node * md + e1) * m + e2;
+                a.T[o] = (float)t;
+                if (a.T64 != nullptr)
+                    a.T64[o] = t;
+            }
+        }
+    }
+}
+
 }  // namespace
 
 bool factor_wave_supported(const CalcArgs& a)
@@ -847,6 +1279,30 @@ hipError_t launch_factor_trace(hipStream_t s, const CalcArgs& a)
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_factor_etkf(hipStream_t s, const CalcArgs& a)
+{
+    if (a.nbatch == 0)
+        return hipSuccess;
+    const dim3 grid((unsigned)a.nbatch);
+    const int g = (a.m + kEtkfBS - 1) / kEtkfBS;   // blocks per side
+    if (g <= 8)
+        hipLaunchKernelGGL(factor_etkf_kernel<8>, grid, dim3(EtkfGeom<8>::NT), 0, s, a);
+    else if (g <= 12)
+        hipLaunchKernelGGL(factor_etkf_kernel<12>, grid, dim3(EtkfGeom<12>::NT), 0, s, a);
+    else if (g <= 16)
+        hipLaunchKernelGGL(factor_etkf_kernel<16>, grid, dim3(EtkfGeom<16>::NT), 0, s, a);
+    else if (g <= 20)
+        hipLaunchKernelGGL(factor_etkf_kernel<20>, grid, dim3(EtkfGeom<20>::NT), 0, s, a);
+    else if (g <= 24)
+        hipLaunchKernelGGL(factor_etkf_kernel<24>, grid, dim3(EtkfGeom<24>::NT), 0, s, a);
+    else
+        return hipErrorInvalidValue;
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !a.pertype)
+        return e;
+    return launch_factor_trace(s, a);   // per-type diagnostics, after the node's own
 }
 
 }  // namespace enkfg
