@@ -97,35 +97,35 @@ __global__ void permute_rows_kernel(const T* __restrict__ src, const int* __rest
 // item_groups_lo, chosen per obs-count bucket by the host's autotuner) of
 // one (node, obstype segment), with no workgroup barrier: the wave
 // stages its own chunks of IKC local obs through a private LDS area, so the
-// waves of one segment share nothing but L2 (the other waves' gathers of the
-// same rows hit there).  Per chunk: the gathers of chunk c+1 and the
+// waves of one segment share nothing but L2 (the other waves' copies of the
+// same rows hit there).  Per chunk: the copies of chunks c+1 and c+2 and the
 // list entries of chunk c+3 are in flight while chunk c feeds the MFMAs.
-// LDS writes of one lane and reads of another need no barrier: a wave
-// executes its LDS instructions in order, and the compiler keeps them in
-// order (the accesses may alias).  The taper is applied in FP64 as an
-// operand is formed, (double) S * lc, as the reference forms Sloc.
-// obs per chunk: four k-steps.  16 rather than 8: the calc pass on the tile
-// takes 888 instead of 904 ms -- the wave's LDS (16 KB) holds the kernel to
-// about 9 waves per CU, and the factor kernels of the previous batch, on the
-// other stream, then overlap it better (32: 984 ms, 222 VGPRs).
+// The taper is applied in FP64 as an operand is formed, (double) S * lc, as
+// the reference forms Sloc.
+// obs per chunk: four k-steps.  With register-staged chunks, 16 rather than 8
+// gave a calc pass on the tile of 888 instead of 904 ms: at 8 the Gram took
+// so many waves per CU that the factor kernels of the previous batch, on the
+// other stream, overlapped it worse (32: 984 ms, 222 VGPRs).
 constexpr int IKC = 16;
-static_assert(kListPad >= 4 * IKC, "list padding too small for the item Gram pipeline");
+// chunk buffers: the rows of chunks c+1 and c+2 are in flight while chunk c
+// feeds the MFMAs
+constexpr int IKB = 3;
+static_assert(kListPad >= (IKB + 1) * IKC, "list padding too small for the item Gram pipeline");
 
+// The list entries and S rows of a chunk are copied from global memory
+// straight into LDS (global_load_lds: lane l of an instruction writes its 4 or
+// 16 bytes at M0 + 4 l or M0 + 16 l), with no VGPRs held for data in flight.
+// Block-major rows: the 16 floats of block X (columns 16X ..) of row r at
+// rows[X][r][0..15]; a k-step reads rows 4s + kq, columns 16X + col, whose 64
+// floats then fall in 64 distinct banks with no padding.
 template <int MT>
 struct ItemLds {
     static constexpr int MP = 16 * MT;
-    static constexpr int RS = MP + 16;   // floats per row: the 4 rows of a k-step fall in distinct banks
-    alignas(16) float rows[2][IKC + 1][RS];   // raw S rows of chunk c in rows[c % 2]; row IKC: discard
-    double lc[4][IKC];         // tapers, chunk c in [c % 4]
-    double sl[2][IKC];         // s * lc, chunk c in [c % 2]
-    int id[4][IKC];            // rows of S (-1 past the end), chunk c in [c % 4]
-};
-
-template <int MT>
-struct ItemRegs {
-    static constexpr int NQ = (IKC * 4 * MT + 63) / 64;
-    float v[NQ][4];            // plain floats: the registers stay promotable across iterations
-    double s;                  // s of row lane % IKC
+    alignas(16) float rows[IKB][MT][IKC][16];   // chunk c in rows[c % IKB]
+    alignas(16) float sv[IKB][64];   // s of the chunk's rows as FP64 in dwords 0..31 (32..63: a copy)
+    // list entries of chunk c in meta[c % 4], as stored (past the list's end
+    // too): ids in dwords 0..15, tapers (FP64) in 16..47
+    alignas(16) unsigned meta[4][64];
 };
 
 // number of tiles of group G of NG
@@ -147,66 +147,71 @@ __device__ __attribute__((always_inline)) inline void item_tiles(const double (&
     }
 }
 
-__device__ __attribute__((always_inline)) inline void item_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c)
+// the copy of the list entries k0 .. k0 + 15 into meta[slot] (lanes 48..63
+// repeat the ids into dwords 48..63).  Past the list's end the entries are
+// read all the same (the list arrays are padded) and masked where used.
+template <int MT>
+__device__ __attribute__((always_inline)) inline void item_meta_copy(const CalcArgs& a, ItemLds<MT>& sh, long long off, int k0, int slot)
 {
-    const int k = k0 + (int)(threadIdx.x & (IKC - 1));
-    const int i = a.ids[off + k];
-    const double l = a.lc[off + k];
-    const bool in = k < cnt;
-    id = in ? i : -1;
-    c = in ? l : 0.0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int lane = threadIdx.x & 63;
+    const unsigned* src = (lane >= 16 && lane < 48)
+                              ? reinterpret_cast<const unsigned*>(a.lc + off + k0 + ((lane - 16) >> 1)) + (lane & 1)
+                              : reinterpret_cast<const unsigned*>(a.ids + off + k0 + (lane & 15));
+    __builtin_amdgcn_global_load_lds(src, &sh.meta[slot][0], 4, 0, 0);
+#endif
 }
 
+// the MT + 1 copies of the rows of chunk k0 / IKC (list entries in
+// meta[slot]) into rows[buf] and sv[buf]: instruction X brings block X of the
+// 16 rows (lane l: row l / 4, columns 16X + 4 (l % 4) ..), the last one s of
+// the rows.  Rows past the list's end and the columns ms .. MP of the last
+// block come from the zero row (row nobs of S, s = 0).
 template <int MT>
-__device__ __attribute__((always_inline)) inline void item_meta_store(ItemLds<MT>& sh, int slot, int id, double c)
+__device__ __attribute__((always_inline)) inline void item_copy(const CalcArgs& a, ItemLds<MT>& sh, int slot, int buf,
+                                                                int k0, int cnt)
 {
-    // lanes l and l + IKC hold the same entry: every lane stores (a store
-    // under a branch would leave its load counted as in flight past it)
-    sh.id[slot][threadIdx.x & (IKC - 1)] = id;
-    sh.lc[slot][threadIdx.x & (IKC - 1)] = c;
+    // the builtin's LDS operand has no form in the host pass, which never
+    // emits this function
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int lane = threadIdx.x & 63;
+    const int r = lane >> 2;
+    const int rs = (lane & 31) >> 1;
+    // loaded unconditionally, then selected (the compiler would otherwise
+    // move the LDS reads under a branch)
+    int idr = (int)sh.meta[slot][r];
+    int sidr = (int)sh.meta[slot][rs];
+    asm volatile("" : "+v"(idr), "+v"(sidr));
+    const int id = k0 + r < cnt ? idr : a.nobs;
+    const int sid = k0 + rs < cnt ? sidr : a.nobs;
+    const float* zero = a.S + (size_t)a.nobs * a.ms;
+    const float* row = a.S + (size_t)id * a.ms + 4 * (lane & 3);
+#pragma unroll
+    for (int X = 0; X < MT - 1; ++X)
+        __builtin_amdgcn_global_load_lds(row + 16 * X, &sh.rows[buf][X][0][0], 16, 0, 0);
+    const bool inrow = 16 * (MT - 1) + 4 * (lane & 3) < a.ms;
+    __builtin_amdgcn_global_load_lds(inrow ? row + 16 * (MT - 1) : zero, &sh.rows[buf][MT - 1][0][0], 16, 0, 0);
+    const unsigned* s = reinterpret_cast<const unsigned*>(a.sv + sid) + (lane & 1);
+    __builtin_amdgcn_global_load_lds(s, &sh.sv[buf][0], 4, 0, 0);
+#endif
 }
 
+// Waits for the copies issued before the last MT + 1 vector memory
+// instructions.  The compiler does not count the LDS writes of
+// global_load_lds against the LDS reads that follow, so the chunk pipeline
+// waits by hand; the memory clobber keeps the compiler's loads and stores on
+// their side of the wait.
 template <int MT>
-__device__ __attribute__((always_inline)) inline void item_gather(const CalcArgs& a, const ItemLds<MT>& sh, int slot, ItemRegs<MT>& L)
+__device__ __attribute__((always_inline)) inline void item_wait()
 {
-    constexpr int NQ = ItemRegs<MT>::NQ;
-    const int qr = a.ms >> 2;
-    // all ids first (one wait), then the loads
-    long long r[NQ];
-    int qd[NQ];
-#pragma unroll
-    for (int j = 0; j < NQ; ++j) {
-        const int q = (int)threadIdx.x + 64 * j;
-        const int row = q / qr;
-        qd[j] = q - row * qr;
-        const int id = sh.id[slot][row < IKC ? row : IKC - 1];
-        r[j] = (row < IKC && id >= 0) ? id : a.nobs;   // the zero row
-    }
-    const int sid = sh.id[slot][threadIdx.x & (IKC - 1)];
-#pragma unroll
-    for (int j = 0; j < NQ; ++j) {
-        const float4 q = *reinterpret_cast<const float4*>(a.S + r[j] * a.ms + 4 * qd[j]);
-        L.v[j][0] = q.x;
-        L.v[j][1] = q.y;
-        L.v[j][2] = q.z;
-        L.v[j][3] = q.w;
-    }
-    L.s = a.sv[sid >= 0 ? sid : a.nobs];
+    static_assert(MT + 1 < 16, "vmcnt field");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(MT + 1) : "memory");
 }
 
-template <int MT>
-__device__ __attribute__((always_inline)) inline void item_stage(const CalcArgs& a, ItemLds<MT>& sh, int rbuf, int mslot, const ItemRegs<MT>& L)
+__device__ __attribute__((always_inline)) inline void item_fence()
 {
-    const int qr = a.ms >> 2;
-#pragma unroll
-    for (int j = 0; j < ItemRegs<MT>::NQ; ++j) {
-        const int q = (int)threadIdx.x + 64 * j;
-        const int row = q / qr;
-        const int qd = q - row * qr;
-        *reinterpret_cast<float4*>(&sh.rows[rbuf][row < IKC ? row : IKC][4 * qd]) =
-            make_float4(L.v[j][0], L.v[j][1], L.v[j][2], L.v[j][3]);
-    }
-    sh.sl[rbuf][threadIdx.x & (IKC - 1)] = L.s * sh.lc[mslot][threadIdx.x & (IKC - 1)];
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 template <int MT, int NG, int G>
@@ -218,9 +223,6 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
     const int kq = lane >> 4;
     const int col = lane & 15;
 
-    // padding columns (ms .. RS) are never written by the staging: zero once
-    for (int q = lane; q < 2 * (IKC + 1) * ItemLds<MT>::RS; q += 64)
-        (&sh.rows[0][0][0])[q] = 0.0f;
     dbl4 acc[NTG > 0 ? NTG : 1];
 #pragma unroll
     for (int f = 0; f < NTG; ++f)
@@ -231,48 +233,57 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
         sts[X] = 0.0;
 
     const int nchunk = (cnt + IKC - 1) / IKC;
-    int mid;
-    double mc;
-    // prologue: list entries of chunks 0..2 in LDS, chunk 0 staged, chunk 1
-    // gathered into registers
-    item_meta_load(a, off, cnt, 0, mid, mc);
-    item_meta_store<MT>(sh, 0, mid, mc);
-    item_meta_load(a, off, cnt, IKC, mid, mc);
-    item_meta_store<MT>(sh, 1, mid, mc);
-    item_meta_load(a, off, cnt, 2 * IKC, mid, mc);
-    item_meta_store<MT>(sh, 2, mid, mc);
-    ItemRegs<MT> L;
-    item_gather<MT>(a, sh, 0, L);
-    item_stage<MT>(a, sh, 0, 0, L);
-    item_gather<MT>(a, sh, 1, L);
+    // prologue, in this order of issue, which the wait at the top of the
+    // loop counts on: list entries of chunks 0 and 1, copies of chunk 0,
+    // list entries of chunk 2, copies of chunk 1
+    item_meta_copy<MT>(a, sh, off, 0, 0);
+    item_meta_copy<MT>(a, sh, off, IKC, 1);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    item_fence();
+    item_copy<MT>(a, sh, 0, 0, 0, cnt);
+    item_fence();
+    item_meta_copy<MT>(a, sh, off, 2 * IKC, 2);
+    item_fence();
+    item_copy<MT>(a, sh, 1, 1, IKC, cnt);
+    item_fence();
+    int buf = 0;
     for (int c = 0; c < nchunk; ++c) {
-        // chunk c+3's list entries: loaded now, stored after the MFMAs
-        item_meta_load(a, off, cnt, (c + 3) * IKC, mid, mc);
-        __builtin_amdgcn_sched_barrier(0);
-        const int buf = c & 1;
+        // in flight, oldest first: copies of chunk c, list entries of chunk
+        // c+2, copies of chunk c+1.  Wait for all but the last group.
+        item_wait<MT>();
+        item_fence();
+        // list entries of chunk c+3 and copies of chunk c+2, into the
+        // buffers chunk c-1 was read from.  Past the end these read padding
+        // and the zero row, issued all the same: the count of the wait
+        // above takes every iteration to issue them.
+        item_meta_copy<MT>(a, sh, off, (c + 3) * IKC, (c + 3) & 3);
+        item_fence();
+        item_copy<MT>(a, sh, (c + 2) & 3, buf == 0 ? IKB - 1 : buf - 1, (c + 2) * IKC, cnt);
+        item_fence();
+        const double* svc = reinterpret_cast<const double*>(&sh.sv[buf][0]);
+        const double* lcc = reinterpret_cast<const double*>(&sh.meta[c & 3][16]);
 #pragma unroll
         for (int s = 0; s < IKC / 4; ++s) {
             const int r = 4 * s + kq;
-            const double l = sh.lc[c & 3][r];
+            const double lr = lcc[r];
+            const double l = c * IKC + r < cnt ? lr : 0.0;
             double frag[MT];
 #pragma unroll
             for (int X = 0; X < MT; ++X)
-                frag[X] = (double)sh.rows[buf][r][16 * X + col] * l;
+                frag[X] = (double)sh.rows[buf][X][r][col] * l;
             item_tiles<MT, NG, G, 0>(frag, acc);
             if (G == 0) {
-                const double sl = sh.sl[buf][r];
+                const double sl = svc[r] * l;
 #pragma unroll
                 for (int X = 0; X < MT; ++X)
                     sts[X] += frag[X] * sl;
             }
         }
-        __builtin_amdgcn_sched_barrier(0);
-        // stage chunk c+1 (gathered during chunk c-1 or the prologue) and
-        // gather chunk c+2; past the end these touch the zero row only
-        item_stage<MT>(a, sh, (c + 1) & 1, (c + 1) & 3, L);
-        item_gather<MT>(a, sh, (c + 2) & 3, L);
-        item_meta_store<MT>(sh, (c + 3) & 3, mid, mc);
+        item_fence();
+        buf = buf == IKB - 1 ? 0 : buf + 1;
     }
+    // the copies of the chunks past the end land before the wave ends
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     // outputs in the factor kernel's register-block layout, both triangles
     // (accumulator register i of lane l holds D[4 i + l / 16][l % 16])
     double* gout = a.gram + (size_t)b * MP * MP;
