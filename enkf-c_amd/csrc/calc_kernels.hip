@@ -107,9 +107,12 @@ __global__ void permute_rows_kernel(const T* __restrict__ src, const int* __rest
 // so many waves per CU that the factor kernels of the previous batch, on the
 // other stream, overlapped it worse (32: 984 ms, 222 VGPRs).
 constexpr int IKC = 16;
-// chunk buffers: the rows of chunks c+1 and c+2 are in flight while chunk c
-// feeds the MFMAs
+// chunk buffers: the rows of chunks c+1 .. c+IKB-1 are in flight while chunk
+// c feeds the MFMAs; list entries are copied one chunk further ahead, into
+// IKM slots
 constexpr int IKB = 3;
+constexpr int IKM = IKB + 1;
+static_assert(IKB >= 3, "the wait at the top of the chunk loop counts one chunk's copies after the list entries");
 static_assert(kListPad >= (IKB + 1) * IKC, "list padding too small for the item Gram pipeline");
 
 // The list entries and S rows of a chunk are copied from global memory
@@ -123,9 +126,9 @@ struct ItemLds {
     static constexpr int MP = 16 * MT;
     alignas(16) float rows[IKB][MT][IKC][16];   // chunk c in rows[c % IKB]
     alignas(16) float sv[IKB][64];   // s of the chunk's rows as FP64 in dwords 0..31 (32..63: a copy)
-    // list entries of chunk c in meta[c % 4], as stored (past the list's end
-    // too): ids in dwords 0..15, tapers (FP64) in 16..47
-    alignas(16) unsigned meta[4][64];
+    // list entries of chunk c in meta[c % IKM], as stored (past the list's
+    // end too): ids in dwords 0..15, tapers (FP64) in 16..47
+    alignas(16) unsigned meta[IKM][64];
 };
 
 // number of tiles of group G of NG
@@ -234,34 +237,40 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
 
     const int nchunk = (cnt + IKC - 1) / IKC;
     // prologue, in this order of issue, which the wait at the top of the
-    // loop counts on: list entries of chunks 0 and 1, copies of chunk 0,
-    // list entries of chunk 2, copies of chunk 1
-    item_meta_copy<MT>(a, sh, off, 0, 0);
-    item_meta_copy<MT>(a, sh, off, IKC, 1);
+    // loop counts on: list entries of chunks 0 .. IKB-2, copies of chunks
+    // 0 .. IKB-3, list entries of chunk IKB-1, copies of chunk IKB-2
+#pragma unroll
+    for (int p = 0; p < IKB - 1; ++p)
+        item_meta_copy<MT>(a, sh, off, p * IKC, p);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     item_fence();
-    item_copy<MT>(a, sh, 0, 0, 0, cnt);
+#pragma unroll
+    for (int p = 0; p < IKB - 2; ++p)
+        item_copy<MT>(a, sh, p, p, p * IKC, cnt);
     item_fence();
-    item_meta_copy<MT>(a, sh, off, 2 * IKC, 2);
+    item_meta_copy<MT>(a, sh, off, (IKB - 1) * IKC, IKB - 1);
     item_fence();
-    item_copy<MT>(a, sh, 1, 1, IKC, cnt);
+    item_copy<MT>(a, sh, IKB - 2, IKB - 2, (IKB - 2) * IKC, cnt);
     item_fence();
-    int buf = 0;
+    int buf = 0;    // c % IKB
+    int slot = 0;   // c % IKM
     for (int c = 0; c < nchunk; ++c) {
-        // in flight, oldest first: copies of chunk c, list entries of chunk
-        // c+2, copies of chunk c+1.  Wait for all but the last group.
+        // in flight, oldest first: copies of chunks c .. c+IKB-3, list
+        // entries of chunk c+IKB-1, copies of chunk c+IKB-2.  Wait for all
+        // but the last group.
         item_wait<MT>();
         item_fence();
-        // list entries of chunk c+3 and copies of chunk c+2, into the
-        // buffers chunk c-1 was read from.  Past the end these read padding
-        // and the zero row, issued all the same: the count of the wait
-        // above takes every iteration to issue them.
-        item_meta_copy<MT>(a, sh, off, (c + 3) * IKC, (c + 3) & 3);
+        // list entries of chunk c+IKB and copies of chunk c+IKB-1, into the
+        // slot and buffer chunk c-1 was read from.  Past the end these read
+        // padding and the zero row, issued all the same: the count of the
+        // wait above takes every iteration to issue them.
+        item_meta_copy<MT>(a, sh, off, (c + IKB) * IKC, slot == 0 ? IKM - 1 : slot - 1);
         item_fence();
-        item_copy<MT>(a, sh, (c + 2) & 3, buf == 0 ? IKB - 1 : buf - 1, (c + 2) * IKC, cnt);
+        item_copy<MT>(a, sh, slot >= 2 ? slot - 2 : slot + IKM - 2, buf == 0 ? IKB - 1 : buf - 1,
+                      (c + IKB - 1) * IKC, cnt);
         item_fence();
         const double* svc = reinterpret_cast<const double*>(&sh.sv[buf][0]);
-        const double* lcc = reinterpret_cast<const double*>(&sh.meta[c & 3][16]);
+        const double* lcc = reinterpret_cast<const double*>(&sh.meta[slot][16]);
 #pragma unroll
         for (int s = 0; s < IKC / 4; ++s) {
             const int r = 4 * s + kq;
@@ -281,6 +290,7 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
         }
         item_fence();
         buf = buf == IKB - 1 ? 0 : buf + 1;
+        slot = slot == IKM - 1 ? 0 : slot + 1;
     }
     // the copies of the chunks past the end land before the wave ends
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
