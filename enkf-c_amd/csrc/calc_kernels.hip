@@ -112,7 +112,7 @@ constexpr int IKC = 16;
 // IKM slots
 constexpr int IKB = 3;
 constexpr int IKM = IKB + 1;
-static_assert(IKB >= 3, "the wait at the top of the chunk loop counts one chunk's copies after the list entries");
+static_assert(IKB >= 2, "at least two chunk buffers");
 static_assert(kListPad >= (IKB + 1) * IKC, "list padding too small for the item Gram pipeline");
 
 // The list entries and S rows of a chunk are copied from global memory
@@ -208,7 +208,10 @@ template <int MT>
 __device__ __attribute__((always_inline)) inline void item_wait()
 {
     static_assert(MT + 1 < 16, "vmcnt field");
-    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(MT + 1) : "memory");
+    if constexpr (IKB >= 3)
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(MT + 1) : "memory");
+    else
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 __device__ __attribute__((always_inline)) inline void item_fence()
@@ -362,15 +365,14 @@ __device__ __attribute__((always_inline)) inline void item_run(const CalcArgs& a
 template <int MT, int NG>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gram_item_kernel(CalcArgs a)
 {
-    // one item per wave.  Blocks are dispatched to the 8 XCDs round-robin
-    // (and handed out as earlier ones finish: the hardware dispatcher is the
-    // work queue); each XCD takes a contiguous range of (segment, group)
-    // items, the NG groups of a segment adjacent, so neighbouring nodes and
-    // the groups of one segment share that XCD's L2
+    // one item per wave.  Blocks are handed out as earlier ones finish (the
+    // hardware dispatcher is the work queue); each XCD takes runs of
+    // kItemRun consecutive (segment, group) items, the NG groups of a
+    // segment adjacent, so neighbouring nodes and the groups of one segment
+    // share that XCD's L2
     __shared__ ItemLds<MT> sh;
     const int nitem = a.nbatch * a.nseg * NG;
-    const int per = (nitem + 7) / 8;
-    const int it = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    const int it = item_of_block();
     if (it < nitem)
         item_run<MT, NG>(a, it, sh);
 }
@@ -939,7 +941,7 @@ hipError_t launch_gram(hipStream_t s, const CalcArgs& a)
     const int lo = item_groups_lo(MT);
     const int ng = (a.tile_groups == lo) ? lo : lo + 1;
     const long long nitem = (long long)a.nbatch * a.nseg * ng;
-    const dim3 igrid((unsigned)(8 * ((nitem + 7) / 8)));
+    const dim3 igrid(item_grid(nitem));
     switch (MT) {
     case 1: launch_gram_item<1>(s, igrid, a, ng); break;
     case 2: launch_gram_item<2>(s, igrid, a, ng); break;

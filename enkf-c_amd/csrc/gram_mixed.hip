@@ -546,15 +546,14 @@ __device__ __attribute__((always_inline)) inline void mix_item_run(const CalcArg
 template <int MT, int NG>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) gram_mixed_item_kernel(CalcArgs a)
 {
-    // one item per wave.  Blocks are dispatched to the 8 XCDs round-robin
-    // (and handed out as earlier ones finish: the hardware dispatcher is the
-    // work queue); each XCD takes a contiguous range of (segment, group)
-    // items, the NG groups of a segment adjacent, so neighbouring nodes and
-    // the groups of one segment share that XCD's L2
+    // one item per wave.  Blocks are handed out as earlier ones finish (the
+    // hardware dispatcher is the work queue); each XCD takes runs of
+    // kItemRun consecutive (segment, group) items, the NG groups of a
+    // segment adjacent, so neighbouring nodes and the groups of one segment
+    // share that XCD's L2
     __shared__ MixItemLds<MT> sh;
     const int nitem = a.nbatch * a.nseg * NG;
-    const int per = (nitem + 7) / 8;
-    const int it = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    const int it = item_of_block();
     if (it < nitem)
         mix_item_run<MT, NG>(a, it, sh);
 }
@@ -585,7 +584,7 @@ hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
         const int lo = item_groups_lo(MT);
         const int ng = (a.tile_groups == lo + 1) ? lo + 1 : lo;
         const long long nitem = (long long)a.nbatch * a.nseg * ng;
-        const dim3 igrid((unsigned)(8 * ((nitem + 7) / 8)));
+        const dim3 igrid(item_grid(nitem));
         switch (MT) {
         case 1: launch_mix_item<1>(s, igrid, a, ng); break;
         case 2: launch_mix_item<2>(s, igrid, a, ng); break;
