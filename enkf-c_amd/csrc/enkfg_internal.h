@@ -116,6 +116,9 @@ struct CalcArgs {
     // (when set), and the factor kernel returns z = M^-1 g (adj_g: [node][m])
     // and w = M^-1 S^T s per listed node (adj_z, adj_w: [list index][m])
     const int* node_list;
+    // item Gram kernels: the segments in the order their items run ([nseg],
+    // obstypes by decreasing total list length), or null for 0, 1, ...
+    const int* seg_order;
     const double* adj_g;
     double* adj_z;
     double* adj_w;

@@ -39,6 +39,32 @@ __device__ inline int item_of_block()
     return ((j / kItemRun) * 8 + x) * kItemRun + j % kItemRun;
 }
 
+// Item `it` of an item Gram launch with NG tile groups per segment: node
+// slot bn of the batch, segment seg, group g.  Items run segment by segment
+// in the order a.seg_order (obstypes by decreasing total list length), the
+// batch's nodes in order within a segment and the NG groups of a (node,
+// segment) adjacent.  So the longest items start first and a launch ends on
+// short ones: with the nodes' four segments interleaved, the last few of the
+// longest items ran alone at the end of each launch (in a copy of the chunk
+// loop with 231, 116, 20 and 20 chunks per segment, 47 against 64 TFLOP/s
+// for uniform items).  Within a run of kItemRun items the nodes are
+// neighbours with lists of one obstype, which share rows in L2.
+struct ItemRef {
+    int bn, seg, g;
+};
+
+__device__ inline ItemRef item_ref(const CalcArgs& a, int it, int NG)
+{
+    const int per = a.nbatch * NG;
+    const int sr = it / per;
+    const int rem = it - sr * per;
+    ItemRef r;
+    r.bn = rem / NG;
+    r.g = rem - r.bn * NG;
+    r.seg = a.seg_order ? a.seg_order[sr] : sr;
+    return r;
+}
+
 // blocks of an item Gram launch: a whole number of rounds of 8 runs, over
 // which item_of_block is one to one
 inline unsigned item_grid(long long nitem)

@@ -526,10 +526,10 @@ __device__ __attribute__((always_inline)) inline void mix_item_dispatch(int g, c
 template <int MT, int NG>
 __device__ __attribute__((always_inline)) inline void mix_item_run(const CalcArgs& a, int it, MixItemLds<MT>& sh)
 {
-    const int b = it / NG;
-    const int g = it - b * NG;
-    const int node = a.node_list ? a.node_list[a.node0 + b / a.nseg] : a.node0 + b / a.nseg;
-    const int seg = b % a.nseg;
+    const ItemRef r = item_ref(a, it, NG);
+    const int g = r.g, seg = r.seg;
+    const int b = r.bn * a.nseg + seg;   // the (node, segment) slot of the Gram scratch
+    const int node = a.node_list ? a.node_list[a.node0 + r.bn] : a.node0 + r.bn;
     long long off;
     int cnt;
     if (a.pertype) {
