@@ -41,7 +41,16 @@ namespace {
 
 constexpr float kMaxVal = 999999.0f;   // MAXOBSVAL, common/global.h:119
 constexpr float kEps = 1.0e-6f;        // EPSF, update/update.c:41
-constexpr int kWaves = 8;              // waves per workgroup, one field tile each
+// waves per workgroup (one cell), one field tile each.  Mixed up to m = 96:
+// 4, so that four workgroups (cells) share a CU (40 KB of LDS and 128 VGPRs
+// each at m = 96): 85.4 -> 75.4 ms on the tile.  FP32: 8; with 4 its 43 KB
+// of LDS allow three workgroups per CU and the kernel measured 109 against
+// 107 ms.  Larger ensembles: 8 (their LDS allows one or two workgroups).
+template <int KT, bool MIX>
+constexpr int upd_waves()
+{
+    return (MIX && KT <= 6) ? 4 : 8;
+}
 
 typedef float flt4 __attribute__((__vector_size__(4 * sizeof(float))));
 typedef _Float16 half8 __attribute__((__vector_size__(8 * sizeof(_Float16))));
@@ -223,7 +232,7 @@ __device__ __attribute__((always_inline)) inline float interp(const Bilin& b, fl
 template <int KT, int OT, bool MIX, bool VEC, int CASE>
 __device__ inline void form_t(const UpdArgs& a, const Bilin& b, TileLds<KT, OT, MIX>& sh, int tid)
 {
-    constexpr int NT = 64 * kWaves;
+    constexpr int NT = 64 * upd_waves<KT, MIX>();
     constexpr bool J = (CASE & 1) != 0, I = (CASE & 2) != 0;
     const int m = a.m, md = a.md;
     if constexpr (VEC) {
@@ -305,7 +314,7 @@ constexpr int tile_waves()
 // compile-time constant (held per member, the tests would take two scalar
 // registers each)
 template <int KT, int OT, bool MIX, bool VEC, bool FULL>
-__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(tile_waves<KT>())))
+__global__ void __launch_bounds__((64 * upd_waves<KT, MIX>())) __attribute__((amdgpu_waves_per_eu(tile_waves<KT>())))
 update_tile_kernel(UpdArgs a)
 {
     using G = TileGeom<KT, OT, MIX>;
@@ -355,16 +364,16 @@ update_tile_kernel(UpdArgs a)
         constexpr int NKX = MIX ? G::NKH : G::NK;
         if (m < NKX) {
             const int np = NKX - m;
-            for (int q = tid; q < G::NO * np; q += 64 * kWaves)
+            for (int q = tid; q < G::NO * np; q += 64 * upd_waves<KT, MIX>())
                 store_t<KT, OT, MIX>(sh, q / np, m + q % np, 0.0f);
         }
         if (md < G::NO) {
             const int nr = G::NO - md;
-            for (int q = tid; q < nr * m; q += 64 * kWaves)
+            for (int q = tid; q < nr * m; q += 64 * upd_waves<KT, MIX>())
                 store_t<KT, OT, MIX>(sh, md + q / m, q % m, 0.0f);
         }
         const Bilin bw = make_bilin(a, j, i, m);
-        for (int e = tid; e < NKX; e += 64 * kWaves) {
+        for (int e = tid; e < NKX; e += 64 * upd_waves<KT, MIX>()) {
             const int ec = min(e, m - 1);
             const float v = bilin(bw, a.w[bw.o00 + ec], a.w[bw.o10 + ec], a.w[bw.o01 + ec], a.w[bw.o11 + ec]);
             sh.wc[e] = (e < m) ? v : 0.0f;
@@ -372,7 +381,7 @@ update_tile_kernel(UpdArgs a)
     }
     __syncthreads();
 
-    for (int t = w; t < ntiles; t += kWaves) {
+    for (int t = w; t < ntiles; t += upd_waves<KT, MIX>()) {
         if (t != w)
             load_tile<KT, VEC>(a, cell, t, col, kq, x);
 
@@ -567,7 +576,7 @@ update_tile_kernel(UpdArgs a)
 template <int KT, int OT, bool MIX>
 void launch_ktm(hipStream_t s, const UpdArgs& a, dim3 grid)
 {
-    const dim3 block(64 * kWaves);
+    const dim3 block(64 * upd_waves<KT, MIX>());
     if (a.m == 16 * KT && a.md == 16 * OT)
         hipLaunchKernelGGL((update_tile_kernel<KT, OT, MIX, true, true>), grid, block, 0, s, a);
     else if (a.m % 4 == 0)
