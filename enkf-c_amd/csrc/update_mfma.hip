@@ -41,15 +41,15 @@ namespace {
 
 constexpr float kMaxVal = 999999.0f;   // MAXOBSVAL, common/global.h:119
 constexpr float kEps = 1.0e-6f;        // EPSF, update/update.c:41
-// waves per workgroup (one cell), one field tile each.  Mixed up to m = 96:
-// 4, so that four workgroups (cells) share a CU (40 KB of LDS and 128 VGPRs
-// each at m = 96): 85.4 -> 75.4 ms on the tile.  FP32: 8; with 4 its 43 KB
-// of LDS allow three workgroups per CU and the kernel measured 109 against
-// 107 ms.  Larger ensembles: 8 (their LDS allows one or two workgroups).
-template <int KT, bool MIX>
+// waves per workgroup (one cell), one field tile each: 4 up to m = 96, so
+// that four workgroups (cells) share a CU (128 VGPRs, 40 KB of LDS mixed and
+// 37 KB FP32 at m = 96): mixed 85.4 -> 75.4 ms on the tile.  The FP32 T_cell
+// rows, padded to 112 floats, took 43 KB and allowed three.  Larger
+// ensembles: 8 (their LDS allows one or two workgroups).
+template <int KT, int OT, bool MIX>
 constexpr int upd_waves()
 {
-    return (MIX && KT <= 6) ? 4 : 8;
+    return (KT <= 6 && (MIX || OT % 2 == 0)) ? 4 : 8;
 }
 
 typedef float flt4 __attribute__((__vector_size__(4 * sizeof(float))));
@@ -63,16 +63,19 @@ struct TileGeom {
     static constexpr int NK = 16 * KT;                        // summation length, padded
     static constexpr int NKH = 32 * ((KT + 1) / 2);           // ... for the K = 32 half-precision steps
     static constexpr int NO = 16 * OT;                        // output members, padded
-    // FP32 row stride of tk == 16 or 48 (mod 64) floats: the four rows a
-    // B-operand read touches fall in disjoint banks
-    static constexpr int LD = NO + ((16 - NO % 32) + 32) % 32;
+    // FP32 rows of tk: with an even OT the 16-column blocks of rows k with
+    // bit 1 of k set are swapped in pairs (block J at J ^ 1), so that the
+    // four rows a B-operand read touches fall in disjoint banks with no
+    // padding; with an odd OT the row stride is 16 or 48 (mod 64) floats
+    static constexpr bool SWAP = OT % 2 == 0;
+    static constexpr int LD = SWAP ? NO : NO + ((16 - NO % 32) + 32) % 32;
     static constexpr int HS = NKH + 8;                        // half row stride of th/tl (16-byte rows)
 };
 
 template <int KT, int OT, bool MIX>
 struct TileLds {
     using G = TileGeom<KT, OT, MIX>;
-    // FP32: tk[k][e1 ^ (k / 16)] = T_cell[e1][e2], k = kpos32(e2) (the column
+    // FP32: tk[k][tk_col(k, e1)] = T_cell[e1][e2], k = kpos32(e2) (the column
     // swizzle within blocks of 16 spreads the rows that the formation's
     // stores touch together over the banks; a B-operand read still takes 16
     // consecutive columns)
@@ -176,6 +179,13 @@ __device__ __attribute__((always_inline)) inline void load_tile(const UpdArgs& a
     }
 }
 
+// column of T_cell row e1 in FP32 row k of tk
+template <int KT, int OT, bool MIX>
+__device__ inline int tk_col(int k, int e1)
+{
+    return e1 ^ (k >> 4) ^ (TileGeom<KT, OT, MIX>::SWAP ? 16 * ((k >> 1) & 1) : 0);
+}
+
 template <int KT, int OT, bool MIX>
 __device__ inline void store_t(TileLds<KT, OT, MIX>& sh, int e1, int e2, float t)
 {
@@ -185,7 +195,7 @@ __device__ inline void store_t(TileLds<KT, OT, MIX>& sh, int e1, int e2, float t
         sh.tl[e1][kpos16(e2)] = (_Float16)(t - (float)h);
     } else {
         const int k = kpos32(e2);
-        sh.tk[k][e1 ^ (k >> 4)] = t;
+        sh.tk[k][tk_col<KT, OT, MIX>(k, e1)] = t;
     }
 }
 
@@ -207,7 +217,7 @@ __device__ inline void store_t4(TileLds<KT, OT, MIX>& sh, int e1, int e2, const 
         const int k = kpos32(e2);   // positions k, k + 4, k + 8, k + 12 (one block of 16)
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            sh.tk[k + 4 * c][e1 ^ (k >> 4)] = t[c];
+            sh.tk[k + 4 * c][tk_col<KT, OT, MIX>(k + 4 * c, e1)] = t[c];
     }
 }
 
@@ -232,7 +242,7 @@ __device__ __attribute__((always_inline)) inline float interp(const Bilin& b, fl
 template <int KT, int OT, bool MIX, bool VEC, int CASE>
 __device__ inline void form_t(const UpdArgs& a, const Bilin& b, TileLds<KT, OT, MIX>& sh, int tid)
 {
-    constexpr int NT = 64 * upd_waves<KT, MIX>();
+    constexpr int NT = 64 * upd_waves<KT, OT, MIX>();
     constexpr bool J = (CASE & 1) != 0, I = (CASE & 2) != 0;
     const int m = a.m, md = a.md;
     if constexpr (VEC) {
@@ -314,7 +324,7 @@ constexpr int tile_waves()
 // compile-time constant (held per member, the tests would take two scalar
 // registers each)
 template <int KT, int OT, bool MIX, bool VEC, bool FULL>
-__global__ void __launch_bounds__((64 * upd_waves<KT, MIX>())) __attribute__((amdgpu_waves_per_eu(tile_waves<KT>())))
+__global__ void __launch_bounds__((64 * upd_waves<KT, OT, MIX>())) __attribute__((amdgpu_waves_per_eu(tile_waves<KT>())))
 update_tile_kernel(UpdArgs a)
 {
     using G = TileGeom<KT, OT, MIX>;
@@ -364,16 +374,16 @@ update_tile_kernel(UpdArgs a)
         constexpr int NKX = MIX ? G::NKH : G::NK;
         if (m < NKX) {
             const int np = NKX - m;
-            for (int q = tid; q < G::NO * np; q += 64 * upd_waves<KT, MIX>())
+            for (int q = tid; q < G::NO * np; q += 64 * upd_waves<KT, OT, MIX>())
                 store_t<KT, OT, MIX>(sh, q / np, m + q % np, 0.0f);
         }
         if (md < G::NO) {
             const int nr = G::NO - md;
-            for (int q = tid; q < nr * m; q += 64 * upd_waves<KT, MIX>())
+            for (int q = tid; q < nr * m; q += 64 * upd_waves<KT, OT, MIX>())
                 store_t<KT, OT, MIX>(sh, md + q / m, q % m, 0.0f);
         }
         const Bilin bw = make_bilin(a, j, i, m);
-        for (int e = tid; e < NKX; e += 64 * upd_waves<KT, MIX>()) {
+        for (int e = tid; e < NKX; e += 64 * upd_waves<KT, OT, MIX>()) {
             const int ec = min(e, m - 1);
             const float v = bilin(bw, a.w[bw.o00 + ec], a.w[bw.o10 + ec], a.w[bw.o01 + ec], a.w[bw.o11 + ec]);
             sh.wc[e] = (e < m) ? v : 0.0f;
@@ -381,7 +391,7 @@ update_tile_kernel(UpdArgs a)
     }
     __syncthreads();
 
-    for (int t = w; t < ntiles; t += upd_waves<KT, MIX>()) {
+    for (int t = w; t < ntiles; t += upd_waves<KT, OT, MIX>()) {
         if (t != w)
             load_tile<KT, VEC>(a, cell, t, col, kq, x);
 
@@ -468,14 +478,20 @@ update_tile_kernel(UpdArgs a)
         int boff = col;
         asm volatile("" : "+v"(boff));
         if constexpr (!MIX) {
+            // block J of the lane's row at 16 (J ^ f), f = bit 1 of kq when
+            // the blocks are swapped (tk_col)
+            const int f = G::SWAP ? (kq >> 1) & 1 : 0;
 #pragma unroll
             for (int s = 0; s < KT; ++s)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const float* brow = &sh.tk[16 * s + 4 * c + kq][0] + (boff ^ s);
+                    const float* beven = brow + 16 * f;   // blocks J even: 16 J + 16 f
+                    const float* bodd = brow - 16 * f;    // J odd: 16 J - 16 f
 #pragma unroll
                     for (int J = 0; J < OT; ++J)
-                        acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s][c], brow[16 * J], acc[J], 0, 0, 0);
+                        acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[s][c], (J & 1) ? bodd[16 * J] : beven[16 * J],
+                                                                      acc[J], 0, 0, 0);
                 }
         } else {
 #pragma unroll
@@ -576,7 +592,7 @@ update_tile_kernel(UpdArgs a)
 template <int KT, int OT, bool MIX>
 void launch_ktm(hipStream_t s, const UpdArgs& a, dim3 grid)
 {
-    const dim3 block(64 * upd_waves<KT, MIX>());
+    const dim3 block(64 * upd_waves<KT, OT, MIX>());
     if (a.m == 16 * KT && a.md == 16 * OT)
         hipLaunchKernelGGL((update_tile_kernel<KT, OT, MIX, true, true>), grid, block, 0, s, a);
     else if (a.m % 4 == 0)
