@@ -28,6 +28,7 @@
 
 #include "enkfg_internal.h"
 #include "gram_layout.h"
+#include "gram_stage.h"
 
 namespace enkfg {
 
@@ -102,34 +103,17 @@ __global__ void permute_rows_kernel(const T* __restrict__ src, const int* __rest
 // list entries of chunk c+3 are in flight while chunk c feeds the MFMAs.
 // The taper is applied in FP64 as an operand is formed, (double) S * lc, as
 // the reference forms Sloc.
-// obs per chunk: four k-steps.  With register-staged chunks, 16 rather than 8
-// gave a calc pass on the tile of 888 instead of 904 ms: at 8 the Gram took
-// so many waves per CU that the factor kernels of the previous batch, on the
-// other stream, overlapped it worse (32: 984 ms, 222 VGPRs).
-constexpr int IKC = 16;
-// chunk buffers: the rows of chunks c+1 .. c+IKB-1 are in flight while chunk
-// c feeds the MFMAs; list entries are copied one chunk further ahead, into
-// IKM slots
+// obs per chunk (kChunk, gram_stage.h): four k-steps.  With
+// register-staged chunks, 16 rather than 8 gave a calc pass on the tile of
+// 888 instead of 904 ms: at 8 the Gram took so many waves per CU that the
+// factor kernels of the previous batch, on the other stream, overlapped it
+// worse (32: 984 ms, 222 VGPRs).
+constexpr int IKC = kChunk;
+// chunk buffers (four measured 712.7 against 679.9 ms: six waves per CU)
 constexpr int IKB = 3;
-constexpr int IKM = IKB + 1;
-static_assert(IKB >= 2, "at least two chunk buffers");
 static_assert(kListPad >= (IKB + 1) * IKC, "list padding too small for the item Gram pipeline");
-
-// The list entries and S rows of a chunk are copied from global memory
-// straight into LDS (global_load_lds: lane l of an instruction writes its 4 or
-// 16 bytes at M0 + 4 l or M0 + 16 l), with no VGPRs held for data in flight.
-// Block-major rows: the 16 floats of block X (columns 16X ..) of row r at
-// rows[X][r][0..15]; a k-step reads rows 4s + kq, columns 16X + col, whose 64
-// floats then fall in 64 distinct banks with no padding.
 template <int MT>
-struct ItemLds {
-    static constexpr int MP = 16 * MT;
-    alignas(16) float rows[IKB][MT][IKC][16];   // chunk c in rows[c % IKB]
-    alignas(16) float sv[IKB][64];   // s of the chunk's rows as FP64 in dwords 0..31 (32..63: a copy)
-    // list entries of chunk c in meta[c % IKM], as stored (past the list's
-    // end too): ids in dwords 0..15, tapers (FP64) in 16..47
-    alignas(16) unsigned meta[IKM][64];
-};
+using ItemLds = ChunkLds<MT, IKB>;
 
 // number of tiles of group G of NG
 template <int MT, int NG, int G>
@@ -148,76 +132,6 @@ __device__ __attribute__((always_inline)) inline void item_tiles(const double (&
         acc[T] = __builtin_amdgcn_mfma_f64_16x16x4f64(frag[I], frag[J], acc[T], 0, 0, 0);
         item_tiles<MT, NG, G, T + 1>(frag, acc);
     }
-}
-
-// the copy of the list entries k0 .. k0 + 15 into meta[slot] (lanes 48..63
-// repeat the ids into dwords 48..63).  Past the list's end the entries are
-// read all the same (the list arrays are padded) and masked where used.
-template <int MT>
-__device__ __attribute__((always_inline)) inline void item_meta_copy(const CalcArgs& a, ItemLds<MT>& sh, long long off, int k0, int slot)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    const int lane = threadIdx.x & 63;
-    const unsigned* src = (lane >= 16 && lane < 48)
-                              ? reinterpret_cast<const unsigned*>(a.lc + off + k0 + ((lane - 16) >> 1)) + (lane & 1)
-                              : reinterpret_cast<const unsigned*>(a.ids + off + k0 + (lane & 15));
-    __builtin_amdgcn_global_load_lds(src, &sh.meta[slot][0], 4, 0, 0);
-#endif
-}
-
-// the MT + 1 copies of the rows of chunk k0 / IKC (list entries in
-// meta[slot]) into rows[buf] and sv[buf]: instruction X brings block X of the
-// 16 rows (lane l: row l / 4, columns 16X + 4 (l % 4) ..), the last one s of
-// the rows.  Rows past the list's end and the columns ms .. MP of the last
-// block come from the zero row (row nobs of S, s = 0).
-template <int MT>
-__device__ __attribute__((always_inline)) inline void item_copy(const CalcArgs& a, ItemLds<MT>& sh, int slot, int buf,
-                                                                int k0, int cnt)
-{
-    // the builtin's LDS operand has no form in the host pass, which never
-    // emits this function
-#if defined(__HIP_DEVICE_COMPILE__)
-    const int lane = threadIdx.x & 63;
-    const int r = lane >> 2;
-    const int rs = (lane & 31) >> 1;
-    // loaded unconditionally, then selected (the compiler would otherwise
-    // move the LDS reads under a branch)
-    int idr = (int)sh.meta[slot][r];
-    int sidr = (int)sh.meta[slot][rs];
-    asm volatile("" : "+v"(idr), "+v"(sidr));
-    const int id = k0 + r < cnt ? idr : a.nobs;
-    const int sid = k0 + rs < cnt ? sidr : a.nobs;
-    const float* zero = a.S + (size_t)a.nobs * a.ms;
-    const float* row = a.S + (size_t)id * a.ms + 4 * (lane & 3);
-#pragma unroll
-    for (int X = 0; X < MT - 1; ++X)
-        __builtin_amdgcn_global_load_lds(row + 16 * X, &sh.rows[buf][X][0][0], 16, 0, 0);
-    const bool inrow = 16 * (MT - 1) + 4 * (lane & 3) < a.ms;
-    __builtin_amdgcn_global_load_lds(inrow ? row + 16 * (MT - 1) : zero, &sh.rows[buf][MT - 1][0][0], 16, 0, 0);
-    const unsigned* s = reinterpret_cast<const unsigned*>(a.sv + sid) + (lane & 1);
-    __builtin_amdgcn_global_load_lds(s, &sh.sv[buf][0], 4, 0, 0);
-#endif
-}
-
-// Waits for the copies issued before the last MT + 1 vector memory
-// instructions.  The compiler does not count the LDS writes of
-// global_load_lds against the LDS reads that follow, so the chunk pipeline
-// waits by hand; the memory clobber keeps the compiler's loads and stores on
-// their side of the wait.
-template <int MT>
-__device__ __attribute__((always_inline)) inline void item_wait()
-{
-    static_assert(MT + 1 < 16, "vmcnt field");
-    if constexpr (IKB >= 3)
-        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(MT + 1) : "memory");
-    else
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-}
-
-__device__ __attribute__((always_inline)) inline void item_fence()
-{
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
 }
 
 template <int MT, int NG, int G>
@@ -239,39 +153,11 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
         sts[X] = 0.0;
 
     const int nchunk = (cnt + IKC - 1) / IKC;
-    // prologue, in this order of issue, which the wait at the top of the
-    // loop counts on: list entries of chunks 0 .. IKB-2, copies of chunks
-    // 0 .. IKB-3, list entries of chunk IKB-1, copies of chunk IKB-2
-#pragma unroll
-    for (int p = 0; p < IKB - 1; ++p)
-        item_meta_copy<MT>(a, sh, off, p * IKC, p);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    item_fence();
-#pragma unroll
-    for (int p = 0; p < IKB - 2; ++p)
-        item_copy<MT>(a, sh, p, p, p * IKC, cnt);
-    item_fence();
-    item_meta_copy<MT>(a, sh, off, (IKB - 1) * IKC, IKB - 1);
-    item_fence();
-    item_copy<MT>(a, sh, IKB - 2, IKB - 2, (IKB - 2) * IKC, cnt);
-    item_fence();
+    chunk_prologue<MT, IKB>(a, sh, off, cnt);
     int buf = 0;    // c % IKB
-    int slot = 0;   // c % IKM
+    int slot = 0;   // c % (IKB + 1)
     for (int c = 0; c < nchunk; ++c) {
-        // in flight, oldest first: copies of chunks c .. c+IKB-3, list
-        // entries of chunk c+IKB-1, copies of chunk c+IKB-2.  Wait for all
-        // but the last group.
-        item_wait<MT>();
-        item_fence();
-        // list entries of chunk c+IKB and copies of chunk c+IKB-1, into the
-        // slot and buffer chunk c-1 was read from.  Past the end these read
-        // padding and the zero row, issued all the same: the count of the
-        // wait above takes every iteration to issue them.
-        item_meta_copy<MT>(a, sh, off, (c + IKB) * IKC, slot == 0 ? IKM - 1 : slot - 1);
-        item_fence();
-        item_copy<MT>(a, sh, slot >= 2 ? slot - 2 : slot + IKM - 2, buf == 0 ? IKB - 1 : buf - 1,
-                      (c + IKB - 1) * IKC, cnt);
-        item_fence();
+        chunk_advance<MT, IKB>(a, sh, off, cnt, c, buf, slot);
         const double* svc = reinterpret_cast<const double*>(&sh.sv[buf][0]);
         const double* lcc = reinterpret_cast<const double*>(&sh.meta[slot][16]);
 #pragma unroll
@@ -291,9 +177,9 @@ __device__ __attribute__((always_inline)) inline void item_wave(const CalcArgs& 
                     sts[X] += frag[X] * sl;
             }
         }
-        item_fence();
+        chunk_fence();
         buf = buf == IKB - 1 ? 0 : buf + 1;
-        slot = slot == IKM - 1 ? 0 : slot + 1;
+        slot = slot == IKB ? 0 : slot + 1;
     }
     // the copies of the chunks past the end land before the wave ends
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");

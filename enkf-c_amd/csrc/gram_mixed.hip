@@ -22,6 +22,7 @@
 
 #include "enkfg_internal.h"
 #include "gram_layout.h"
+#include "gram_stage.h"
 
 namespace enkfg {
 
@@ -282,40 +283,27 @@ __global__ void __launch_bounds__(256) gram_mixed_kernel(CalcArgs a)
 
 // ------------------------------------------------------------------ one wavefront per work item
 //
-// For m <= 96: independent wavefronts, no workgroup barrier.  A wavefront
+// For m <= 160: independent wavefronts, no workgroup barrier.  A wavefront
 // computes one of NG groups of the tri_count(MT) tiles (f = g, g + NG, ...;
 // NG = 2: 11 and 10 tiles for m = 96, NG = 3: 7 each, FP32 accumulators;
 // NG is chosen per obs-count bucket by the host's autotuner) of one (node,
-// obstype segment); every wave of a segment stages the whole chunk (the
-// others find the rows in L2).  Chunks of MKC = 16 local obs (one K = 16 MFMA step,
-// v_mfma_f32_16x16x16_bf16, which halves the gathered registers in flight
-// against K = 32) are gathered into registers one chunk ahead -- item
-// u = lane + 64 j is member u % MP of obs rows 4 (u / MP) .. + 3 -- and
-// staged member-major as BF16 hi
-// and lo planes in a private LDS area after the MFMAs of the previous chunk
-// (the wave's own program order keeps the two apart).  The taper is applied
-// in FP32 (x = S * (float) lc; the split that follows already carries about
-// 2^-17 relative error); S^T s is summed in FP32 over a chunk and in FP64
-// over chunks.
-constexpr int MKC = 16;
-constexpr int MKR = MKC + 8;   // bf16 per member row (48 bytes: 8-byte aligned, staggered banks)
+// obstype segment).  The segment's list entries and S rows are copied into
+// the wave's private LDS area in chunks of kChunk = 16 local obs, MKB chunks
+// deep, with no VGPRs held for data in flight (gram_stage.h, the same
+// pipeline as the FP64 item kernel; the other waves of a segment find the
+// rows in L2).  A chunk is one K = 16 step of v_mfma_f32_16x16x16_bf16: lane
+// (col, kg) forms its operand of block X -- obs 4 kg .. 4 kg + 3 of member
+// 16 X + col -- from four FP32 values of the block-major rows, x = S *
+// (float) lc, split into BF16 hi = bf16(x) and lo = bf16(x - hi) (the split
+// carries about 2^-17 relative error); each element is converted once per
+// wave.  S^T s is summed in FP32 over a chunk and in FP64 over chunks.
+// Against the earlier kernel, which gathered one chunk ahead into registers
+// and staged BF16 planes in LDS, the results are the same bit for bit.
+constexpr int MKB = 3;
+static_assert(kListPad >= (MKB + 1) * kChunk, "list padding too small for the mixed item Gram pipeline");
+template <int MT>
+using MixItemLds = ChunkLds<MT, MKB>;
 typedef short sh4 __attribute__((ext_vector_type(4)));
-
-template <int MT>
-struct MixItemLds {
-    static constexpr int MP = 16 * MT;
-    alignas(16) __bf16 hi[MP][MKR];
-    alignas(16) __bf16 lo[MP][MKR];
-    float sl[MKC];             // s * lc of the staged chunk
-    double lc[4][MKC];         // tapers, chunk c in [c % 4]
-    int id[4][MKC];            // rows of S (-1 past the end), chunk c in [c % 4]
-};
-
-template <int MT>
-struct MixItemRegs {
-    float v[MT][4];            // items j: 4 obs rows of one member
-    double s;                  // s of row lane % MKC
-};
 
 template <int MT, int NG, int G>
 constexpr int mix_item_ntile()
@@ -338,76 +326,6 @@ __device__ __attribute__((always_inline)) inline void mix_item_tiles(const sh4 (
     }
 }
 
-__device__ __attribute__((always_inline)) inline void mix_item_meta_load(const CalcArgs& a, long long off, int cnt,
-                                                                         int k0, int& id, double& c)
-{
-    const int k = k0 + (int)(threadIdx.x & (MKC - 1));
-    const int i = a.ids[off + k];
-    const double l = a.lc[off + k];
-    const bool in = k < cnt;
-    id = in ? i : -1;
-    c = in ? l : 0.0;
-}
-
-template <int MT>
-__device__ __attribute__((always_inline)) inline void mix_item_meta_store(MixItemLds<MT>& sh, int slot, int id,
-                                                                          double c)
-{
-    // lanes l and l + MKC hold the same entry: every lane stores
-    sh.id[slot][threadIdx.x & (MKC - 1)] = id;
-    sh.lc[slot][threadIdx.x & (MKC - 1)] = c;
-}
-
-template <int MT>
-__device__ __attribute__((always_inline)) inline void mix_item_gather(const CalcArgs& a, const MixItemLds<MT>& sh,
-                                                                      int slot, MixItemRegs<MT>& L)
-{
-    constexpr int MP = 16 * MT;
-    // byte offsets from S in 32 bits (S is below 4 GiB for this kernel,
-    // launch_gram_mixed): one base register for all 8 MT loads
-    const char* base = reinterpret_cast<const char*>(a.S);
-    static_assert(MKC == 16, "4 row groups of 4 obs per chunk");
-#pragma unroll
-    for (int j = 0; j < MT; ++j) {
-        const int u = (int)threadIdx.x + 64 * j;
-        const int rg = u / MP;               // < MKC / 4 since 64 MT = 4 MP
-        const int e = min(u - rg * MP, a.m - 1);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int id = sh.id[slot][4 * rg + k];
-            const unsigned row = (unsigned)(id >= 0 ? id : a.nobs);
-            const unsigned boff = (row * (unsigned)a.ms + (unsigned)e) * 4u;
-            L.v[j][k] = *reinterpret_cast<const float*>(base + boff);
-        }
-    }
-    const int id = sh.id[slot][threadIdx.x & (MKC - 1)];
-    L.s = a.sv[id >= 0 ? id : a.nobs];
-}
-
-template <int MT>
-__device__ __attribute__((always_inline)) inline void mix_item_stage(const CalcArgs& a, MixItemLds<MT>& sh, int mslot,
-                                                                     const MixItemRegs<MT>& L)
-{
-    constexpr int MP = 16 * MT;
-#pragma unroll
-    for (int j = 0; j < MT; ++j) {
-        const int u = (int)threadIdx.x + 64 * j;
-        const int rg = u / MP;
-        const int e = u - rg * MP;
-        const bool real = e < a.m;
-        bf16x4 h, l;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float x = real ? L.v[j][k] * (float)sh.lc[mslot][4 * rg + k] : 0.0f;
-            h[k] = (__bf16)x;
-            l[k] = (__bf16)(x - (float)h[k]);
-        }
-        *reinterpret_cast<bf16x4*>(&sh.hi[e][4 * rg]) = h;
-        *reinterpret_cast<bf16x4*>(&sh.lo[e][4 * rg]) = l;
-    }
-    sh.sl[threadIdx.x & (MKC - 1)] = (float)(L.s * sh.lc[mslot][threadIdx.x & (MKC - 1)]);
-}
-
 template <int MT, int NG, int G>
 __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcArgs& a, int b, long long off, int cnt,
                                                                     MixItemLds<MT>& sh)
@@ -415,7 +333,7 @@ __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcAr
     constexpr int MP = 16 * MT;
     constexpr int NTG = mix_item_ntile<MT, NG, G>();
     const int lane = threadIdx.x & 63;
-    const int kg = lane >> 4;      // which 8 obs of the K = 32 step
+    const int kg = lane >> 4;      // which 4 obs of the K = 16 step
     const int col = lane & 15;
 
     flt4 acc[NTG > 0 ? NTG : 1];
@@ -427,37 +345,41 @@ __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcAr
     for (int X = 0; X < MT; ++X)
         sts[X] = 0.0;
 
-    const int nchunk = (cnt + MKC - 1) / MKC;
-    int mid;
-    double mc;
-    mix_item_meta_load(a, off, cnt, 0, mid, mc);
-    mix_item_meta_store<MT>(sh, 0, mid, mc);
-    mix_item_meta_load(a, off, cnt, MKC, mid, mc);
-    mix_item_meta_store<MT>(sh, 1, mid, mc);
-    mix_item_meta_load(a, off, cnt, 2 * MKC, mid, mc);
-    mix_item_meta_store<MT>(sh, 2, mid, mc);
-    MixItemRegs<MT> L;
-    mix_item_gather<MT>(a, sh, 0, L);
-    mix_item_stage<MT>(a, sh, 0, L);
-    mix_item_gather<MT>(a, sh, 1, L);
+    const int nchunk = (cnt + kChunk - 1) / kChunk;
+    chunk_prologue<MT, MKB>(a, sh, off, cnt);
+    int buf = 0;    // c % MKB
+    int slot = 0;   // c % (MKB + 1)
     const int k0 = 4 * kg;
     for (int c = 0; c < nchunk; ++c) {
-        mix_item_meta_load(a, off, cnt, (c + 3) * MKC, mid, mc);
-        __builtin_amdgcn_sched_barrier(0);
+        chunk_advance<MT, MKB>(a, sh, off, cnt, c, buf, slot);
+        const double* svc = reinterpret_cast<const double*>(&sh.sv[buf][0]);
+        const double* lcc = reinterpret_cast<const double*>(&sh.meta[slot][16]);
+        // tapers of the lane's 4 obs (0 past the list's end) and s * lc
+        float lcf[4], slv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double lr = lcc[k0 + q];
+            const double l = c * kChunk + k0 + q < cnt ? lr : 0.0;
+            lcf[q] = (float)l;
+            slv[q] = (float)(svc[k0 + q] * l);
+        }
         // operand of v_mfma_f32_16x16x16_bf16: lane l supplies 4 consecutive
         // obs (k = 4 (l / 16) ..) of member row l % 16 of a tile
         sh4 h[MT], l[MT];
 #pragma unroll
         for (int X = 0; X < MT; ++X) {
-            h[X] = *reinterpret_cast<const sh4*>(&sh.hi[16 * X + col][k0]);
-            l[X] = *reinterpret_cast<const sh4*>(&sh.lo[16 * X + col][k0]);
+            bf16x4 hb, lb;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float x = sh.rows[buf][X][k0 + q][col] * lcf[q];
+                hb[q] = (__bf16)x;
+                lb[q] = (__bf16)(x - (float)hb[q]);
+            }
+            h[X] = __builtin_bit_cast(sh4, hb);
+            l[X] = __builtin_bit_cast(sh4, lb);
         }
         mix_item_tiles<MT, NG, G, 0>(h, l, acc);
         if (G == 0) {
-            float slv[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                slv[q] = sh.sl[k0 + q];
 #pragma unroll
             for (int X = 0; X < MT; ++X) {
                 const bf16x4 hb = __builtin_bit_cast(bf16x4, h[X]);
@@ -469,13 +391,12 @@ __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcAr
                 sts[X] += (double)t;
             }
         }
-        __builtin_amdgcn_sched_barrier(0);
-        // stage chunk c+1 over chunk c (read above, in this wave's order),
-        // gather chunk c+2; past the end these touch the zero row only
-        mix_item_stage<MT>(a, sh, (c + 1) & 3, L);
-        mix_item_gather<MT>(a, sh, (c + 2) & 3, L);
-        mix_item_meta_store<MT>(sh, (c + 3) & 3, mid, mc);
+        chunk_fence();
+        buf = buf == MKB - 1 ? 0 : buf + 1;
+        slot = slot == MKB ? 0 : slot + 1;
     }
+    // the copies of the chunks past the end land before the wave ends
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     // v_mfma_f32_16x16x16_bf16 result: register i of lane l holds
     // D[4 (l / 16) + i][l % 16]
     double* gout = a.gram + (size_t)b * MP * MP;
