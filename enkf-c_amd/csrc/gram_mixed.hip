@@ -304,6 +304,8 @@ static_assert(kListPad >= (MKB + 1) * kChunk, "list padding too small for the mi
 template <int MT>
 using MixItemLds = ChunkLds<MT, MKB>;
 typedef short sh4 __attribute__((ext_vector_type(4)));
+typedef float flt2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
 
 template <int MT, int NG, int G>
 constexpr int mix_item_ntile()
@@ -365,16 +367,22 @@ __device__ __attribute__((always_inline)) inline void mix_item_wave(const CalcAr
         }
         // operand of v_mfma_f32_16x16x16_bf16: lane l supplies 4 consecutive
         // obs (k = 4 (l / 16) ..) of member row l % 16 of a tile
+        // (pairs converted together: v_cvt_pk_bf16_f32)
         sh4 h[MT], l[MT];
 #pragma unroll
         for (int X = 0; X < MT; ++X) {
-            bf16x4 hb, lb;
+            flt2v x[2], r[2];
+            bf16x2v hp[2], lp[2];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float x = sh.rows[buf][X][k0 + q][col] * lcf[q];
-                hb[q] = (__bf16)x;
-                lb[q] = (__bf16)(x - (float)hb[q]);
+            for (int p = 0; p < 2; ++p) {
+                x[p][0] = sh.rows[buf][X][k0 + 2 * p][col] * lcf[2 * p];
+                x[p][1] = sh.rows[buf][X][k0 + 2 * p + 1][col] * lcf[2 * p + 1];
+                hp[p] = __builtin_convertvector(x[p], bf16x2v);
+                r[p] = x[p] - __builtin_convertvector(hp[p], flt2v);
+                lp[p] = __builtin_convertvector(r[p], bf16x2v);
             }
+            const bf16x4 hb = {hp[0][0], hp[0][1], hp[1][0], hp[1][1]};
+            const bf16x4 lb = {lp[0][0], lp[0][1], lp[1][0], lp[1][1]};
             h[X] = __builtin_bit_cast(sh4, hb);
             l[X] = __builtin_bit_cast(sh4, lb);
         }
