@@ -457,7 +457,7 @@ def main():
             ach = F_gram / (phases[1] * 1e-3)
             # the item kernel in every tile shape the autotuner chose (per
             # obs-count bucket): the FP64 Gram phase is their launches together
-            kname = ("gram_item_kernel<%d, *>" % mt if m <= 96 else "gram_kernel<%d>" % mt)
+            kname = "gram_item_kernel<%d, *>" % mt
             roof = dict(bound="mfma", achieved=ach / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s",
                         frac=ach / PEAK_FP64, traffic=profiled_traffic(tag, kname), kernel=kname + " (FP64 MFMA)",
                         launches_per_step=nl, avg_launch_ms=phases[1] / max(nl, 1),

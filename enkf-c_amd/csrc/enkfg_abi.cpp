@@ -1180,8 +1180,7 @@ int enkfg_calc_transforms(enkfg_ctx* c, const enkfg_nodes* nodes)
     // buckets share one choice, the nodes run in plain grid order (better L2
     // reuse between neighbours) with that choice.
     const bool mixed = c->cfg.precision == ENKFG_PREC_MIXED;
-    const bool items = (m + 15) / 16 <= 10 &&
-                       (!mixed || (unsigned long long)(c->nobs + 1) * (unsigned long long)a.ms * 4ull < (1ull << 32));
+    const bool items = (m + 15) / 16 <= 10;
     const int kTuneMin = std::min(nbatch, 256);
     int* tg = c->tile_groups[mixed ? 1 : 0];
     const int tg_lo = item_groups_lo((m + 15) / 16);

@@ -101,65 +101,4 @@ __host__ __device__ constexpr size_t gram_index(int i, int j)
     return ((size_t)((i / B) * 16 + (j / B)) * B + (i % B)) * B + (j % B);
 }
 
-// ---- chunk staging shared by the Gram kernels
-//
-// Local obs are streamed in chunks of kGramKC.  All loads are unconditional
-// -- list entries past a segment's end are read from the padding of the
-// list arrays (kListPad entries) and masked to id -1 / taper 0, and the S
-// row of id -1 is the zero row `nobs` -- so no load writes a register on
-// only some paths, and the loads of chunk c+1 stay in flight while chunk c
-// feeds the MFMAs (a conditionally loaded register forces the compiler to
-// wait for every outstanding load before it can be redefined).
-constexpr int kGramKC = 64;   // the largest chunk of the Gram kernels
-// a segment's last chunk c reads list entries up to (c + 3) KC - 1, at most
-// 3 KC - 1 past the segment's end
-static_assert(kListPad >= 3 * kGramKC, "list padding too small for the Gram chunk pipeline");
-
-// list entries k0 + (tid % KC) of a segment (every thread loads; threads
-// below KC store)
-template <int KC>
-__device__ inline void gram_meta_load(const CalcArgs& a, long long off, int cnt, int k0, int& id, double& c)
-{
-    const int k = k0 + (int)(threadIdx.x & (KC - 1));
-    const int i = a.ids[off + k];
-    const double l = a.lc[off + k];
-    const bool in = k < cnt;
-    id = in ? i : -1;
-    c = in ? l : 0.0;
-}
-
-// One chunk in registers: quad q = tid + 256 j of the KC x (ms / 4) chunk is
-// row q / (ms / 4), members 4 (q % (ms / 4)) ..; NQ quads per thread cover
-// the chunk for any m <= 16 MT.
-template <int MT, int KC>
-struct GramRegs {
-    static constexpr int NQ = (KC * 4 * MT + 255) / 256;
-    float4 v[NQ];
-    double s;                 // s of row tid % KC
-};
-
-template <int KC>
-__device__ inline int gram_row_id(const int* ids_lds, int row)
-{
-    const int id = ids_lds[row < KC ? row : KC - 1];
-    return row < KC ? id : -1;
-}
-
-template <int MT, int KC>
-__device__ inline void gram_gather(const CalcArgs& a, const int* ids_lds, GramRegs<MT, KC>& L)
-{
-    const int qr = a.ms >> 2;
-#pragma unroll
-    for (int j = 0; j < GramRegs<MT, KC>::NQ; ++j) {
-        const int q = (int)threadIdx.x + 256 * j;
-        const int row = q / qr;
-        const int qd = q - row * qr;
-        const int id = gram_row_id<KC>(ids_lds, row);
-        const long long r = id >= 0 ? id : a.nobs;
-        L.v[j] = *reinterpret_cast<const float4*>(a.S + r * a.ms + 4 * qd);
-    }
-    const int id = ids_lds[threadIdx.x & (KC - 1)];
-    L.s = a.sv[id >= 0 ? id : a.nobs];
-}
-
 }  // namespace enkfg

@@ -2,22 +2,16 @@
 // precision: BF16 MFMA with FP32 accumulation on a hi/lo split of the tapered
 // anomalies.
 //
-// Each tapered value x = (float)((double)S * lc) is split as x = hi + lo + r
-// with hi = bf16(x), lo = bf16(x - hi) and |r| <= 2^-18 |x|; the product
-// x_a x_b is taken as hi_a hi_b + hi_a lo_b + lo_a hi_b, which leaves out
-// lo_a lo_b (<= 2^-18 relative), so every term of S^T S carries a relative
-// error of about 2^-17 before the FP32 accumulation.  S^T s accumulates in
-// FP64 from hi + lo.  The result is written in the FP64 Gram layout and the
-// factorisation that follows stays in FP64 (calc_kernels.hip).
+// Each tapered value x = S * (float) lc is split as x = hi + lo + r with
+// hi = bf16(x), lo = bf16(x - hi) and |r| <= 2^-18 |x|; the product x_a x_b
+// is taken as hi_a hi_b + hi_a lo_b + lo_a hi_b, which leaves out lo_a lo_b
+// (<= 2^-18 relative), so every term of S^T S carries a relative error of
+// about 2^-17 before the FP32 accumulation.  S^T s accumulates in FP64 from
+// hi + lo.  The result is written in the FP64 Gram layout and the
+// factorisation that follows stays in FP64 (factor_wave.hip).
 //
-// Work item, chunk pipeline and XCD-aware mapping are those of the FP64 Gram
-// kernel: one workgroup of 4 waves per (node, obstype segment), local obs
-// streamed in chunks of 64 (the gathers of chunk c+1 are in flight while chunk
-// c feeds the MFMAs).  A chunk is stored in LDS member-major, 64 obs of one
-// member contiguous as bf16, so that an operand of
-// v_mfma_f32_16x16x32_bf16 -- 8 consecutive obs of one member -- is one
-// 16-byte LDS read; the gathers are arranged so that each lane holds 4
-// consecutive obs of one member and stores them as one 8-byte write.
+// Work items, chunk pipeline and XCD placement are those of the FP64 item
+// Gram kernel (calc_kernels.hip, gram_stage.h, gram_layout.h).
 #include <algorithm>
 
 #include "enkfg_internal.h"
@@ -28,258 +22,8 @@ namespace enkfg {
 
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
 typedef float flt4 __attribute__((__vector_size__(4 * sizeof(float))));
-
-constexpr int KC = kGramKC;    // obs per chunk (two K = 32 MFMA steps)
-constexpr int KR = KC + 8;     // bf16 per member row in LDS (144 B: 16-byte aligned, staggered banks)
-
-template <int MT>
-struct MixLds {
-    static constexpr int MP = 16 * MT;
-    alignas(16) __bf16 hi[2][MP][KR];
-    alignas(16) __bf16 lo[2][MP][KR];
-    double sl[2][KC];          // s_o * lc_o
-    double lc[3][KC];
-    int id[3][KC];
-};
-
-template <int MT>
-__device__ inline void meta_store(MixLds<MT>& sh, int buf, int id, double c)
-{
-    if (threadIdx.x < KC) {
-        sh.id[buf][threadIdx.x] = id;
-        sh.lc[buf][threadIdx.x] = c;
-    }
-}
-
 typedef __bf16 bf16x4 __attribute__((__vector_size__(4 * sizeof(__bf16))));
-
-// One chunk in registers, member-contiguous: item u = tid + 256 j holds
-// member u % MP of rows 4 (u / MP) .. 4 (u / MP) + 3, so a wave's loads of
-// one row are coalesced and its LDS stores are 4 consecutive obs (8 bytes)
-// of one member.  Loads are unconditional (zero row for ids -1, the last
-// member for padding members, which are not stored).
-template <int MT>
-struct MixRegs {
-    float v[MT][4];
-    double s;                  // s of row tid % KC
-};
-
-template <int MT>
-__device__ inline void mix_gather(const CalcArgs& a, const int* ids_lds, MixRegs<MT>& L)
-{
-    constexpr int MP = 16 * MT;
-#pragma unroll
-    for (int j = 0; j < MT; ++j) {
-        const int u = (int)threadIdx.x + 256 * j;
-        const int rg = u / MP;               // < KC / 4 since 256 MT = 16 MP
-        const int e = min(u - rg * MP, a.m - 1);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int id = ids_lds[4 * rg + k];
-            const long long r = id >= 0 ? id : a.nobs;
-            L.v[j][k] = a.S[r * a.ms + e];
-        }
-    }
-    const int id = ids_lds[threadIdx.x & (KC - 1)];
-    L.s = a.sv[id >= 0 ? id : a.nobs];
-}
-
-// taper, split, store 4 obs of one member into the member-major rows of
-// buffer rbuf
-template <int MT>
-__device__ inline void vals_store(const CalcArgs& a, MixLds<MT>& sh, int rbuf, int mbuf, const MixRegs<MT>& L)
-{
-    constexpr int MP = 16 * MT;
-#pragma unroll
-    for (int j = 0; j < MT; ++j) {
-        const int u = (int)threadIdx.x + 256 * j;
-        const int rg = u / MP;
-        const int e = u - rg * MP;
-        if (e < a.m) {
-            bf16x4 h, l;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float x = (float)((double)L.v[j][k] * sh.lc[mbuf][4 * rg + k]);
-                h[k] = (__bf16)x;
-                l[k] = (__bf16)(x - (float)h[k]);
-            }
-            *reinterpret_cast<bf16x4*>(&sh.hi[rbuf][e][4 * rg]) = h;
-            *reinterpret_cast<bf16x4*>(&sh.lo[rbuf][e][4 * rg]) = l;
-        }
-    }
-    if (threadIdx.x < KC)
-        sh.sl[rbuf][threadIdx.x] = L.s * sh.lc[mbuf][threadIdx.x];
-}
-
-// the three split products for the tiles (I, J) of wave W
-template <int MT, int W, int T>
-__device__ inline void mix_tiles(const bf16x8 (&h)[MT], const bf16x8 (&l)[MT], flt4* acc)
-{
-    if constexpr (W + 4 * T < tri_count(MT)) {
-        constexpr int f = W + 4 * T;
-        constexpr int I = tri_I(MT, f), J = tri_J(MT, f);
-        acc[T] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h[I], h[J], acc[T], 0, 0, 0);
-        acc[T] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h[I], l[J], acc[T], 0, 0, 0);
-        acc[T] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l[I], h[J], acc[T], 0, 0, 0);
-        mix_tiles<MT, W, T + 1>(h, l, acc);
-    }
-}
-
-template <int MT>
-constexpr int ntile_of(int W)
-{
-    return (tri_count(MT) - W + 3) / 4 > 0 ? (tri_count(MT) - W + 3) / 4 : 1;
-}
-
-template <int MT, int W>
-__device__ inline void mix_wave_chunk(const MixLds<MT>& sh, int buf, flt4 (&acc)[ntile_of<MT>(W)],
-                                      double (&sts)[(MT + 3) / 4])
-{
-    const int lane = threadIdx.x & 63;
-    const int kg = lane >> 4;      // which 4 obs of the K = 16 step
-    const int col = lane & 15;
-#pragma unroll
-    for (int ks = 0; ks < KC / 32; ++ks) {
-        const int k0 = 32 * ks + 8 * kg;
-        bf16x8 h[MT], l[MT];
-#pragma unroll
-        for (int X = 0; X < MT; ++X) {
-            h[X] = *reinterpret_cast<const bf16x8*>(&sh.hi[buf][16 * X + col][k0]);
-            l[X] = *reinterpret_cast<const bf16x8*>(&sh.lo[buf][16 * X + col][k0]);
-        }
-        mix_tiles<MT, W, 0>(h, l, acc);
-        // S^T s for the member tiles X = W, W + 4, ... (FP64)
-#pragma unroll
-        for (int u = 0; u < (MT + 3) / 4; ++u) {
-            const int X = W + 4 * u;
-            if (X < MT) {
-                double t = 0.0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    t += ((double)(float)h[X][q] + (double)(float)l[X][q]) * sh.sl[buf][k0 + q];
-                sts[u] += t;
-            }
-        }
-    }
-}
-
-template <int MT, int W>
-__device__ inline void mix_wave(const CalcArgs& a, long long off, int cnt, double* __restrict__ gout,
-                                double* __restrict__ stsout, MixLds<MT>& sh)
-{
-    constexpr int NTW = ntile_of<MT>(W);
-    constexpr int NU = (MT + 3) / 4;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int kg = lane >> 4;
-    const int col = lane & 15;
-
-    flt4 acc[NTW];
-#pragma unroll
-    for (int t = 0; t < NTW; ++t)
-        acc[t] = flt4{0.0f, 0.0f, 0.0f, 0.0f};
-    double sts[NU];
-#pragma unroll
-    for (int u = 0; u < NU; ++u)
-        sts[u] = 0.0;
-
-    // zero both buffers once: members >= m stay zero
-    {
-        unsigned int* z = reinterpret_cast<unsigned int*>(&sh.hi[0][0][0]);
-        const int nw = (int)(2 * sizeof(sh.hi) / sizeof(unsigned int));   // hi and lo are adjacent
-        for (int q = tid; q < nw; q += 256)
-            z[q] = 0u;
-    }
-    MixRegs<MT> L;
-    const int nchunk = (cnt + KC - 1) / KC;
-    int mid;
-    double mc;
-    gram_meta_load<KC>(a, off, cnt, 0, mid, mc);
-    meta_store<MT>(sh, 0, mid, mc);
-    gram_meta_load<KC>(a, off, cnt, KC, mid, mc);
-    meta_store<MT>(sh, 1, mid, mc);
-    meta_store<MT>(sh, 2, -1, 0.0);   // every id buffer holds valid rows or -1
-    __syncthreads();
-    mix_gather<MT>(a, sh.id[0], L);
-    vals_store<MT>(a, sh, 0, 0, L);
-    __syncthreads();
-    for (int c = 0; c < nchunk; ++c) {
-        const bool more = c + 1 < nchunk;
-        const bool more2 = c + 2 < nchunk;
-        // unconditional loads (list padding, zero row), see gram_layout.h
-        mix_gather<MT>(a, sh.id[(c + 1) % 3], L);
-        gram_meta_load<KC>(a, off, cnt, (c + 2) * KC, mid, mc);
-        __builtin_amdgcn_sched_barrier(0);   // loads stay ahead of the MFMAs
-        mix_wave_chunk<MT, W>(sh, c & 1, acc, sts);
-        if (more)
-            vals_store<MT>(a, sh, (c + 1) & 1, (c + 1) % 3, L);
-        if (more2)
-            meta_store<MT>(sh, (c + 2) % 3, mid, mc);
-        __syncthreads();
-    }
-    // v_mfma_f32_16x16x32_bf16 result: register i of lane l holds
-    // D[4 (l / 16) + i][l % 16]
-#pragma unroll
-    for (int t = 0; t < NTW; ++t) {
-        const int f = W + 4 * t;
-        if (f >= tri_count(MT))
-            break;
-        const int I = tri_I(MT, f), J = tri_J(MT, f);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = 16 * I + 4 * kg + i;
-            const int cc = 16 * J + col;
-            const double v = (double)acc[t][i];
-            gout[gram_index<MT>(r, cc)] = v;
-            if (I != J)
-                gout[gram_index<MT>(cc, r)] = v;
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-        const int X = W + 4 * u;
-        double v = sts[u];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        if (X < MT && kg == 0)
-            stsout[16 * X + col] = v;
-    }
-}
-
-template <int MT>
-__global__ void __launch_bounds__(256) gram_mixed_kernel(CalcArgs a)
-{
-    constexpr int MP = 16 * MT;
-    const int nwork = a.nbatch * a.nseg;
-    const int per = (nwork + 7) / 8;
-    const int b = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (b >= nwork)
-        return;
-    const int node = a.node_list ? a.node_list[a.node0 + b / a.nseg] : a.node0 + b / a.nseg;
-    const int seg = b % a.nseg;
-    long long off;
-    int cnt;
-    if (a.pertype) {
-        off = a.offsets[(size_t)node * a.ntypes + seg];
-        cnt = a.counts[(size_t)node * a.ntypes + seg];
-    } else {
-        off = a.offsets[(size_t)node * a.ntypes];
-        cnt = (int)(a.offsets[(size_t)(node + 1) * a.ntypes] - off);
-    }
-    if (cnt == 0)
-        return;
-    __shared__ MixLds<MT> sh;
-    double* gout = a.gram + (size_t)b * MP * MP;
-    double* stsout = a.sts + (size_t)b * MP;
-    switch (threadIdx.x >> 6) {
-    case 0: mix_wave<MT, 0>(a, off, cnt, gout, stsout, sh); break;
-    case 1: mix_wave<MT, 1>(a, off, cnt, gout, stsout, sh); break;
-    case 2: mix_wave<MT, 2>(a, off, cnt, gout, stsout, sh); break;
-    default: mix_wave<MT, 3>(a, off, cnt, gout, stsout, sh); break;
-    }
-}
 
 // ------------------------------------------------------------------ one wavefront per work item
 //
@@ -502,12 +246,9 @@ void launch_mix_item(hipStream_t s, dim3 grid, const CalcArgs& a, int ng)
 hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
 {
     const int MT = (a.m + 15) / 16;
-    dim3 grid((unsigned)(8 * ((a.nbatch * a.nseg + 7) / 8)));
     if (a.nbatch == 0)
         return hipSuccess;
-    // the item kernel addresses S with 32-bit byte offsets
-    const bool small_s = (unsigned long long)(a.nobs + 1) * (unsigned long long)a.ms * 4ull < (1ull << 32);
-    if (MT <= 10 && small_s) {
+    if (MT >= 1 && MT <= 10) {
         // tile groups per segment: one of the two candidates of this MT (the
         // smaller by default)
         const int lo = item_groups_lo(MT);
@@ -528,20 +269,7 @@ hipError_t launch_gram_mixed(hipStream_t s, const CalcArgs& a)
         }
         return hipGetLastError();
     }
-    switch (MT) {
-    case 1: hipLaunchKernelGGL(gram_mixed_kernel<1>, grid, dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(gram_mixed_kernel<2>, grid, dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(gram_mixed_kernel<3>, grid, dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(gram_mixed_kernel<4>, grid, dim3(256), 0, s, a); break;
-    case 5: hipLaunchKernelGGL(gram_mixed_kernel<5>, grid, dim3(256), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(gram_mixed_kernel<6>, grid, dim3(256), 0, s, a); break;
-    case 7: hipLaunchKernelGGL(gram_mixed_kernel<7>, grid, dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(gram_mixed_kernel<8>, grid, dim3(256), 0, s, a); break;
-    case 9: hipLaunchKernelGGL(gram_mixed_kernel<9>, grid, dim3(256), 0, s, a); break;
-    case 10: hipLaunchKernelGGL(gram_mixed_kernel<10>, grid, dim3(256), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return hipErrorInvalidValue;   // m > 160 is refused at configuration
 }
 
 }  // namespace enkfg
