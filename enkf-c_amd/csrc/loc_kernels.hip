@@ -117,7 +117,10 @@ __device__ inline void range_walk(const KdNode* __restrict__ tree, int ndim, con
 // of the stride grid and one obstype, so its lanes walk nearly the same
 // paths of one tree and their node loads coalesce.  ni is the grid's row
 // length (nnode = ni nj); tiles are row-major within a type.
-constexpr int kWalkRun = 64;   // consecutive walk items (one 8 x 8 block of tiles) per XCD run
+// consecutive walk items per XCD run: four 8 x 8 blocks of tiles (runs of
+// one block: localisation 103-105 ms on the tile, of four 101-102 ms, of
+// sixteen 135 ms: the XCDs with the densest regions then finish last)
+constexpr int kWalkRun = 256;
 
 __device__ inline bool thread_node(int nnode, int ni, int ntypes, int& t, int& node)
 {
@@ -127,8 +130,8 @@ __device__ inline bool thread_node(int nnode, int ni, int ntypes, int& t, int& n
     const long long per_type = (long long)sti * stj * 64 * 64;
     // Tiles are ordered in 8 x 8 blocks of tiles (64 x 64 nodes), row-major
     // within and between blocks.  Blocks go to the 8 XCDs round-robin; each
-    // XCD takes runs of kWalkRun consecutive (type, tile) items -- one block
-    // of tiles -- so the walks in flight on one L2 cover a compact patch of
+    // XCD takes runs of kWalkRun consecutive (type, tile) items -- four
+    // blocks of tiles -- so the walks in flight on one L2 cover a compact patch of
     // the grid, and the runs are dealt round-robin, so dense and sparse
     // regions spread over all XCDs (walk_grid rounds the grid up to a
     // multiple of 8 runs; items past the grid's edge return)
