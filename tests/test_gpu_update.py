@@ -89,6 +89,31 @@ def test_update_member_count_not_multiple_of_four(nmem):
     E.close()
 
 
+@pytest.mark.parametrize("nmem,precision", [(48, enkfg.PREC_FP64), (80, enkfg.PREC_FP64), (64, enkfg.PREC_FP64),
+                                             (80, enkfg.PREC_MIXED), (112, enkfg.PREC_MIXED)])
+def test_update_tile_layouts(nmem, precision):
+    """The update kernel's T_cell layouts and workgroup shapes: FP32 with an
+    odd number of 16-member output tiles (48, 80: rows padded, 8-wave
+    workgroups) and an even one (64: 16-column blocks swapped, 4-wave
+    workgroups); mixed up to 96 members in 4-wave workgroups and beyond (112)
+    in 8-wave ones."""
+    print("case update layout nmem=%d precision=%d" % (nmem, precision), flush=True)
+    P = synth.make_problem(seed=60 + nmem, nmem=nmem, nobs=1000, locrad=60.0, nj=25, ni=31, stride=3, land_frac=0.1)
+    T, w = transforms(P, 1, 1, nmem)
+    bufs, fl = fields_of(P, nmem)
+    ref = [b.copy() for b in bufs]
+    gpu = [b.copy() for b in bufs]
+    infl_o = O.update_fields(1, nmem, nmem, 3, 0, 0, P.nlevels, T, w, fl, ref, 0, P.nj - 1, want_infl=True)
+    E = enkfg.Engine(nmem, mode=enkfg.MODE_ENKF, precision=precision)
+    E.set_transforms(T, w, 3)
+    infl_g = E.update_fields(P.nlevels, 3, 0, 0, fl, gpu, 0, P.nj - 1, want_infl=True)
+    tol, itol = (1e-5, 1e-5) if precision == enkfg.PREC_FP64 else (2e-5, 1e-4)
+    for a, b in zip(gpu, ref):
+        check(a, b, tol)
+    assert np.abs(infl_g - infl_o).max() <= itol
+    E.close()
+
+
 def test_update_tile_rows():
     print("case update j-tile", flush=True)
     P = synth.make_problem(seed=21, nmem=24, nobs=1000, locrad=60.0, nj=33, ni=40, stride=3)
