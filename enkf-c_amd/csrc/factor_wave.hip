@@ -1160,8 +1160,11 @@ factor_etkf_kernel(CalcArgs a)
         nn = group_sum<NT>(nn, tmp, lane);
         if (last)
             break;
-        if (dn <= 1e-20 * nn)
-            last = true;   // converged quadratically: one more unscaled step
+        // a relative change of 1e-8: the iteration converges quadratically,
+        // so one more unscaled step leaves Z within rounding of its limit
+        // (1e-10: one more iteration of the pass, about the same T)
+        if (dn <= 1e-16 * nn)
+            last = true;
 #pragma unroll
         for (int r = 0; r < BS; ++r)
 #pragma unroll
