@@ -176,6 +176,8 @@ class Engine:
         self.nmem = nmem
         self.md = nmem if nmem_dynamic is None else nmem_dynamic
         self.mode = mode
+        self.scheme, self.alpha, self.gamma = scheme, alpha, gamma
+        self.keep_fp64 = keep_fp64
         cfg = Config(mode, scheme, precision, nmem, self.md, alpha, gamma, device,
                      (FLAG_PERTYPE_DIAG if pertype_diag else 0) | (FLAG_KEEP_FP64 if keep_fp64 else 0)
                      | (FLAG_GRAPH if graph else 0))

@@ -9,8 +9,11 @@
  * values, big-endian, at the offset ("begin") its header entry records,
  * padded to 4 bytes.  Version 2 stores begin as a 64-bit offset, so files
  * may exceed 2 GiB; only the last variable may be larger than 4 GiB (its
- * vsize is then written as 2^32 - 1).  No record (unlimited) dimension is
- * written; the reader accepts a header with numrecs 0 only.
+ * vsize is then written as 2^32 - 1).  A dimension defined with length 0 is
+ * the record (unlimited) dimension, written with no records (numrecs 0):
+ * the reference's files have one where a count is zero (a point log without
+ * local obs, common/pointlog.c:115).  Its variables, which must have it
+ * first, hold no values; their begin is the end of the fixed data.
  *
  * Layouts of the path's files follow the reference writers:
  * calc/transforms.c:54-135 (transforms.nc), 485-520 (enkf_diag.nc),
@@ -210,7 +213,7 @@ static int put_atts(buf* b, const ncatt* a, int n)
 static int build_header(ncfile* f, buf* b, uint64_t data0)
 {
     int k, d, r;
-    uint64_t off = data0;
+    uint64_t off = data0, recoff = data0;
 
     b->n = 0;
     if ((r = buf_put(b, "CDF\002", 4)) || (r = put_u32(b, 0)))
@@ -234,6 +237,10 @@ static int build_header(ncfile* f, buf* b, uint64_t data0)
     }
     if ((r = put_u32(b, TAG_VARIABLE)) || (r = put_u32(b, (uint32_t) f->nvars)))
         return r;
+    /* the records follow every fixed variable's values */
+    for (k = 0; k < f->nvars; ++k)
+        if (!f->vars[k].isrec)
+            recoff += pad4(f->vars[k].nbytes);
     for (k = 0; k < f->nvars; ++k) {
         ncvar* v = &f->vars[k];
         uint64_t vsize = pad4(v->nbytes);
@@ -249,10 +256,15 @@ static int build_header(ncfile* f, buf* b, uint64_t data0)
             return r;
         if ((r = put_u32(b, vsize > 0xFFFFFFFCull ? 0xFFFFFFFFu : (uint32_t) vsize)))
             return r;
-        v->begin = off;
-        if ((r = put_u64(b, off)))
+        if (v->isrec) {
+            v->begin = recoff;
+            recoff += vsize;
+        } else {
+            v->begin = off;
+            off += vsize;
+        }
+        if ((r = put_u64(b, v->begin)))
             return r;
-        off += vsize;
     }
     return ENKFIO_OK;
 }
@@ -290,7 +302,7 @@ int ncf_def_dim(ncfile* f, const char* name, size_t len)
 {
     ncdim* d;
 
-    if (f == NULL || !f->defmode || !valid_name(name) || len == 0 || len > 0xFFFFFFFFu)
+    if (f == NULL || !f->defmode || !valid_name(name) || len > 0xFFFFFFFFu || (len == 0 && f->recdim >= 0))
         return ENKFIO_EARG;
     d = realloc(f->dims, (f->ndims + 1) * sizeof(ncdim));
     if (d == NULL)
@@ -298,6 +310,8 @@ int ncf_def_dim(ncfile* f, const char* name, size_t len)
     f->dims = d;
     d[f->ndims].name = strdup(name);
     d[f->ndims].len = len;
+    if (len == 0)
+        f->recdim = f->ndims;
     return f->ndims++;
 }
 
@@ -311,9 +325,10 @@ int ncf_def_var(ncfile* f, const char* name, int type, int ndims, const int* dim
         ndims > ENKFIO_MAXDIMS || (ndims > 0 && dimids == NULL))
         return ENKFIO_EARG;
     for (k = 0; k < ndims; ++k) {
-        if (dimids[k] < 0 || dimids[k] >= f->ndims)
+        if (dimids[k] < 0 || dimids[k] >= f->ndims || (k > 0 && dimids[k] == f->recdim))
             return ENKFIO_EARG;
-        n *= f->dims[dimids[k]].len;
+        if (k > 0 || dimids[k] != f->recdim)
+            n *= f->dims[dimids[k]].len;
     }
     v = realloc(f->vars, (f->nvars + 1) * sizeof(ncvar));
     if (v == NULL)
@@ -326,6 +341,8 @@ int ncf_def_var(ncfile* f, const char* name, int type, int ndims, const int* dim
     v->ndims = ndims;
     for (k = 0; k < ndims; ++k)
         v->dimids[k] = dimids[k];
+    v->isrec = ndims > 0 && dimids[0] == f->recdim;
+    /* a record variable: the size of one record (the header's vsize) */
     v->nbytes = n * type_size(type);
     return f->nvars++;
 }
@@ -381,11 +398,15 @@ int ncf_enddef(ncfile* f)
         r = build_header(f, &b, b.n);
     if (r == ENKFIO_OK && fwrite(b.p, 1, b.n, f->fp) != b.n)
         r = ENKFIO_EIO;
-    /* extend the file to its full size: every variable's values exist */
+    /* extend the file to its full size: every fixed variable's values exist
+       (record variables have no records) */
     if (r == ENKFIO_OK && f->nvars > 0) {
-        const ncvar* v = &f->vars[f->nvars - 1];
+        int k;
 
-        end = v->begin + pad4(v->nbytes);
+        end = 0;
+        for (k = 0; k < f->nvars; ++k)
+            if (!f->vars[k].isrec && f->vars[k].begin + pad4(f->vars[k].nbytes) > end)
+                end = f->vars[k].begin + pad4(f->vars[k].nbytes);
         if (end > b.n && (fseeko(f->fp, (off_t) (end - 1), SEEK_SET) != 0 || fputc(0, f->fp) == EOF))
             r = ENKFIO_EIO;
     }
