@@ -162,6 +162,9 @@ struct enkfg_ctx {
     unsigned char* scratch = nullptr;
     // ensemble observations: H operator tables, obs records, spread
     DevBuf d_hg_tab, d_hg_obs, d_hg_std;
+    double kfactor = NAN;            // das_moderateobs; NaN = off
+    DevBuf d_kmod;
+    std::vector<int> kmod;           // per type, from the last moderation
     // adjoint
     DevBuf d_adjg, d_adjflag, d_adjlist, d_adjz, d_adjw, d_gy, d_gR, d_adjD, d_adjinfl, d_minv, d_adjB;
     float* attached_E = nullptr;
@@ -622,6 +625,7 @@ int enkfg_set_obs(enkfg_ctx* c, int nobs, const short* type, const short* status
     c->olon.assign(lon, lon + nobs);
     c->olat.assign(lat, lat + nobs);
     c->oestd.assign(estd, estd + nobs);
+    c->kmod.clear();
 
     // trees (obs_createkdtrees)
     std::vector<BuildNode> nodes;
@@ -974,12 +978,51 @@ int enkfg_calc_HE(enkfg_ctx* c, const enkfg_grid* g, int ntypes, const enkfg_hop
     if (std_out != nullptr)
         HIPCHECK(c, hipMemcpyAsync(std_out, c->d_hg_std.p, (size_t)nobs * sizeof(double), hipMemcpyDeviceToHost,
                                    c->stream));
+    // das_moderateobs (enkf_calc.c:387 runs it when KFACTOR is finite):
+    // the engine's obs errors become the moderated ones before S is
+    // standardised with them
+    if (std::isfinite(c->kfactor)) {
+        HIPCHECK(c, c->d_kmod.reserve(std::max(nt, 1) * sizeof(int)));
+        HIPCHECK(c, hipMemsetAsync(c->d_kmod.p, 0, std::max(nt, 1) * sizeof(int), c->stream));
+        HIPCHECK(c, launch_moderate(c->stream, nobs, c->kfactor, d_st, d_ty, c->d_hg_std.as<double>(),
+                                    c->d_s.as<double>(), c->d_estd.as<float>(), c->d_kmod.as<int>()));
+        c->kmod.assign(std::max(nt, 1), 0);
+        HIPCHECK(c, hipMemcpyAsync(c->kmod.data(), c->d_kmod.p, std::max(nt, 1) * sizeof(int), hipMemcpyDeviceToHost,
+                                   c->stream));
+        HIPCHECK(c, hipMemcpyAsync(c->oestd.data(), c->d_estd.p, (size_t)nobs * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
     // the engine's S and s, as enkfg_set_S(HA, s, standardised = 0) leaves them
     HIPCHECK(c, launch_he_clear_bad(c->stream, d_st, nobs, m, c->d_S.as<float>(), c->d_s.as<double>()));
     int r = finish_S(c, false);
     if (r != ENKFG_OK)
         return r;
     c->S_set = true;
+    return ENKFG_OK;
+}
+
+int enkfg_set_kfactor(enkfg_ctx* c, double kfactor)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!(kfactor > 0.0) && !std::isnan(kfactor))
+        return fail(c, ENKFG_ERR_ARG, "KFACTOR must be positive (or NaN: off)");
+    c->kfactor = kfactor;
+    return ENKFG_OK;
+}
+
+int enkfg_get_obs_estd(enkfg_ctx* c, float* estd, int* nmodified)
+{
+    if (check_ctx(c))
+        return ENKFG_ERR_ARG;
+    if (!c->obs_set)
+        return fail(c, ENKFG_ERR_STATE, "observations not set");
+    if (estd != nullptr && c->nobs > 0)
+        std::copy(c->oestd.begin(), c->oestd.end(), estd);
+    if (nmodified != nullptr)
+        for (int t = 0; t < (int)c->types.size(); ++t)
+            nmodified[t] = t < (int)c->kmod.size() ? c->kmod[t] : 0;
     return ENKFG_OK;
 }
 

@@ -213,6 +213,8 @@ struct HeGetArgs {
 hipError_t launch_he(hipStream_t s, const HeGetArgs& a);
 // rows of HA and entries of s of the obs that are not OK set to 0
 hipError_t launch_he_clear_bad(hipStream_t s, const short* status, int nobs, int m, float* HA, double* sv);
+hipError_t launch_moderate(hipStream_t s, int nobs, double kfactor, const short* status, const short* type,
+                           const double* std, const double* inn, float* estd, int* nmod);
 
 // adjoint of the analysis mean (adjoint.hip)
 // g_nodes[node][m] += sum over cells of beta(cell, node) sum_f dJ/dmean[f][cell] A^f[f][cell][:]

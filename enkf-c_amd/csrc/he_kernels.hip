@@ -240,4 +240,38 @@ hipError_t launch_he_clear_bad(hipStream_t s, const short* status, int nobs, int
     return hipGetLastError();
 }
 
+// das_moderateobs (calc/ensobs.c:749-799): the K-factor moderation of the
+// error of every good observation from its forecast spread and innovation,
+//   estd' = sqrt(sqrt((svar + ovar)^2 + svar inn^2 / K / K) - svar),
+// in double with every operation rounded separately (no contraction into
+// fma, as the reference's host build evaluates it), stored as float.  The
+// obs whose error variance more than doubles are counted per type.
+__global__ void moderate_kernel(int nobs, double kfactor, const short* status, const short* type, const double* std,
+                                const double* inn, float* estd, int* nmod)
+{
+#pragma clang fp contract(off)
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nobs || status[o] != ENKFG_STATUS_OK)
+        return;
+    const double svar = std[o] * std[o];
+    const double ov = (double)estd[o];
+    const double ovar = ov * ov;
+    const double x = inn[o];
+    const double sum = svar + ovar;
+    const double e = sqrt(sqrt(sum * sum + svar * x * x / kfactor / kfactor) - svar);
+    if (svar > 0.0 && e * e / ovar > 2.0)
+        atomicAdd(&nmod[type[o]], 1);
+    estd[o] = (float)e;
+}
+
+hipError_t launch_moderate(hipStream_t s, int nobs, double kfactor, const short* status, const short* type,
+                           const double* std, const double* inn, float* estd, int* nmod)
+{
+    if (nobs == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(moderate_kernel, dim3((unsigned)((nobs + 255) / 256)), dim3(256), 0, s, nobs, kfactor, status,
+                       type, std, inn, estd, nmod);
+    return hipGetLastError();
+}
+
 }  // namespace enkfg

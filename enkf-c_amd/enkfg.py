@@ -28,7 +28,7 @@ NLOBSMAX_NONE = 2147483647
 MAXLOCRAD = 8
 
 EXPORTED = [
-    "enkfg_create", "enkfg_destroy", "enkfg_last_error", "enkfg_sync", "enkfg_set_obstypes", "enkfg_set_obs",
+    "enkfg_create", "enkfg_set_kfactor", "enkfg_get_obs_estd", "enkfg_destroy", "enkfg_last_error", "enkfg_sync", "enkfg_set_obstypes", "enkfg_set_obs",
     "enkfg_set_obs_status", "enkfg_set_S", "enkfg_get_S", "enkfg_calc_transforms", "enkfg_get_transforms",
     "enkfg_get_diag", "enkfg_get_stats", "enkfg_localise", "enkfg_get_local_counts", "enkfg_get_localobs",
     "enkfg_set_transforms", "enkfg_update_fields", "enkfg_update_bg", "enkfg_ens_attach", "enkfg_ens_update",
@@ -129,6 +129,8 @@ def lib():
         L.enkfg_set_rand48.argtypes = [vp, C.c_ulonglong]
         L.enkfg_get_rand48.argtypes = [vp, C.POINTER(C.c_ulonglong)]
         L.enkfg_rand48_normals.argtypes = [C.POINTER(C.c_ulonglong), C.c_int, vp]
+        L.enkfg_set_kfactor.argtypes = [vp, C.c_double]
+        L.enkfg_get_obs_estd.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -253,6 +255,18 @@ class Engine:
         self._check(lib().enkfg_calc_HE(self.h, C.byref(g), len(hops), arr, _p(fi), _p(fj), _p(fk), _p(value),
                                         _p(HA), _p(s), _p(sd)))
         return HA, s, sd
+
+    def set_kfactor(self, kfactor):
+        """das_moderateobs in the next calc_HE (NaN: off)"""
+        self._check(lib().enkfg_set_kfactor(self.h, C.c_double(kfactor)))
+
+    def obs_estd(self):
+        """the engine's obs errors and the per-type counts of obs whose error
+        variance the moderation more than doubled"""
+        e = np.zeros(max(self.nobs, 1), np.float32)
+        n = np.zeros(max(self.ntypes, 1), np.int32)
+        self._check(lib().enkfg_get_obs_estd(self.h, _p(e), _p(n)))
+        return e[:self.nobs], n[:self.ntypes]
 
     def get_S(self):
         S = np.zeros((self.nmem, self.nobs), np.float32)

@@ -203,6 +203,16 @@ int enkfg_calc_HE(enkfg_ctx* ctx, const enkfg_grid* grid, int ntypes, const enkf
 /* The standardised S and s_f (the reference standardises in place; callers
  * such as das_updateHE read them back). */
 int enkfg_get_S(enkfg_ctx* ctx, float* S, double* s);
+/* das_moderateobs (calc/ensobs.c:749-799; KFACTOR, enkfprm.c): with a
+ * finite K, enkfg_calc_HE moderates the error of every good observation from
+ * its forecast spread and innovation before S is standardised.  NaN (the
+ * default) turns it off. */
+int enkfg_set_kfactor(enkfg_ctx* ctx, double kfactor);
+/* the engine's obs errors (moderated ones after a moderating enkfg_calc_HE)
+ * [nobs], and per obs type the count of obs whose error variance more than
+ * doubled (obstype nmodified) [ntypes]; either may be NULL */
+int enkfg_get_obs_estd(enkfg_ctx* ctx, float* estd, int* nmodified);
+
 
 /* ---------------------------------------------------------------- calc */
 /* das_calctransforms for one grid: local search, gather and taper, per-column

@@ -296,6 +296,31 @@ class OracleObs:
             pass
 
 
+def moderateobs(kfactor, ntypes, type_, status, estd, std, inn):
+    """das_moderateobs (calc/ensobs.c:749-799): for every obs with status OK,
+    estd' = sqrt(sqrt((svar + ovar)^2 + svar inn^2 / K / K) - svar) in double,
+    left to right with every operation rounded (numpy does not contract),
+    stored as float; counts per type the obs with svar > 0 whose error
+    variance more than doubled.  NaN K: nothing changes (ensobs.c:758-759)."""
+    e32 = np.array(estd, np.float32)
+    cnt = np.zeros(ntypes, np.int64)
+    if not np.isfinite(kfactor):
+        return e32, cnt
+    std = np.asarray(std, np.float64)
+    inn = np.asarray(inn, np.float64)
+    ov = e32.astype(np.float64)
+    svar = std * std
+    ovar = ov * ov
+    sm = svar + ovar
+    with np.errstate(invalid="ignore"):
+        new = np.sqrt(np.sqrt(sm * sm + svar * inn * inn / kfactor / kfactor) - svar)
+    ok = np.asarray(status) == 0
+    mod = ok & (svar > 0.0) & (new * new / ovar > 2.0)
+    np.add.at(cnt, np.asarray(type_)[mod], 1)
+    e32[ok] = new[ok].astype(np.float32)
+    return e32, cnt
+
+
 def standardise(nmem, obs, S, s):
     S = np.ascontiguousarray(S, np.float32).copy()
     s = np.ascontiguousarray(s, np.float64).copy()

@@ -118,3 +118,38 @@ def test_calc_HE_reports_a_bad_forecast():
         E.calc_HE(P.nlevels, 0, 0, o["hops"], o["fi"], o["fj"], o["fk"], o["value"])
     assert ei.value.code == enkfg.ERR_NONFINITE
     E.close()
+
+
+@pytest.mark.parametrize("kfactor", [0.5, 2.0])
+def test_calc_HE_kfactor_moderation(kfactor):
+    """das_moderateobs on the device (calc/ensobs.c:749-799): the moderated
+    obs errors equal the oracle's bit for bit, the per-type counts of obs whose
+    error variance more than doubled are the oracle's, and S and s are
+    standardised with the moderated errors."""
+    print("case calc_HE kfactor=%g" % kfactor, flush=True)
+    m = 24
+    P, fields, fl, o = problem(31, m, 0.1)
+    HA_o, s_o, sd_o = O.getHE(enkfg.MODE_ENKF, m, m, 1.0, P.nlevels, 0, 0, fields, o["hops"], o["typ"], o["status"],
+                              o["fi"], o["fj"], o["fk"], o["value"])
+    nt = len(o["types"])
+    e_o, cnt_o = O.moderateobs(kfactor, nt, o["typ"], o["status"], o["estd"], sd_o, s_o)
+    assert cnt_o.sum() > 0, "the case must moderate some obs substantially"
+    E = enkfg.Engine(m, mode=enkfg.MODE_ENKF, scheme=enkfg.SCHEME_DENKF, keep_fp64=True)
+    E.set_obstypes(o["types"])
+    E.set_obs(o["typ"], o["status"], o["lon"], o["lat"], o["estd"])
+    E.set_kfactor(kfactor)
+    import synth_gpu
+    Ecell = np.stack([f.reshape(m, -1).T for f in fields]).astype(np.float32)
+    dE = synth_gpu.DeviceArray.from_host(Ecell)
+    E.ens_attach(P.nlevels, P.stride, 0, 0, fl, dE.ptr)
+    E.calc_HE(P.nlevels, 0, 0, o["hops"], o["fi"], o["fj"], o["fk"], o["value"])
+    e, cnt = E.obs_estd()
+    assert np.array_equal(e.view(np.uint32), e_o.view(np.uint32))
+    assert np.array_equal(cnt, cnt_o)
+    print("  %d obs moderated substantially, per type %s" % (int(cnt.sum()), list(cnt)), flush=True)
+    ok = o["status"] == 0
+    ob = O.OracleObs(o["types"], o["typ"], o["lon"], o["lat"], e_o)
+    S_ref, s_ref = O.standardise(m, ob, np.where(ok[None, :], HA_o, 0.0).astype(np.float32), np.where(ok, s_o, 0.0))
+    Sg, sg = E.get_S()
+    assert np.array_equal(Sg, S_ref) and np.array_equal(sg, s_ref)
+    E.close()

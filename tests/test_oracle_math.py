@@ -685,3 +685,31 @@ def test_spread_population_std_float_square_and_limits():
     assert np.all(O.spread((z * 1e5 + 2e6).astype(np.float32)) == 0.0)
     assert np.all(O.spread((z * 1e5 - 2e6).astype(np.float32)) == 0.0)
     assert np.all(O.spread((z * 1e5 + 4e5).astype(np.float32)) > 1e4)
+
+
+def test_moderateobs_closed_forms():
+    """das_moderateobs (calc/ensobs.c:749-799) pinned by what the formula
+    fixes: no innovation or no spread leaves the error unchanged (exactly,
+    for dyadic values), the known answer std 3, estd 4, inn = K sqrt(59) gives
+    5, large innovations give estd'^2 -> std |inn| / K, bad obs and a NaN K
+    change nothing, and only obs whose error variance more than doubles are
+    counted, per type."""
+    K = 2.0
+    typ = np.array([0, 0, 1, 1, 1, 0, 1])
+    st = np.array([0, 0, 0, 0, 0, 7, 0])
+    estd = np.array([0.25, 0.5, 4.0, 1.0, 0.75, 0.5, 0.125], np.float32)
+    std = np.array([0.5, 0.0, 3.0, 1e-3, 1.5, 2.0, 0.5])
+    inn = np.array([0.0, 5.0, K * np.sqrt(59.0), 1e8, 40.0, 9.0, 0.0])
+    e, cnt = O.moderateobs(K, 2, typ, st, estd, std, inn)
+    assert e[0] == estd[0] and e[1] == estd[1] and e[6] == estd[6]        # no innovation / no spread
+    assert abs(e[2] - 5.0) < 1e-6                                        # sqrt(34 - 9)
+    assert abs(e[3] ** 2 / (1e-3 * 1e8 / K) - 1.0) < 1e-2                # the large-innovation limit
+    assert e[5] == estd[5]                                               # bad status
+    big = [k for k in range(7) if st[k] == 0 and std[k] > 0 and float(e[k]) ** 2 / float(estd[k]) ** 2 > 2.0]
+    assert sorted(big) == [3, 4]
+    assert list(cnt) == [0, 2]
+    # a larger K moderates less; NaN turns it off
+    e4, _ = O.moderateobs(4.0, 2, typ, st, estd, std, inn)
+    assert (e4 <= e).all() and (e4 >= estd - 1e-7).all()
+    en, cn = O.moderateobs(float("nan"), 2, typ, st, estd, std, inn)
+    assert np.array_equal(en, estd) and not cn.any()
